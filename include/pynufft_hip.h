@@ -1,0 +1,149 @@
+/*
+ * pynufft_hip.h -- C ABI of the MI355X min-max NUFFT library (libpynufft_hip.so).
+ *
+ * The reference (pynufft) is pure Python and has no FFI of its own; each
+ * entry point below replaces one run-time stage of the reference's NUFFT
+ * class and is what a ctypes binding of that stage calls.  Citations are to
+ * the reference checkout (pynufft/pynufft).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Every buffer is device memory owned by
+ *    the caller (PyTorch tensors in pynufft_amd); the library allocates no
+ *    device memory.  Scratch space is passed in as `ws` and sized by
+ *    pnh_workspace_bytes().
+ *  - Complex data are interleaved (re, im): complex64 = 2 x float,
+ *    complex128 = 2 x double.  Arrays are C order with the coil (batch)
+ *    index fastest: image Nd x nb, grid Kd x nb, samples M x nb.
+ *  - Every call is asynchronous on `stream` (a hipStream_t; NULL = default).
+ *  - Return value: 0 on success, < 0 on error (-1 bad argument, -2
+ *    unsupported configuration, -3 HIP error); pnh_last_error() gives the
+ *    message of the calling thread's last failure.  Nothing throws across
+ *    the ABI.
+ *  - Distinct plans may be used from different threads concurrently.  Calls
+ *    on one plan that share a workspace must be stream-ordered.
+ */
+#ifndef PYNUFFT_HIP_H
+#define PYNUFFT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PNH_MAXD 3
+
+enum pnh_dtype { PNH_C64 = 0, PNH_C128 = 1 };
+
+/* Operators whose scratch needs are reported by pnh_workspace_bytes(). */
+enum pnh_op {
+  PNH_OP_FORWARD = 0,
+  PNH_OP_ADJOINT = 1,
+  PNH_OP_SELFADJOINT = 2,
+  PNH_OP_Y2K = 3,
+  PNH_OP_K2Y2K = 4,
+  PNH_OP_SOLVE_CG = 5
+};
+
+/*
+ * Plan descriptor: geometry plus device pointers to the interpolator built on
+ * the host in FP64 (reference src/_helper/helper.py:620-802, plan()).  Samples
+ * are held in a trajectory-sorted row order; row r is sample perm[r].
+ */
+typedef struct pnh_plan_desc {
+  int32_t ndim;              /* 1..3 */
+  int32_t batch;             /* coils sharing the trajectory (>= 1) */
+  int64_t M;                 /* number of samples */
+  int64_t Nd[PNH_MAXD];      /* image shape */
+  int64_t Kd[PNH_MAXD];      /* oversampled grid shape */
+  int32_t Jd[PNH_MAXD];      /* interpolator size per dim */
+
+  const int32_t* perm;       /* [M] sorted row -> sample index */
+  const int32_t* base;       /* [ndim][M] first support cell per dim (row order) */
+  const void* u_c64;         /* complex64 per-dim factors u_d[row][j], concatenated */
+  const void* u_c128;        /* complex128 factors (FP64 operators), may be NULL */
+  int64_t u_off[PNH_MAXD];   /* element offset of dim d's block */
+  const void* csr_c64;       /* complex64 full coefficients, slices of 64 rows
+                                interleaved: [(M+63)/64][prod Jd][64]; may be NULL */
+  const int32_t* csr_col;    /* int32 grid cell of each entry, same layout */
+
+  const float* sn_f32;       /* [prod Nd] apodization (reference st['sn']) */
+  const double* sn_f64;      /* [prod Nd] FP64 apodization, may be NULL */
+  const void* tw_c64[PNH_MAXD];   /* exp(-2 pi i t / Kd[d]), t < Kd[d] */
+  const void* tw_c128[PNH_MAXD];  /* same in complex128, may be NULL */
+
+  int32_t tile[PNH_MAXD];    /* gridding tile extent per dim */
+  int64_t n_items;           /* gridding work items */
+  const int32_t* items;      /* [n_items][4]: tile id, entry start, entry count,
+                                partial slot (-1 = writes the grid directly) */
+  const int32_t* bin_rows;   /* work item entries (sorted row ids) */
+  int64_t n_merge;           /* split tiles */
+  const int32_t* merge;      /* [n_merge][3]: tile id, first slot, slot count */
+  int64_t n_slots;           /* partial tile slots used by split items */
+} pnh_plan_desc;
+
+typedef struct pnh_plan pnh_plan;
+
+/* Library version string. */
+const char* pnh_version(void);
+/* Message of the calling thread's last error ("" if none). */
+const char* pnh_last_error(void);
+/* Number of visible HIP devices, or < 0 on error.
+ * Replaces reference src/_helper/helper.py:1172-1224 (device_list discovery). */
+int pnh_device_count(void);
+
+/* Validate a descriptor and build the per-axis FFT passes.  The descriptor's
+ * pointers must stay valid for the plan's lifetime.
+ * Replaces the offload step of reference nufft/_nufft_class_methods_device.py:183-254. */
+int pnh_plan_create(const pnh_plan_desc* desc, pnh_plan** out);
+int pnh_plan_destroy(pnh_plan* plan);
+/* Bytes of `ws` needed by operator `op` in precision `dtype`. */
+int pnh_workspace_bytes(const pnh_plan* plan, int op, int dtype, size_t* bytes);
+
+/* xx = x * sn.  Reference nufft/_nufft_class_methods_cpu.py:224-230 (_x2xx_cpu). */
+int pnh_x2xx(const pnh_plan* plan, const void* x, void* xx, void* stream);
+/* x = xx * sn.  Reference nufft/_nufft_class_methods_cpu.py:346-351 (_xx2x_cpu). */
+int pnh_xx2x(const pnh_plan* plan, const void* xx, void* x, void* stream);
+/* k = fftn(zero_pad(xx)) over all axes (front-corner pad), complex64.
+ * Reference nufft/_nufft_class_methods_cpu.py:232-249 (_xx2k_cpu). */
+int pnh_xx2k(const pnh_plan* plan, const void* xx, void* k, void* stream);
+/* xx = crop(ifftn(k)); k is overwritten (transformed in place, as
+ * reference nufft/_nufft_class_methods_device.py:432 does).
+ * Reference nufft/_nufft_class_methods_cpu.py:316-329 (_k2xx_cpu). */
+int pnh_k2xx(const pnh_plan* plan, void* k, void* xx, void* stream);
+/* y = P k.  dtype selects complex64 (fp32) or complex128 (FP64) operands.
+ * Reference nufft/_nufft_class_methods_cpu.py:269-288 (_k2y_cpu). */
+int pnh_k2y(const pnh_plan* plan, int dtype, const void* k, void* y, void* stream);
+/* k = P^H y (deterministic binned gridding).
+ * Reference nufft/_nufft_class_methods_cpu.py:290-314 (_y2k_cpu). */
+int pnh_y2k(const pnh_plan* plan, int dtype, const void* y, void* k, void* ws,
+            void* stream);
+/* k_out = P^H P k.  Reference nufft/_nufft_class_methods_cpu.py:353-362 (_k2y2k_cpu). */
+int pnh_k2y2k(const pnh_plan* plan, int dtype, const void* k, void* k_out,
+              void* ws, void* stream);
+/* y = k2y(xx2k(x2xx(x))), complex64.
+ * Reference nufft/_nufft_class_methods_cpu.py:168-179 (_forward_cpu). */
+int pnh_forward(const pnh_plan* plan, const void* x, void* y, void* ws, void* stream);
+/* x = xx2x(k2xx(y2k(y))), complex64.
+ * Reference nufft/_nufft_class_methods_cpu.py:181-193 (_adjoint_cpu). */
+int pnh_adjoint(const pnh_plan* plan, const void* y, void* x, void* ws, void* stream);
+/* x_out = xx2x(k2xx(k2y2k(xx2k(x2xx(x))))), complex64.
+ * Reference nufft/_nufft_class_methods_cpu.py:201-214 (_selfadjoint_cpu). */
+int pnh_selfadjoint(const pnh_plan* plan, const void* x, void* x_out, void* ws,
+                    void* stream);
+/* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
+ * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
+ * start of an iteration or after maxiter iterations; result
+ * x = crop(ifftn(k)) / sn cast to complex64.  The loop runs on the device with
+ * no host synchronisation; iters_out (device int32[batch]) receives the
+ * iteration count per coil.
+ * Reference linalg/solve_cpu.py:256-288 with scipy.sparse.linalg.cg. */
+int pnh_solve_cg(const pnh_plan* plan, const void* y, void* x, int maxiter,
+                 double rtol, double atol, void* ws, int32_t* iters_out,
+                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYNUFFT_HIP_H */

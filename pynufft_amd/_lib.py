@@ -1,0 +1,117 @@
+"""ctypes binding of libpynufft_hip.so (C ABI declared in include/pynufft_hip.h).
+
+torch is imported first so that the library binds to the HIP runtime torch
+already loaded (both carry the soname libamdhip64.so.7).  There is no CPU
+fallback: if the library is missing, every operator raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libpynufft_hip.so')
+
+MAXD = 3
+C64, C128 = 0, 1
+OP_FORWARD, OP_ADJOINT, OP_SELFADJOINT, OP_Y2K, OP_K2Y2K, OP_SOLVE_CG = range(6)
+
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ('ndim', ctypes.c_int32),
+        ('batch', ctypes.c_int32),
+        ('M', ctypes.c_int64),
+        ('Nd', ctypes.c_int64 * MAXD),
+        ('Kd', ctypes.c_int64 * MAXD),
+        ('Jd', ctypes.c_int32 * MAXD),
+        ('perm', _vp),
+        ('base', _vp),
+        ('u_c64', _vp),
+        ('u_c128', _vp),
+        ('u_off', ctypes.c_int64 * MAXD),
+        ('csr_c64', _vp),
+        ('csr_col', _vp),
+        ('sn_f32', _vp),
+        ('sn_f64', _vp),
+        ('tw_c64', _vp * MAXD),
+        ('tw_c128', _vp * MAXD),
+        ('tile', ctypes.c_int32 * MAXD),
+        ('n_items', ctypes.c_int64),
+        ('items', _vp),
+        ('bin_rows', _vp),
+        ('n_merge', ctypes.c_int64),
+        ('merge', _vp),
+        ('n_slots', ctypes.c_int64),
+    ]
+
+
+class HipError(RuntimeError):
+    """A HIP runtime failure reported by the library."""
+
+
+_lib = None
+
+EXPORTS = {
+    'pnh_version': (ctypes.c_char_p, []),
+    'pnh_last_error': (ctypes.c_char_p, []),
+    'pnh_device_count': (ctypes.c_int, []),
+    'pnh_plan_create': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)]),
+    'pnh_plan_destroy': (ctypes.c_int, [_vp]),
+    'pnh_workspace_bytes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_x2xx': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    'pnh_xx2x': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    'pnh_xx2k': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    'pnh_k2xx': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    'pnh_k2y': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    'pnh_y2k': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    'pnh_k2y2k': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    'pnh_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_selfadjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, _vp, _vp, _vp]),
+}
+
+
+def load(path=LIB_PATH):
+    """Load the library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError('libpynufft_hip.so not found at %s; build it with '
+                          '`python -m pynufft_amd.build`' % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc == 0:
+        return
+    msg = load().pnh_last_error().decode(errors='replace')
+    if rc == -1:
+        raise ValueError(msg)
+    if rc == -2:
+        raise NotImplementedError(msg)
+    raise HipError(msg)
+
+
+def device_count():
+    n = load().pnh_device_count()
+    if n < 0:
+        check(n)
+    return n
+
+
+def version():
+    return load().pnh_version().decode()

@@ -1,0 +1,338 @@
+// LDS-resident Stockham FFT over lines of a strided N-d array (gfx950).
+//
+// One launch transforms every line along one axis.  A workgroup loads LB lines
+// into LDS (zero-filling past n_in, so zero-padding is fused), runs all radix
+// passes in LDS (ping-pong buffers, radices 8/4/2/3/5 and a generic small odd
+// radix), and writes the first n_out outputs of each line (so cropping is
+// fused).  The real apodization factor sn and a scalar may be applied on the
+// load side (forward: x2xx fused) or on the store side (adjoint: xx2x fused).
+//
+// Semantics follow numpy.fft: the forward kernel is exp(-2 pi i t/n), the
+// inverse exp(+2 pi i t/n) with the 1/n factors folded into `scale`.
+#include "common.h"
+#include "fft.h"
+
+namespace {
+
+template <typename C, typename T>
+__device__ __forceinline__ C mul_neg_i(C a, int sign) {
+  // multiply by (sign * i): sign=-1 -> -i, sign=+1 -> +i
+  C z;
+  if (sign < 0) { z.x = a.y; z.y = -a.x; } else { z.x = -a.y; z.y = a.x; }
+  return z;
+}
+
+template <typename T>
+__device__ __forceinline__ void dft2(typename cplx<T>::t* v) {
+  typename cplx<T>::t a = v[0], b = v[1];
+  v[0] = cadd(a, b); v[1] = csub(a, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(typename cplx<T>::t* v, int sign) {
+  typedef typename cplx<T>::t C;
+  C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  C t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+  C t3r = mul_neg_i<C, T>(t3, sign);  // sign*i*(v1-v3)
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3r);
+  v[3] = csub(t1, t3r);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft8(typename cplx<T>::t* v, int sign) {
+  typedef typename cplx<T>::t C;
+  const T h = (T)0.70710678118654752440;
+  // even/odd split: E = DFT4(v0,v2,v4,v6), O = DFT4(v1,v3,v5,v7)
+  C e[4] = {v[0], v[2], v[4], v[6]};
+  C o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<T>(e, sign);
+  dft4<T>(o, sign);
+  // twiddles w8^q = exp(sign*2*pi*i*q/8), q = 0..3
+  C o1, o2, o3;
+  T s = (T)sign;
+  // q=1: (1 + s i) h
+  o1.x = (o[1].x - s * o[1].y) * h;
+  o1.y = (o[1].y + s * o[1].x) * h;
+  // q=2: s i
+  o2 = mul_neg_i<C, T>(o[2], sign);
+  // q=3: (-1 + s i) h
+  o3.x = (-o[3].x - s * o[3].y) * h;
+  o3.y = (-o[3].y + s * o[3].x) * h;
+  v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(typename cplx<T>::t* v, int sign) {
+  typedef typename cplx<T>::t C;
+  const T c = (T)-0.5, s = (T)0.86602540378443864676 * (T)sign;
+  C t1 = cadd(v[1], v[2]);
+  C t2 = csub(v[1], v[2]);
+  C m = cadd(v[0], cscale(t1, c));
+  // s*i*t2
+  C r; r.x = -s * t2.y; r.y = s * t2.x;
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(typename cplx<T>::t* v, int sign) {
+  typedef typename cplx<T>::t C;
+  const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+  const T s1 = (T)0.95105651629515357212 * (T)sign;
+  const T s2 = (T)0.58778525229247312917 * (T)sign;
+  C a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+  C a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+  C m1 = cadd(v[0], cadd(cscale(a1, c1), cscale(a2, c2)));
+  C m2 = cadd(v[0], cadd(cscale(a1, c2), cscale(a2, c1)));
+  // i*(s1 b1 + s2 b2), i*(s2 b1 - s1 b2)
+  C n1 = cadd(cscale(b1, s1), cscale(b2, s2));
+  C n2 = csub(cscale(b1, s2), cscale(b2, s1));
+  C r1; r1.x = -n1.y; r1.y = n1.x;
+  C r2; r2.x = -n2.y; r2.y = n2.x;
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(m1, r1); v[4] = csub(m1, r1);
+  v[2] = cadd(m2, r2); v[3] = csub(m2, r2);
+}
+
+// Generic small odd radix R via the length-n twiddle table: w_R^k = tw[k*(n/R)].
+template <typename T, int R>
+__device__ __forceinline__ void dftg(typename cplx<T>::t* v, int n,
+                                     const typename cplx<T>::t* tw, int sign) {
+  typedef typename cplx<T>::t C;
+  C out[R];
+  const int step = n / R;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    C acc = v[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      C w = tw[((r * q) % R) * step];
+      if (sign > 0) w.y = -w.y;
+      acc = cadd(acc, cmul(v[r], w));
+    }
+    out[q] = acc;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = out[q];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void dft_r(typename cplx<T>::t* v, int n,
+                                      const typename cplx<T>::t* tw, int sign) {
+  if (R == 2) dft2<T>(v);
+  else if (R == 3) dft3<T>(v, sign);
+  else if (R == 4) dft4<T>(v, sign);
+  else if (R == 5) dft5<T>(v, sign);
+  else if (R == 8) dft8<T>(v, sign);
+  else dftg<T, R>(v, n, tw, sign);
+}
+
+// One Stockham radix-R pass over LB lines held in LDS (src -> dst).
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
+                                              typename cplx<T>::t* dst, int n,
+                                              int ld, int LB, int ns,
+                                              const typename cplx<T>::t* tw,
+                                              int sign, int tid, int nthr) {
+  typedef typename cplx<T>::t C;
+  const int nb = n / R;
+  const int twstep = n / (ns * R);
+  const int total = LB * nb;
+  for (int idx = tid; idx < total; idx += nthr) {
+    const int l = idx / nb;
+    const int j = idx - l * nb;
+    const int k = j % ns;
+    C v[R];
+    const C* s = src + l * ld;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = s[j + r * nb];
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        C w = tw[r * k * twstep];
+        if (sign > 0) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dft_r<T, R>(v, n, tw, sign);
+    C* dd = dst + l * ld + (j / ns) * ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dd[r * ns] = v[r];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_sn(const void* p, int64_t i) {
+  return (T)((const T*)p)[i];
+}
+
+// Offsets of line `l` (flattened over outer x inner) in a buffer.
+__device__ __forceinline__ int64_t line_base(const FftSide& s, int64_t o0,
+                                             int64_t o1, int64_t b) {
+  return o0 * s.o_stride[0] + o1 * s.o_stride[1] + b * s.b_stride;
+}
+
+// POW2: every radix is 2, 4 or 8 (smaller register footprint).
+// in_ and out_ may alias (in-place passes): every input of the block is read
+// into LDS before the first store.
+template <typename T, bool POW2>
+__global__ void __launch_bounds__(256)
+fft_lines_kernel(FftDesc d, const void* in_, void* out_,
+                 const void* __restrict__ tw_, const int* __restrict__ skip) {
+  typedef typename cplx<T>::t C;
+  if (skip && *skip) return;
+  extern __shared__ unsigned char smem_raw[];
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
+  const int n = d.n;
+  const int ld = n + d.lds_pad;           // padded LDS line length
+  C* buf1 = buf0 + (size_t)d.lines_per_block * ld;
+  const C* in = (const C*)in_;
+  C* out = (C*)out_;
+  const C* __restrict__ tw = (const C*)tw_;
+
+  const int LB = d.lines_per_block;
+  const int64_t line0 = (int64_t)blockIdx.x * LB;
+  const int64_t n_lines = d.n_outer * d.n_inner;
+  const int nthr = blockDim.x;
+  const int tid = threadIdx.x;
+
+  // ---- load (with zero padding and optional apodization) ----
+  const int tot_in = LB * n;
+  for (int idx = tid; idx < tot_in; idx += nthr) {
+    int l, t;
+    if (d.inner_major) { t = idx / LB; l = idx - t * LB; }
+    else { l = idx / n; t = idx - l * n; }
+    int64_t L = line0 + l;
+    C v = mk<T>((T)0, (T)0);
+    if (L < n_lines && t < d.n_in) {
+      int64_t o = L / d.n_inner, b = L - o * d.n_inner;
+      int64_t o0 = o / d.o_split, o1 = o - o0 * d.o_split;
+      v = in[line_base(d.src, o0, o1, b) + (int64_t)t * d.src.t_stride];
+      if (d.sn_side == 1) {
+        T s = ld_sn<T>(d.sn, o0 * d.sn_o_stride[0] + o1 * d.sn_o_stride[1] +
+                                 (int64_t)t * d.sn_t_stride);
+        v.x *= s; v.y *= s;
+      }
+    }
+    buf0[l * ld + t] = v;
+  }
+  __syncthreads();
+
+  // ---- Stockham passes in LDS ----
+  C* src = buf0;
+  C* dst = buf1;
+  const int sign = d.sign;
+  for (int p = 0; p < d.npass; ++p) {
+    const int R = d.radix[p];
+    const int ns = d.ns[p];
+    if (POW2) {
+      if (R == 8) stockham_pass<T, 8>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+      else if (R == 4) stockham_pass<T, 4>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+      else stockham_pass<T, 2>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+    } else switch (R) {
+      case 2: stockham_pass<T, 2>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 3: stockham_pass<T, 3>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 4: stockham_pass<T, 4>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 5: stockham_pass<T, 5>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 7: stockham_pass<T, 7>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 8: stockham_pass<T, 8>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 11: stockham_pass<T, 11>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      default: stockham_pass<T, 13>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+    }
+    __syncthreads();
+    C* tmp = src; src = dst; dst = tmp;
+  }
+
+  // ---- store (crop, scale, optional apodization) ----
+  const int tot_out = LB * d.n_out;
+  const T scale = (T)d.scale;
+  for (int idx = tid; idx < tot_out; idx += nthr) {
+    int l, t;
+    if (d.inner_major) { t = idx / LB; l = idx - t * LB; }
+    else { l = idx / d.n_out; t = idx - l * d.n_out; }
+    int64_t L = line0 + l;
+    if (L >= n_lines) continue;
+    int64_t o = L / d.n_inner, b = L - o * d.n_inner;
+    int64_t o0 = o / d.o_split, o1 = o - o0 * d.o_split;
+    C v = src[l * ld + t];
+    T s = scale;
+    if (d.sn_side == 2)
+      s *= ld_sn<T>(d.sn, o0 * d.sn_o_stride[0] + o1 * d.sn_o_stride[1] +
+                              (int64_t)t * d.sn_t_stride);
+    v.x *= s; v.y *= s;
+    out[line_base(d.dst, o0, o1, b) + (int64_t)t * d.dst.t_stride] = v;
+  }
+}
+
+}  // namespace
+
+int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
+                   const void* tw, const int* skip, hipStream_t stream) {
+  const int64_t n_lines = d.n_outer * d.n_inner;
+  if (n_lines <= 0) return 0;
+  const int64_t nblk = (n_lines + d.lines_per_block - 1) / d.lines_per_block;
+  const size_t esz = fp64 ? 16 : 8;
+  const size_t lds = 2 * (size_t)d.lines_per_block * (d.n + d.lds_pad) * esz;
+  dim3 grid((unsigned)nblk), block(d.threads);
+  bool pow2 = true;
+  for (int p = 0; p < d.npass; ++p)
+    pow2 = pow2 && (d.radix[p] == 2 || d.radix[p] == 4 || d.radix[p] == 8);
+  if (fp64) {
+    if (pow2)
+      hipLaunchKernelGGL((fft_lines_kernel<double, true>), grid, block, lds, stream, d,
+                         in, out, tw, skip);
+    else
+      hipLaunchKernelGGL((fft_lines_kernel<double, false>), grid, block, lds, stream, d,
+                         in, out, tw, skip);
+  } else {
+    if (pow2)
+      hipLaunchKernelGGL((fft_lines_kernel<float, true>), grid, block, lds, stream, d,
+                         in, out, tw, skip);
+    else
+      hipLaunchKernelGGL((fft_lines_kernel<float, false>), grid, block, lds, stream, d,
+                         in, out, tw, skip);
+  }
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+// Factor n into radices (8,4,2 first, then 3,5, then generic odd primes).
+int pnh_fft_factor(int n, int* radix, int* ns, int* npass) {
+  int m = n, cnt = 0;
+  int prod = 1;
+  while (m > 1) {
+    int R;
+    if (m % 8 == 0) R = 8;
+    else if (m % 4 == 0) R = 4;
+    else if (m % 2 == 0) R = 2;
+    else if (m % 3 == 0) R = 3;
+    else if (m % 5 == 0) R = 5;
+    else {
+      R = 0;
+      const int generic[3] = {7, 11, 13};
+      for (int q = 0; q < 3; ++q)
+        if (m % generic[q] == 0) { R = generic[q]; break; }
+      if (R == 0) {
+        pnh_set_error("FFT length %d has a prime factor other than 2, 3, 5, 7, 11, 13", n);
+        return -1;
+      }
+    }
+    if (cnt >= PNH_FFT_MAX_PASS) {
+      pnh_set_error("FFT length %d needs too many passes", n);
+      return -1;
+    }
+    radix[cnt] = R;
+    ns[cnt] = prod;
+    prod *= R;
+    m /= R;
+    ++cnt;
+  }
+  *npass = cnt;
+  return 0;
+}

@@ -1,0 +1,40 @@
+// Argument blocks for the interpolation / gridding kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include "common.h"
+
+struct InterpArgs {
+  int D;                       // ndims (1..3)
+  int nb;                      // coils sharing the trajectory
+  int64_t M;                   // samples
+  int K[PNH_MAXD];             // grid size per dim
+  int64_t Kstride[PNH_MAXD];   // cell stride per dim (C order, in cells)
+  int J[PNH_MAXD];             // support per dim
+  const int* perm;             // [M] sorted row -> sample
+  const int* base;             // [D][M] first support cell per dim
+  const void* u;               // per-dim factors (complex, dtype of the op)
+  int64_t u_off[PNH_MAXD];     // element offset of dim d's [M][J_d] block
+  const void* csr_val;         // complex64 slice-interleaved table or NULL
+  const int* csr_col;          // int32 slice-interleaved table or NULL
+};
+
+struct BinArgs {
+  int tile[PNH_MAXD];          // tile extent per dim
+  int tgrid[PNH_MAXD];         // tiles per dim
+  int tile_cells;              // prod(tile)
+  int waves_per_block;
+  size_t lds_budget;
+  int64_t n_items;
+  const int* items;            // [n_items][4]
+  const int* bin_rows;
+  int64_t n_merge;
+  const int* merge;            // [n_merge][3]
+};
+
+int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
+                   const int* skip, hipStream_t s);
+int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
+                   const void* y, void* grid, void* part, const int* skip,
+                   hipStream_t s);

@@ -1,0 +1,35 @@
+// Element-wise and CG vector kernels (host launch wrappers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PNH_RED_BLOCKS 512
+#define PNH_CG_MAXNB 64
+
+enum { CG_PHASE_INIT = 0, CG_PHASE_PQ = 1, CG_PHASE_RR = 2 };
+
+// Device-resident CG state, one entry per coil (independent systems).
+struct CgScalars {
+  double rr[PNH_CG_MAXNB];
+  double rr_prev[PNH_CG_MAXNB];
+  double tol[PNH_CG_MAXNB];
+  double2 pq[PNH_CG_MAXNB];
+  int done[PNH_CG_MAXNB];
+  int iters[PNH_CG_MAXNB];
+  int all_done;
+};
+
+int pnh_scale_sn(int fp64, const void* in, void* out, const void* sn,
+                 int64_t n_img, int nb, hipStream_t s);
+int pnh_c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s);
+int pnh_cast_div_sn(const void* in, void* out, const float* sn, int64_t n_img,
+                    int nb, hipStream_t s);
+int pnh_zero(int fp64, void* p, int64_t n, hipStream_t s);
+int pnh_cg_dot(int mode, const void* a, const void* b, int64_t n_cells, int nb,
+               void* part, const int* skip, hipStream_t s);
+int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter,
+                    int maxiter, double rtol, double atol, hipStream_t s);
+int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
+                    const CgScalars* sc, int first, hipStream_t s);
+int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
+                     int nb, const CgScalars* sc, void* part, hipStream_t s);

@@ -1,0 +1,294 @@
+// Element-wise apodization, dtype casts and the FP64 conjugate-gradient
+// vector kernels.  Reductions are deterministic: a fixed grid of PNH_RED_BLOCKS
+// blocks, each summing a fixed strided subset in a fixed order and reducing
+// in LDS, then one block summing the partials in block order.
+#include "common.h"
+#include "vec.h"
+
+namespace {
+
+template <typename T>
+__global__ void scale_sn_kernel(const void* __restrict__ in_, void* __restrict__ out_,
+                                const T* __restrict__ sn, int64_t n_img, int nb) {
+  typedef typename cplx<T>::t C;
+  const C* in = (const C*)in_;
+  C* out = (C*)out_;
+  const int64_t n = n_img * nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T s = sn[i / nb];
+    C v = in[i];
+    v.x *= s; v.y *= s;
+    out[i] = v;
+  }
+}
+
+__global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __restrict__ out,
+                                   int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = in[i];
+    out[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+// x = complex64(xx) / sn (numpy complex64 division by a real-valued
+// complex64: both parts multiplied by 1/sn in single precision).
+__global__ void cast_div_sn_kernel(const double2* __restrict__ in, float2* __restrict__ out,
+                                   const float* __restrict__ sn, int64_t n_img, int nb) {
+  const int64_t n = n_img * nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float inv = 1.0f / sn[i / nb];
+    const double2 v = in[i];
+    out[i] = make_float2((float)v.x * inv, (float)v.y * inv);
+  }
+}
+
+template <typename C>
+__global__ void zero_kernel(C* __restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    C z; z.x = 0; z.y = 0; p[i] = z;
+  }
+}
+
+// --- deterministic reductions over K cells x nb coils ---------------------
+// mode 0: sum conj(a) b (complex); mode 1: sum |a|^2 (real part only)
+template <int MODE>
+__global__ void __launch_bounds__(256)
+dot_partials_kernel(const double2* __restrict__ a, const double2* __restrict__ b,
+                    int64_t n_cells, int nb, double2* __restrict__ part,
+                    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ double2 red[256];
+  for (int c = 0; c < nb; ++c) {
+    double sx = 0.0, sy = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_cells;
+         g += (int64_t)gridDim.x * blockDim.x) {
+      const double2 u = a[g * nb + c];
+      if (MODE == 0) {
+        const double2 v = b[g * nb + c];
+        sx += u.x * v.x + u.y * v.y;
+        sy += u.x * v.y - u.y * v.x;
+      } else {
+        sx += u.x * u.x + u.y * u.y;
+      }
+    }
+    red[threadIdx.x] = make_double2(sx, sy);
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        red[threadIdx.x].x += red[threadIdx.x + s].x;
+        red[threadIdx.x].y += red[threadIdx.x + s].y;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nb + c] = red[0];
+    __syncthreads();
+  }
+}
+
+// Sum the block partials of coil `c` (one block per coil) in block order
+// using a fixed tree, then apply the CG bookkeeping of `phase`.
+__global__ void __launch_bounds__(256)
+cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
+                   CgScalars* __restrict__ s, int phase, int iter, int maxiter,
+                   double rtol, double atol) {
+  __shared__ double2 red[256];
+  const int c = blockIdx.x;
+  if (phase != CG_PHASE_INIT && s->all_done) return;
+  double sx = 0.0, sy = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    const double2 v = part[(int64_t)i * nb + c];
+    sx += v.x; sy += v.y;
+  }
+  red[threadIdx.x] = make_double2(sx, sy);
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[threadIdx.x].x += red[threadIdx.x + st].x;
+      red[threadIdx.x].y += red[threadIdx.x + st].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double2 tot = red[0];
+  if (phase == CG_PHASE_INIT) {
+    const double bn = sqrt(tot.x);
+    const double tol = fmax(atol, rtol * bn);
+    s->tol[c] = tol;
+    s->rr[c] = tot.x;
+    s->rr_prev[c] = tot.x;
+    const int done = (bn == 0.0) || (bn < tol);
+    s->done[c] = done;
+    s->iters[c] = done ? 0 : maxiter;
+  } else if (phase == CG_PHASE_PQ) {
+    if (!s->done[c]) s->pq[c] = tot;
+  } else {  // CG_PHASE_RR
+    if (!s->done[c]) {
+      s->rr_prev[c] = s->rr[c];
+      s->rr[c] = tot.x;
+      if (sqrt(tot.x) < s->tol[c]) {
+        s->done[c] = 1;
+        s->iters[c] = iter + 1;
+      }
+    }
+  }
+}
+
+// all_done = AND over coils (single thread, runs after every finalize)
+__global__ void cg_all_done_kernel(CgScalars* __restrict__ s, int nb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int all = 1;
+  for (int c = 0; c < nb; ++c) all &= s->done[c];
+  s->all_done = all;
+}
+
+// p = r (first iteration) or p = r + beta p, beta = rr / rr_prev
+__global__ void __launch_bounds__(256)
+cg_p_update_kernel(double2* __restrict__ p, const double2* __restrict__ r,
+                   int64_t n_cells, int nb, const CgScalars* __restrict__ s,
+                   int first) {
+  if (s->all_done) return;
+  const int64_t n = n_cells * nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % nb);
+    if (s->done[c]) continue;
+    const double2 rv = r[i];
+    if (first) { p[i] = rv; continue; }
+    const double beta = s->rr[c] / s->rr_prev[c];
+    double2 pv = p[i];
+    pv.x = rv.x + beta * pv.x;
+    pv.y = rv.y + beta * pv.y;
+    p[i] = pv;
+  }
+}
+
+// alpha = rr / pq; x += alpha p; r -= alpha q; block partial of |r|^2
+__global__ void __launch_bounds__(256)
+cg_xr_update_kernel(double2* __restrict__ x, double2* __restrict__ r,
+                    const double2* __restrict__ p, const double2* __restrict__ q,
+                    int64_t n_cells, int nb, const CgScalars* __restrict__ s,
+                    double2* __restrict__ part) {
+  if (s->all_done) return;
+  __shared__ double2 red[256];
+  for (int c = 0; c < nb; ++c) {
+    double acc = 0.0;
+    if (!s->done[c]) {
+      const double2 pq = s->pq[c];
+      const double rho = s->rr[c];
+      const double den = pq.x * pq.x + pq.y * pq.y;
+      const double ax = rho * pq.x / den, ay = -rho * pq.y / den;
+      for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_cells;
+           g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = g * nb + c;
+        const double2 pv = p[i], qv = q[i];
+        double2 xv = x[i], rv = r[i];
+        xv.x += ax * pv.x - ay * pv.y;
+        xv.y += ax * pv.y + ay * pv.x;
+        rv.x -= ax * qv.x - ay * qv.y;
+        rv.y -= ax * qv.y + ay * qv.x;
+        x[i] = xv;
+        r[i] = rv;
+        acc += rv.x * rv.x + rv.y * rv.y;
+      }
+    }
+    red[threadIdx.x] = make_double2(acc, 0.0);
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x].x += red[threadIdx.x + st].x;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nb + c] = red[0];
+    __syncthreads();
+  }
+}
+
+inline unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+}  // namespace
+
+int pnh_scale_sn(int fp64, const void* in, void* out, const void* sn,
+                 int64_t n_img, int nb, hipStream_t s) {
+  if (fp64)
+    hipLaunchKernelGGL(scale_sn_kernel<double>, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
+                       in, out, (const double*)sn, n_img, nb);
+  else
+    hipLaunchKernelGGL(scale_sn_kernel<float>, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
+                       in, out, (const float*)sn, n_img, nb);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(c64_to_c128_kernel, dim3(grid_for(n)), dim3(256), 0, s,
+                     (const float2*)in, (double2*)out, n);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cast_div_sn(const void* in, void* out, const float* sn, int64_t n_img,
+                    int nb, hipStream_t s) {
+  hipLaunchKernelGGL(cast_div_sn_kernel, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
+                     (const double2*)in, (float2*)out, sn, n_img, nb);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_zero(int fp64, void* p, int64_t n, hipStream_t s) {
+  if (fp64)
+    hipLaunchKernelGGL(zero_kernel<double2>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (double2*)p, n);
+  else
+    hipLaunchKernelGGL(zero_kernel<float2>, dim3(grid_for(n)), dim3(256), 0, s,
+                       (float2*)p, n);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cg_dot(int mode, const void* a, const void* b, int64_t n_cells, int nb,
+               void* part, const int* skip, hipStream_t s) {
+  if (mode == 0)
+    hipLaunchKernelGGL(dot_partials_kernel<0>, dim3(PNH_RED_BLOCKS), dim3(256), 0, s,
+                       (const double2*)a, (const double2*)b, n_cells, nb, (double2*)part, skip);
+  else
+    hipLaunchKernelGGL(dot_partials_kernel<1>, dim3(PNH_RED_BLOCKS), dim3(256), 0, s,
+                       (const double2*)a, (const double2*)b, n_cells, nb, (double2*)part, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter,
+                    int maxiter, double rtol, double atol, hipStream_t s) {
+  hipLaunchKernelGGL(cg_finalize_kernel, dim3(nb), dim3(256), 0, s,
+                     (const double2*)part, PNH_RED_BLOCKS, nb, sc, phase, iter, maxiter,
+                     rtol, atol);
+  PNH_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cg_all_done_kernel, dim3(1), dim3(64), 0, s, sc, nb);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
+                    const CgScalars* sc, int first, hipStream_t s) {
+  hipLaunchKernelGGL(cg_p_update_kernel, dim3(grid_for(n_cells * nb)), dim3(256), 0, s,
+                     (double2*)p, (const double2*)r, n_cells, nb, sc, first);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
+                     int nb, const CgScalars* sc, void* part, hipStream_t s) {
+  hipLaunchKernelGGL(cg_xr_update_kernel, dim3(PNH_RED_BLOCKS), dim3(256), 0, s,
+                     (double2*)x, (double2*)r, (const double2*)p, (const double2*)q,
+                     n_cells, nb, sc, (double2*)part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}

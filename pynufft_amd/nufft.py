@@ -1,0 +1,445 @@
+"""NUFFT: pynufft's operator class on MI355X through the HIP C ABI.
+
+Same public surface as the reference's ``NUFFT`` (``nufft/__init__.py:
+496-728``): ``plan``, ``forward``, ``adjoint``, ``selfadjoint``, ``solve``
+and the stage methods ``x2xx, xx2k, k2y, y2k, k2xx, xx2x, k2y2k,
+xx2k_one2one, k2xx_one2one``.  numpy inputs give numpy outputs (like the
+reference's ``_*_host`` wrappers, ``_nufft_class_methods_cpu.py:365-445``);
+torch tensors give torch tensors on the plan's device.
+
+Precision: forward/adjoint/selfadjoint and the stages run on complex64 data
+with fp32 arithmetic (as the reference device path); ``k2y``/``y2k``/
+``k2y2k`` given complex128 input run in FP64, and ``solve(..., 'cg')`` runs
+the whole CG in FP64 as the reference CPU path does (scipy on complex128).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import helper
+
+_c64 = torch.complex64
+_c128 = torch.complex128
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class NUFFT:
+    """Min-max NUFFT operator on one HIP device.
+
+    ``NUFFT()`` uses the current HIP device; ``NUFFT(helper.device_list()[i])``
+    selects device i.  ``legacy=True`` stores the full prod(Jd) coefficient
+    table (the reference's CSR device path, ``nufft/__init__.py:538-540``);
+    the default keeps per-axis factors (the reference's pELL device path).
+    """
+
+    def __init__(self, device_indx=None, legacy=None):
+        lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError('pynufft_amd needs a HIP device (none visible)')
+        if device_indx is None:
+            dev = torch.device('cuda', torch.cuda.current_device())
+        elif isinstance(device_indx, tuple):
+            dev = torch.device(device_indx[4]) if isinstance(device_indx[4], torch.device) \
+                else torch.device('cuda', int(device_indx[2]))
+        elif isinstance(device_indx, torch.device):
+            dev = device_indx
+        else:
+            dev = torch.device('cuda', int(device_indx))
+        self._libh = lib
+        self.device = dev
+        self.processor = 'hip'
+        self.legacy = bool(legacy)
+        self.dtype = np.complex64
+        self.wavefront = 64
+        self.Nd = ()
+        self.Kd = ()
+        self.Jd = ()
+        self.ndims = 0
+        self.ft_axes = ()
+        self.batch = None
+        self._plan = None
+        self._ws = {}
+
+    # ------------------------------------------------------------ plan --
+    def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None):
+        """Design the interpolator on the host in FP64 and upload it.
+
+        Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
+        ``_plan_device`` (``_nufft_class_methods_device.py:98-181``); ``batch``
+        follows the deprecated ``NUFFT_hsa.plan`` (``linalg/nufft_hsa.py:
+        152-242``): coils share the trajectory, coil index fastest.
+        ``radix == len(Nd)`` stores the full coefficient table like
+        ``legacy=True``.  Returns 0.
+        """
+        self.release()
+        st = helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes)
+        dd = len(Nd)
+        self.ndims = dd
+        self.ft_axes = tuple(range(dd)) if ft_axes is None else tuple(ft_axes)
+        self.Nd, self.Kd, self.Jd = tuple(Nd), tuple(Kd), tuple(Jd)
+        if batch is not None and int(batch) < 1:
+            raise ValueError('batch must be >= 1')
+        self.batch = None if batch is None else int(batch)
+        self._nb = 1 if batch is None else int(batch)
+        M = int(st['M'])
+        self.M = (M,)
+        self.Kdprod = int(np.prod(Kd))
+        self.Jdprod = int(np.prod(Jd))
+        self.Ndprod = int(np.prod(Nd))
+        self.prodKd = (self.Kdprod,)
+        self.sn = np.asarray(st['sn'].astype(np.complex64), order='C')
+        self.st = {k: st[k] for k in ('Nd', 'Kd', 'Jd', 'M', 'om', 'tol',
+                                       'alpha', 'beta')}
+        self._host = st
+        self._format = 'CSR' if (self.legacy or (radix is not None and radix >= dd)) \
+            else 'factor'
+        dev = self.device
+        perm = st['perm']
+        self._t = {}
+        t = self._t
+        t['perm'] = torch.from_numpy(perm.astype(np.int32)).to(dev)
+        t['base'] = torch.from_numpy(
+            np.ascontiguousarray(st['base'][:, perm]).astype(np.int32)).to(dev)
+        u_sorted = [np.ascontiguousarray(st['ud'][a][perm]) for a in range(dd)]
+        self._u_off = [0] * _lib.MAXD
+        off = 0
+        for a in range(dd):
+            self._u_off[a] = off
+            off += u_sorted[a].size
+        u128 = np.concatenate([u.ravel() for u in u_sorted]) if M else \
+            np.zeros(1, np.complex128)
+        t['u_c64'] = torch.from_numpy(u128.astype(np.complex64)).to(dev)
+        self._u128_host = u128
+        t['sn_f32'] = torch.from_numpy(st['sn'].astype(np.float32).ravel()).to(dev)
+        t['sn_f64'] = torch.from_numpy(st['sn'].astype(np.float64).ravel()).to(dev)
+        for a in range(dd):
+            tw = helper.twiddles(Kd[a])
+            t['tw64_%d' % a] = torch.from_numpy(tw.astype(np.complex64)).to(dev)
+            t['tw128_%d' % a] = torch.from_numpy(tw).to(dev)
+        t['items'] = torch.from_numpy(np.ascontiguousarray(st['items'])).to(dev)
+        t['bin_rows'] = torch.from_numpy(
+            st['bin_rows'] if st['bin_rows'].size else np.zeros(1, np.int32)).to(dev)
+        t['merge'] = torch.from_numpy(np.ascontiguousarray(
+            st['merge'] if st['merge'].size else np.zeros((1, 3), np.int32))).to(dev)
+        if self._format == 'CSR':
+            self._build_csr()
+        self._create()
+        return 0
+
+    def _ensure_fp64(self):
+        if 'u_c128' not in self._t:
+            self._t['u_c128'] = torch.from_numpy(self._u128_host).to(self.device)
+            self._create()
+
+    def _build_csr(self):
+        """Full prod(Jd) coefficient table from the FP64 factors (rounded to
+        complex64) and its int32 cell indices, rows in slices of 64 with the
+        entry index outermost inside a slice."""
+        st = self._host
+        dd = self.ndims
+        M = self.M[0]
+        P = self.Jdprod
+        perm = st['perm']
+        dev = self.device
+        Mp = (M + 63) // 64 * 64
+        val = torch.zeros((Mp // 64, P, 64), dtype=_c64, device=dev)
+        col = torch.zeros((Mp // 64, P, 64), dtype=torch.int32, device=dev)
+        kstride = [int(np.prod(self.Kd[a + 1:])) for a in range(dd)]
+        step = 1 << 16
+        for s in range(0, M, step):
+            e = min(M, s + step)
+            rows = perm[s:e]
+            coef = torch.ones((e - s, 1), dtype=_c128, device=dev)
+            cell = torch.zeros((e - s, 1), dtype=torch.int64, device=dev)
+            for a in range(dd):
+                ua = torch.from_numpy(np.ascontiguousarray(st['ud'][a][rows])).to(dev)
+                ca = torch.from_numpy(np.mod(st['base'][a][rows][:, None] +
+                                             np.arange(self.Jd[a])[None, :],
+                                             self.Kd[a]) * kstride[a]).to(dev)
+                coef = (coef[:, :, None] * ua[:, None, :]).reshape(e - s, -1)
+                cell = (cell[:, :, None] + ca[:, None, :]).reshape(e - s, -1)
+            r = torch.arange(s, e, device=dev)
+            val[r // 64, :, r % 64] = coef.to(_c64)
+            col[r // 64, :, r % 64] = cell.to(torch.int32)
+        self._t['csr_val'] = val
+        self._t['csr_col'] = col
+
+    def _create(self):
+        if self._plan is not None:
+            self._libh.pnh_plan_destroy(self._plan)
+            self._plan = None
+        t = self._t
+        st = self._host
+        d = _lib.PlanDesc()
+        d.ndim = self.ndims
+        d.batch = self._nb
+        d.M = self.M[0]
+        for a in range(self.ndims):
+            d.Nd[a] = self.Nd[a]
+            d.Kd[a] = self.Kd[a]
+            d.Jd[a] = self.Jd[a]
+            d.u_off[a] = self._u_off[a]
+            d.tw_c64[a] = t['tw64_%d' % a].data_ptr()
+            d.tw_c128[a] = t['tw128_%d' % a].data_ptr()
+            d.tile[a] = st['tile'][a]
+        d.perm = t['perm'].data_ptr()
+        d.base = t['base'].data_ptr()
+        d.u_c64 = t['u_c64'].data_ptr()
+        d.u_c128 = t['u_c128'].data_ptr() if 'u_c128' in t else None
+        d.csr_c64 = t['csr_val'].data_ptr() if 'csr_val' in t else None
+        d.csr_col = t['csr_col'].data_ptr() if 'csr_col' in t else None
+        d.sn_f32 = t['sn_f32'].data_ptr()
+        d.sn_f64 = t['sn_f64'].data_ptr()
+        d.n_items = int(st['items'].shape[0])
+        d.items = t['items'].data_ptr()
+        d.bin_rows = t['bin_rows'].data_ptr()
+        d.n_merge = int(st['merge'].shape[0])
+        d.merge = t['merge'].data_ptr()
+        d.n_slots = int(st['n_slots'])
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self._libh.pnh_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._desc = d
+        self._plan = h
+        self._ws = {}
+
+    def release(self):
+        """Free the device plan and buffers (reference ``release``,
+        ``_nufft_class_methods_device.py:653-683``)."""
+        if getattr(self, '_plan', None) is not None:
+            self._libh.pnh_plan_destroy(self._plan)
+            self._plan = None
+        self._t = {}
+        self._ws = {}
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------- plumbing --
+    def _need_plan(self):
+        if self._plan is None:
+            raise RuntimeError('call plan() first')
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _workspace(self, op, dtype):
+        key = (op, dtype)
+        ws = self._ws.get(key)
+        if ws is None:
+            n = ctypes.c_size_t()
+            _lib.check(self._libh.pnh_workspace_bytes(self._plan, op, dtype,
+                                                      ctypes.byref(n)))
+            ws = torch.empty(max(int(n.value), 256), dtype=torch.uint8,
+                             device=self.device)
+            self._ws[key] = ws
+        return ws
+
+    def _shape(self, kind):
+        extra = () if self.batch is None else (self.batch,)
+        return {'x': self.Nd, 'k': self.Kd, 'y': self.M}[kind] + extra
+
+    def _in(self, a, kind, allow128=False, copy=False):
+        """Device tensor (contiguous complex) for a numpy/torch input."""
+        shape = self._shape(kind)
+        is_np = not isinstance(a, torch.Tensor)
+        if is_np:
+            a = np.asarray(a)
+            want = np.complex128 if (allow128 and a.dtype == np.complex128) \
+                else np.complex64
+            if a.size != int(np.prod(shape)):
+                raise ValueError('expected %s of shape %s, got %s'
+                                 % (kind, shape, a.shape))
+            t = torch.from_numpy(np.ascontiguousarray(a.reshape(shape), dtype=want))
+            t = t.to(self.device)
+        else:
+            want = _c128 if (allow128 and a.dtype == _c128) else _c64
+            if a.numel() != int(np.prod(shape)):
+                raise ValueError('expected %s of shape %s, got %s'
+                                 % (kind, shape, tuple(a.shape)))
+            t = a.to(device=self.device, dtype=want).reshape(shape)
+            t = t.clone() if copy else t.contiguous()
+        return t, is_np
+
+    @staticmethod
+    def _out(t, is_np):
+        return t.cpu().numpy() if is_np else t
+
+    def _empty(self, kind, dtype=_c64):
+        return torch.empty(self._shape(kind), dtype=dtype, device=self.device)
+
+    def _call(self, fn, *args):
+        with torch.cuda.device(self.device):
+            _lib.check(fn(self._plan, *args))
+
+    # --------------------------------------------------------- stages --
+    def x2xx(self, x):
+        """xx = x * sn (reference ``_x2xx_cpu``, ``_nufft_class_methods_cpu.py:224-230``)."""
+        self._need_plan()
+        xt, is_np = self._in(x, 'x')
+        out = self._empty('x')
+        self._call(self._libh.pnh_x2xx, _ptr(xt), _ptr(out), self._stream())
+        return self._out(out, is_np)
+
+    def xx2x(self, xx):
+        """x = xx * sn (reference ``_xx2x_cpu``, ``:346-351``)."""
+        self._need_plan()
+        xt, is_np = self._in(xx, 'x')
+        out = self._empty('x')
+        self._call(self._libh.pnh_xx2x, _ptr(xt), _ptr(out), self._stream())
+        return self._out(out, is_np)
+
+    def xx2k(self, xx):
+        """k = fftn(zero-pad(xx)) (reference ``_xx2k_cpu``, ``:232-249``)."""
+        self._need_plan()
+        xt, is_np = self._in(xx, 'x')
+        out = self._empty('k')
+        self._call(self._libh.pnh_xx2k, _ptr(xt), _ptr(out), self._stream())
+        return self._out(out, is_np)
+
+    def k2xx(self, k):
+        """xx = crop(ifftn(k)) (reference ``_k2xx_cpu``, ``:316-329``)."""
+        self._need_plan()
+        kt, is_np = self._in(k, 'k', copy=True)
+        out = self._empty('x')
+        self._call(self._libh.pnh_k2xx, _ptr(kt), _ptr(out), self._stream())
+        return self._out(out, is_np)
+
+    def xx2k_one2one(self, xx):
+        """Reference ``_xx2k_one2one_cpu`` (``:251-267``)."""
+        return self.xx2k(xx)
+
+    def k2xx_one2one(self, k):
+        """Reference ``_k2xx_one2one_cpu`` (``:331-344``)."""
+        return self.k2xx(k)
+
+    def k2y(self, k):
+        """y = P k (reference ``_k2y_cpu``, ``:282-288``); complex128 in -> FP64."""
+        self._need_plan()
+        kt, is_np = self._in(k, 'k', allow128=True)
+        fp64 = kt.dtype == _c128
+        if fp64:
+            self._ensure_fp64()
+        out = self._empty('y', kt.dtype)
+        self._call(self._libh.pnh_k2y, _lib.C128 if fp64 else _lib.C64,
+                   _ptr(kt), _ptr(out), self._stream())
+        return self._out(out, is_np)
+
+    def y2k(self, y):
+        """k = P^H y (reference ``_y2k_cpu``, ``:308-314``); complex128 in -> FP64."""
+        self._need_plan()
+        yt, is_np = self._in(y, 'y', allow128=True)
+        fp64 = yt.dtype == _c128
+        if fp64:
+            self._ensure_fp64()
+        dt = _lib.C128 if fp64 else _lib.C64
+        out = self._empty('k', yt.dtype)
+        ws = self._workspace(_lib.OP_Y2K, dt)
+        self._call(self._libh.pnh_y2k, dt, _ptr(yt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def k2y2k(self, k):
+        """P^H P k (reference ``_k2y2k_cpu``, ``:353-362``); complex128 in -> FP64."""
+        self._need_plan()
+        kt, is_np = self._in(k, 'k', allow128=True)
+        fp64 = kt.dtype == _c128
+        if fp64:
+            self._ensure_fp64()
+        dt = _lib.C128 if fp64 else _lib.C64
+        out = self._empty('k', kt.dtype)
+        ws = self._workspace(_lib.OP_K2Y2K, dt)
+        self._call(self._libh.pnh_k2y2k, dt, _ptr(kt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    # ------------------------------------------------------- operators --
+    def forward(self, x):
+        """y = k2y(xx2k(x2xx(x))) (reference ``_forward_cpu``, ``:168-179``)."""
+        self._need_plan()
+        xt, is_np = self._in(x, 'x')
+        out = self._empty('y')
+        ws = self._workspace(_lib.OP_FORWARD, _lib.C64)
+        self._call(self._libh.pnh_forward, _ptr(xt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def adjoint(self, y):
+        """x = xx2x(k2xx(y2k(y))) (reference ``_adjoint_cpu``, ``:181-193``)."""
+        self._need_plan()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_ADJOINT, _lib.C64)
+        self._call(self._libh.pnh_adjoint, _ptr(yt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def selfadjoint(self, x):
+        """A^H A x / prod(Kd) (reference ``_selfadjoint_cpu``, ``:201-214``)."""
+        self._need_plan()
+        xt, is_np = self._in(x, 'x')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SELFADJOINT, _lib.C64)
+        self._call(self._libh.pnh_selfadjoint, _ptr(xt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def solve(self, y, solver=None, *args, **kwargs):
+        """Solve with ``solver='cg'`` (reference ``linalg/solve_cpu.py:185-288``).
+
+        CG on the k-space normal equations P^H P k = P^H y with scipy's
+        semantics (x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the start
+        of an iteration, ``maxiter`` default 10 * prod(Kd)), all in FP64 on the
+        device; returns crop(ifftn(k)) / sn.  After the call ``self.cg_iters``
+        holds the iterations used (per coil with ``batch``).
+        """
+        if solver is None:
+            raise KeyError('solver must be one of: cg')
+        if solver in ('cgs', 'qmr', 'minres'):
+            raise TypeError(solver + ' requires real symmetric matrix')
+        if solver != 'cg':
+            raise NotImplementedError("solver %r is not provided; use 'cg'" % (solver,))
+        if args:
+            raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
+        maxiter = kwargs.pop('maxiter', None)
+        rtol = kwargs.pop('rtol', kwargs.pop('tol', 1e-5))
+        atol = kwargs.pop('atol', 0.0)
+        if kwargs:
+            raise TypeError('unsupported cg options: %s' % sorted(kwargs))
+        if maxiter is None:
+            maxiter = 10 * self.Kdprod
+        self._need_plan()
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_CG, _lib.C128)
+        iters = torch.zeros(self._nb, dtype=torch.int32, device=self.device)
+        self._call(self._libh.pnh_solve_cg, _ptr(yt), _ptr(out), int(maxiter),
+                   float(rtol), float(atol), _ptr(ws), _ptr(iters), self._stream())
+        self._cg_iters_dev = iters
+        return self._out(out, is_np)
+
+    @property
+    def cg_iters(self):
+        it = self._cg_iters_dev.cpu().numpy()
+        return int(it[0]) if self.batch is None else it
+
+    # ----------------------------------------------------------- misc --
+    def to_device(self, x, shape=None):
+        """complex64 copy on the plan's device (reference ``to_device``)."""
+        a = np.asarray(x) if not isinstance(x, torch.Tensor) else x
+        if isinstance(a, np.ndarray):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex64)).to(self.device)
+        return a.to(device=self.device, dtype=_c64).contiguous()
+
+    def to_host(self, data):
+        """numpy copy of a device tensor (reference ``to_host``)."""
+        return data.cpu().numpy()
