@@ -1,0 +1,202 @@
+"""Benchmark: 3-D NUFFT forward+adjoint pairs/s on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+
+Workload (default ``uniform3d``): Nd 128^3, Kd 256^3, Jd 6^3, M 2 097 152
+coordinates uniform in [-pi, pi)^3 (seed 301), complex64 image with N(0,1)
+parts (seed 302).  A step is one ``forward`` followed by one ``adjoint`` with
+the image, the samples and the plan resident in HBM.  Prints one JSON line.
+
+``roofline`` times the dominant kernel (k2y or y2k, whichever takes longer)
+with HIP events on the stream it is launched on and divides its algorithmic
+bytes per launch by the mean launch time.  ``cpu_baseline`` times the CPU
+oracle (a restatement of pynufft's NUFFT() path) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("3D 128³ NUFFT fwd+adj pairs/s (% HBM peak); "
+          "batched vs plain CG solves/s")
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel, fmt, M, Jd, Kd, nb=1):
+    """Bytes one launch must move at minimum (per-unit figures x units).
+
+    factor format, per sample: sum(Jd) complex64 factors + ndim int32 first
+    cells + int32 row->sample + nb complex64 samples; plus the Kd grid once.
+    CSR format, per sample: prod(Jd) complex64 coefficients + prod(Jd) int32
+    cells + int32 row->sample + nb complex64 samples; plus the grid once.
+    """
+    d = len(Kd)
+    K = int(np.prod(Kd))
+    P = int(np.prod(Jd))
+    if fmt == 'CSR':
+        per_sample = P * 8 + P * 4 + 4 + nb * 8
+    else:
+        per_sample = int(np.sum(Jd)) * 8 + d * 4 + 4 + nb * 8
+    return M * per_sample + K * 8 * nb
+
+
+def fft_bytes(Nd, Kd, nb=1):
+    """Pruned axis-by-axis FFT traffic with pad/crop fused (one direction)."""
+    d = len(Nd)
+    tot = 0
+    for a in range(d):
+        pre = int(np.prod(Nd[:a]))
+        post = int(np.prod(Kd[a + 1:]))
+        tot += pre * Nd[a] * post + pre * Kd[a] * post
+    return 8 * nb * tot
+
+
+def time_kernel(fn, reps, torch):
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps  # ms
+
+
+def cpu_baseline(w, m_sample, seed=7):
+    """Oracle pair time on the full grid with a sample subset, scaled to M.
+
+    t_pair = t(x2xx+xx2k) + t(k2xx+xx2x) + (t(k2y) + t(y2k)) * M / m_sample.
+    """
+    import oracle
+    om, x = w['om'], w['x']
+    M = om.shape[0]
+    rng = np.random.default_rng(seed)
+    sel = np.sort(rng.choice(M, m_sample, replace=False))
+    A = oracle.OracleNUFFT()
+    t0 = time.perf_counter()
+    A.plan(om[sel], w['Nd'], w['Kd'], w['Jd'])
+    t_plan = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    k = A.xx2k(A.x2xx(x))
+    t1 = time.perf_counter()
+    y = A.k2y(k)
+    t2 = time.perf_counter()
+    k2 = A.y2k(y)
+    t3 = time.perf_counter()
+    A.xx2x(A.k2xx(k2))
+    t4 = time.perf_counter()
+    scale = M / m_sample
+    t_pair = (t1 - t0) + (t4 - t3) + ((t2 - t1) + (t3 - t2)) * scale
+    return {
+        'value': 1.0 / t_pair, 'unit': 'pairs/s', 'cores': 1, 'kind': 'port',
+        'sample': ('oracle forward+adjoint on the full %s->%s grid with %d of %d '
+                   'samples; interpolation time scaled by M/%d (plan %.1f s '
+                   'excluded); numpy pocketfft and scipy CSR matvec are '
+                   'single-threaded' % (w['Nd'], w['Kd'], m_sample, M, m_sample,
+                                        t_plan)),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='uniform3d',
+                    choices=['uniform3d', 'radial2d', 'plain2d'])
+    ap.add_argument('--legacy', action='store_true',
+                    help='full coefficient table (CSR) instead of factors')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-sample', type=int, default=131072)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU: the operator does not shard across '
+              'GPUs (see the sharding design); run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pynufft_amd import workloads as W
+    from pynufft_amd.nufft import NUFFT
+
+    w = getattr(W, args.workload)()
+    t0 = time.perf_counter()
+    A = NUFFT(legacy=args.legacy)
+    A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    dev = A.device
+    x = torch.from_numpy(w['x']).to(dev)
+    M = w['om'].shape[0]
+
+    def pair():
+        y = A.forward(x)
+        return A.adjoint(y)
+
+    for _ in range(args.warmup):
+        pair()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pair()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1000.0 * dt / args.steps
+    value = args.steps / dt
+
+    # dominant kernel: k2y vs y2k timed alone with HIP events
+    k = A.xx2k(x)
+    y = A.k2y(k)
+    reps = max(5, min(50, args.steps))
+    t_k2y = time_kernel(lambda: A.k2y(k), reps, torch)
+    t_y2k = time_kernel(lambda: A.y2k(y), reps, torch)
+    t_fft = time_kernel(lambda: A.xx2k(x), reps, torch)
+    fmt = A._format
+    if t_k2y >= t_y2k:
+        kname, kms = 'k2y', t_k2y
+    else:
+        kname, kms = 'y2k', t_y2k
+    nbytes = algorithmic_bytes(kname, fmt, M, w['Jd'], w['Kd'])
+    achieved = nbytes / (kms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(HERE, 'profiles', 'traffic.json')
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get('%s_%s_%s' % (args.workload, fmt, kname))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, min(args.cpu_sample, M))
+
+    pair_bytes = (2 * algorithmic_bytes('k2y', fmt, M, w['Jd'], w['Kd']) +
+                  2 * fft_bytes(w['Nd'], w['Kd']))
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'pairs/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'c64', 'data': 'synthetic',
+        'config': {'workload': args.workload, 'Nd': list(w['Nd']),
+                   'Kd': list(w['Kd']), 'Jd': list(w['Jd']), 'M': M,
+                   'format': fmt, 'step': 'forward+adjoint'},
+        'roofline': {'bound': 'hbm', 'kernel': kname, 'achieved': achieved,
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'algorithmic_bytes': nbytes, 'launch_ms': kms},
+        'kernels_ms': {'k2y': t_k2y, 'y2k': t_y2k, 'xx2k': t_fft},
+        'pair_hbm_frac': pair_bytes * value / 1e9 / HBM_PEAK_GBS,
+        'pair_algorithmic_bytes': pair_bytes,
+        'plan_s': t_plan,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

@@ -17,7 +17,9 @@ the device needs:
 Error behaviour follows the reference: non-tuple Nd/Kd/Jd raise TypeError,
 length mismatches raise KeyError (helper.py:644-654).
 """
+import concurrent.futures
 import math
+import os
 
 import numpy as np
 import scipy.linalg
@@ -124,33 +126,45 @@ def overlap_pinv(N, J, K, alpha, beta):
     return scipy.linalg.pinv(acc)
 
 
-def minmax_axis(om, N, J, K, alpha, beta, Tinv, chunk=1 << 18):
+def _minmax_chunk(o, N, J, K, alpha, beta, Tinv):
+    gam = 2.0 * np.pi / (K * 1.0)
+    L = np.size(alpha) - 1
+    ratio = 1.0 * K / N
+    koff = np.floor(1.0 * o / gam - 1.0 * J / 2.0)
+    arg = np.add.outer(-np.arange(1, J + 1) * 1.0, 1.0 * o / gam - koff)  # (J, m)
+    r = np.zeros(arg.shape, dtype=np.float32)
+    for l in range(-L, L + 1):
+        r = r + (alpha[abs(l)] * 1.0) * np.sinc((arg + 1.0 * l * beta) / ratio)
+    c = Tinv.dot(r)
+    v = np.exp(1.0j * gam * (N * 1.0 - 1.0) / 2.0 * arg) * c
+    v = np.exp(-1.0j * o * N / 2.0) * v
+    return v.T.conj(), np.mod(koff.astype(np.int64) + 1, K)
+
+
+def minmax_axis(om, N, J, K, alpha, beta, Tinv, chunk=1 << 16, pool=None):
     """Per-sample min-max factors along one axis.
 
     Returns u (M, J) complex128 -- the reference's ``ud[d]`` (helper.py:606-618
     with OMEGA_u :148-162) -- and base (M,) int64, the first support cell
-    mod(floor(om/gam - J/2) + 1, K) (OMEGA_k :164-210).  Processed in sample
-    chunks to bound host memory.
+    mod(floor(om/gam - J/2) + 1, K) (OMEGA_k :164-210).  Sample chunks are
+    independent and run on ``pool`` (numpy releases the GIL); the arithmetic
+    per sample is the same as unchunked.
     """
     M = om.shape[0]
     u = np.empty((M, J), dtype=np.complex128)
     base = np.empty(M, dtype=np.int64)
-    gam = 2.0 * np.pi / (K * 1.0)
-    L = np.size(alpha) - 1
-    ratio = 1.0 * K / N
-    jneg = -np.arange(1, J + 1) * 1.0
-    for s in range(0, M, chunk):
-        o = om[s:s + chunk]
-        koff = np.floor(1.0 * o / gam - 1.0 * J / 2.0)
-        arg = np.add.outer(jneg, 1.0 * o / gam - koff)          # (J, m)
-        r = np.zeros(arg.shape, dtype=np.float32)
-        for l in range(-L, L + 1):
-            r = r + (alpha[abs(l)] * 1.0) * np.sinc((arg + 1.0 * l * beta) / ratio)
-        c = Tinv.dot(r)
-        v = np.exp(1.0j * gam * (N * 1.0 - 1.0) / 2.0 * arg) * c
-        v = np.exp(-1.0j * o * N / 2.0) * v
-        u[s:s + chunk] = v.T.conj()
-        base[s:s + chunk] = np.mod(koff.astype(np.int64) + 1, K)
+
+    def job(s):
+        uu, bb = _minmax_chunk(om[s:s + chunk], N, J, K, alpha, beta, Tinv)
+        u[s:s + chunk] = uu
+        base[s:s + chunk] = bb
+
+    starts = range(0, M, chunk)
+    if pool is None:
+        for s in starts:
+            job(s)
+    else:
+        list(pool.map(job, starts))
     return u, base
 
 
@@ -205,11 +219,11 @@ def sort_and_bin(base, Kd, Jd, tile, chunk_entries=None):
         ta = per_axis[a]
         tid = (tid[:, :, None] * tgrid[a] + ta[:, None, :]).reshape(M, -1)
         valid = (valid[:, :, None] & (ta[:, None, :] >= 0)).reshape(M, -1)
-    samp = np.broadcast_to(np.arange(M)[:, None], tid.shape)[valid]
-    ptile = tid[valid]
-    order = np.lexsort((samp, ptile))
-    ptile = ptile[order]
-    rows = inv[samp[order]].astype(np.int32)
+    samp = np.broadcast_to(np.arange(M, dtype=np.int64)[:, None], tid.shape)[valid]
+    key = tid[valid] * max(M, 1) + samp          # (tile, sample) lexicographic
+    key.sort()
+    ptile = key // max(M, 1)
+    rows = inv[key - ptile * max(M, 1)].astype(np.int32)
     counts = np.bincount(ptile, minlength=n_tiles).astype(np.int64)
     starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
     total = int(counts.sum())
@@ -270,15 +284,18 @@ def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_entries=None):
     st = {'Nd': Nd, 'Kd': Kd, 'Jd': Jd, 'M': np.int32(M), 'om': om,
           'tol': 0, 'alpha': [], 'beta': []}
     snd, ud, based = [], [], []
-    for a in range(dd):
-        alpha, beta = fit_scaling(Nd[a], Jd[a], Kd[a])
-        st['alpha'].append(alpha)
-        st['beta'].append(beta)
-        snd.append(scaling_1d(Nd[a], Kd[a], alpha, beta))
-        Tinv = overlap_pinv(Nd[a], Jd[a], Kd[a], alpha, beta)
-        u, b = minmax_axis(om[:, a], Nd[a], Jd[a], Kd[a], alpha, beta, Tinv)
-        ud.append(u)
-        based.append(b)
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for a in range(dd):
+            alpha, beta = fit_scaling(Nd[a], Jd[a], Kd[a])
+            st['alpha'].append(alpha)
+            st['beta'].append(beta)
+            snd.append(scaling_1d(Nd[a], Kd[a], alpha, beta))
+            Tinv = overlap_pinv(Nd[a], Jd[a], Kd[a], alpha, beta)
+            u, b = minmax_axis(om[:, a], Nd[a], Jd[a], Kd[a], alpha, beta, Tinv,
+                               pool=pool if M > (1 << 17) else None)
+            ud.append(u)
+            based.append(b)
     # full scaling factor = real part of the outer product (helper.py:558-584)
     sn = np.reshape(1.0, (1,) * dd)
     for a in range(dd):
