@@ -199,6 +199,10 @@ def sort_and_bin(base, Kd, Jd, tile, chunk_entries=None):
     perm = np.argsort(tkey, kind='stable').astype(np.int64)
     inv = np.empty(M, dtype=np.int64)
     inv[perm] = np.arange(M)
+    if M == 0:
+        items = np.stack([np.arange(n_tiles), np.zeros(n_tiles), np.zeros(n_tiles),
+                          -np.ones(n_tiles)], axis=1).astype(np.int32)
+        return (perm, items, np.zeros(0, np.int32), np.zeros((0, 3), np.int32), 0)
     # distinct tiles touched per axis, per sample: cells (b + j) mod K
     per_axis = []
     for a in range(d):

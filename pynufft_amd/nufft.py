@@ -102,9 +102,12 @@ class NUFFT:
         perm = st['perm']
         self._t = {}
         t = self._t
-        t['perm'] = torch.from_numpy(perm.astype(np.int32)).to(dev)
+        # (at least one element each, so that M = 0 still passes valid pointers)
+        t['perm'] = torch.from_numpy(
+            perm.astype(np.int32) if M else np.zeros(1, np.int32)).to(dev)
         t['base'] = torch.from_numpy(
-            np.ascontiguousarray(st['base'][:, perm]).astype(np.int32)).to(dev)
+            np.ascontiguousarray(st['base'][:, perm]).astype(np.int32)
+            if M else np.zeros(dd, np.int32)).to(dev)
         u_sorted = [np.ascontiguousarray(st['ud'][a][perm]) for a in range(dd)]
         self._u_off = [0] * _lib.MAXD
         off = 0

@@ -162,6 +162,14 @@ def make_radial2d(ref):
     sa = A.selfadjoint(x)
     kk = A.xx2k(A.x2xx(x))
     cg = A.solve(y_noisy, 'cg', maxiter=30)
+    cg10 = A.solve(y_noisy, 'cg', maxiter=10)
+    cg20 = A.solve(y_noisy, 'cg', maxiter=20)
+    # the same problem with the samples in another order: the spread of the
+    # reference's own 30-iteration result under an FP64 summation reordering
+    perm = np.random.default_rng(2025).permutation(M)
+    B = ref.NUFFT()
+    B.plan(om[perm], Nd, Kd, Jd)
+    cg_perm = B.solve(y_noisy[perm], 'cg', maxiter=30)
     save('radial2d', x_fp=np.array(W.fingerprint(x)),
          om_fp=np.array(W.fingerprint(om)), ysel=ysel, xsel=xsel, ksel=ksel,
          forward_sel=y[ysel], forward_fp=np.array(W.fingerprint(y)),
@@ -169,7 +177,9 @@ def make_radial2d(ref):
          adjoint=adj.astype(np.complex64), selfadjoint_sel=_subset(sa, xsel),
          selfadjoint_norm=np.linalg.norm(sa), xx2k_sel=_subset(kk, ksel),
          xx2k_norm=np.linalg.norm(kk), solve_cg=cg.astype(np.complex64),
-         cg_iters=np.array(30))
+         solve_cg10=cg10.astype(np.complex64),
+         solve_cg20=cg20.astype(np.complex64),
+         solve_cg_permuted=cg_perm.astype(np.complex64), cg_iters=np.array(30))
 
 
 def make_uniform3d(ref):

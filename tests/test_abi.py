@@ -1,0 +1,84 @@
+"""The C ABI library: it loads, exports every symbol include/pynufft_hip.h
+declares, its descriptor layout matches the ctypes mirror, and argument errors
+come back as status codes with a message (no GPU needed for any of this)."""
+import ctypes
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, 'include', 'pynufft_hip.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(pnh_[a-z0-9_]+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from pynufft_amd import _lib
+    return _lib.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 17
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    from pynufft_amd import _lib
+    assert set(names) == set(_lib.EXPORTS), 'ctypes prototypes out of sync'
+
+
+def test_version_and_error_convention(lib):
+    from pynufft_amd import _lib
+    assert _lib.version().startswith('pynufft_hip')
+    d = _lib.PlanDesc()
+    h = ctypes.c_void_p()
+    rc = lib.pnh_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == -1
+    assert b'ndim' in lib.pnh_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+    n = ctypes.c_size_t()
+    assert lib.pnh_workspace_bytes(None, 0, 0, ctypes.byref(n)) == -1
+    assert lib.pnh_forward(None, None, None, None, None) == -1
+
+
+def test_plan_desc_layout_matches_header():
+    """Compile a C probe against the header; compare sizeof/offsetof."""
+    from pynufft_amd import _lib
+    cc = shutil.which('gcc') or shutil.which('cc')
+    if cc is None:
+        pytest.skip('no C compiler')
+    fields = [f for f, _ in _lib.PlanDesc._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "pynufft_hip.h"',
+           'int main(void){', 'printf("%zu\\n", sizeof(pnh_plan_desc));']
+    src += ['printf("%%zu\\n", offsetof(pnh_plan_desc, %s));' % f for f in fields]
+    src += ['return 0;}']
+    with tempfile.TemporaryDirectory() as tmp:
+        c = os.path.join(tmp, 'probe.c')
+        exe = os.path.join(tmp, 'probe')
+        with open(c, 'w') as fh:
+            fh.write('\n'.join(src))
+        subprocess.check_call([cc, '-I', os.path.dirname(HEADER), c, '-o', exe])
+        out = [int(v) for v in subprocess.check_output([exe]).split()]
+    assert out[0] == ctypes.sizeof(_lib.PlanDesc)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(_lib.PlanDesc, f).offset == off, f
+
+
+def test_operator_requires_gpu_or_raises_loudly():
+    """No silent CPU fallback: without a device the operator refuses."""
+    import torch
+    from pynufft_amd.nufft import NUFFT
+    if torch.cuda.is_available():
+        pytest.skip('device present')
+    with pytest.raises(RuntimeError):
+        NUFFT()

@@ -1,0 +1,128 @@
+"""The CPU oracle against golden vectors written by the reference itself.
+
+Fixtures come from tests/golden/make_golden.py, which ran pynufft's own CPU
+NUFFT() path on these inputs.  Plan factors and stage outputs must agree to
+the last bit (or within 1e-15 where an FP64 product order could differ).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pynufft_amd import workloads as W
+from conftest import TINY_CASES, golden, geometry, rel
+
+
+@pytest.mark.parametrize('name', TINY_CASES)
+def test_plan_matches_reference(name):
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    st = oracle.plan(g['om'], Nd, Kd, Jd)
+    for d in range(len(Nd)):
+        assert np.array_equal(st['alpha'][d], g['alpha_%d' % d])
+        assert st['beta'][d] == int(g['beta_%d' % d])
+        assert np.array_equal(st['snd'][d], g['snd_%d' % d])
+    assert np.array_equal(st['sn'], g['sn'])
+    p = st['p']
+    assert np.array_equal(p.indices, g['p_indices'])
+    assert np.array_equal(p.indptr, g['p_indptr'])
+    assert np.array_equal(p.data, g['p_data'])
+
+
+@pytest.mark.parametrize('name', TINY_CASES)
+def test_stages_match_reference(name):
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    x, yin = g['x'], g['adjoint_in']
+    assert np.array_equal(A.sn, g['sn_c64'])
+    assert np.array_equal(A.xx2k(A.x2xx(x)), g['xx2k'])
+    assert np.array_equal(A.forward(x), g['forward'])
+    assert np.array_equal(A.adjoint(yin), g['adjoint'])
+    assert np.array_equal(A.y2k(yin), g['y2k'])
+    assert np.array_equal(A.selfadjoint(x), g['selfadjoint'])
+    assert np.array_equal(A.solve(yin, 'cg', maxiter=int(g['cg_iters'])),
+                          g['solve_cg'])
+
+
+def test_plain2d_matches_reference():
+    g = golden('plain2d')
+    Nd, Kd, Jd = geometry(g)
+    w = W.plain2d()
+    assert np.array_equal(w['om'], g['om']) and np.array_equal(w['x'], g['x'])
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    P = int(np.prod(Jd))
+    rows = g['p_rows']
+    sel = (rows[:, None] * P + np.arange(P)[None, :]).ravel()
+    assert np.array_equal(A.sp.data[sel], g['p_rows_data'])
+    assert np.array_equal(A.sp.indices[sel], g['p_rows_indices'])
+    assert np.allclose(W.fingerprint(A.sp.data), g['p_fingerprint'], rtol=1e-13)
+    assert int(A.sp.indices.astype(np.int64).sum()) == int(g['p_index_sum'])
+    x, yin = g['x'], g['adjoint_in']
+    assert np.array_equal(A.forward(x), g['forward'])
+    assert np.array_equal(A.adjoint(yin), g['adjoint'])
+    assert np.array_equal(A.selfadjoint(x), g['selfadjoint'])
+    assert np.array_equal(A.solve(yin, 'cg', maxiter=30), g['solve_cg'])
+
+
+def test_grid_j1_matches_reference():
+    g = golden('grid_j1')
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    assert np.array_equal(A.forward(g['x']), g['forward'])
+    assert np.array_equal(A.adjoint(g['forward'].astype(np.complex64)), g['adjoint'])
+
+
+@pytest.mark.slow
+def test_small3d_matches_reference():
+    g = golden('small3d')
+    Nd, Kd, Jd = geometry(g)
+    om = W.uniform_om(20000, 3, 303)
+    x = W.gaussian_image(Nd, 304)
+    assert np.allclose(W.fingerprint(om), g['om_fp'], rtol=0, atol=0)
+    assert np.allclose(W.fingerprint(x), g['x_fp'], rtol=0, atol=0)
+    A = oracle.OracleNUFFT()
+    A.plan(om, Nd, Kd, Jd)
+    y = A.forward(x)
+    assert rel(y, g['forward']) < 1e-15
+    yin = g['forward'].astype(np.complex64)
+    assert rel(A.adjoint(yin), g['adjoint']) < 1e-15
+    assert rel(A.selfadjoint(x), g['selfadjoint']) < 1e-15
+    assert rel(A.solve(yin, 'cg', maxiter=30), g['solve_cg']) < 1e-6
+
+
+@pytest.mark.slow
+def test_radial2d_matches_reference():
+    g = golden('radial2d')
+    w = W.radial2d()
+    assert np.array_equal(np.array(W.fingerprint(w['om'])), g['om_fp'])
+    assert np.array_equal(np.array(W.fingerprint(w['x'])), g['x_fp'])
+    A = oracle.OracleNUFFT()
+    A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
+    y = A.forward(w['x'])
+    assert rel(y[g['ysel']], g['forward_sel']) < 1e-15
+    assert abs(np.linalg.norm(y) - float(g['forward_norm'])) < 1e-12 * float(g['forward_norm'])
+    assert rel(A.adjoint(g['y_noisy']), g['adjoint']) < 1e-7
+    assert rel(A.solve(g['y_noisy'], 'cg', maxiter=20), g['solve_cg20']) < 1e-6
+
+
+def test_uniform3d_rows_match_reference():
+    """64 full CSR rows of the 128^3/256^3 problem (plan is row-local)."""
+    g = golden('uniform3d')
+    w = W.uniform3d()
+    assert np.array_equal(np.array(W.fingerprint(w['om'])), g['om_fp'])
+    assert np.array_equal(np.array(W.fingerprint(w['x'])), g['x_fp'])
+    sel = g['ysel'][:64]
+    st = oracle.plan(w['om'][sel], w['Nd'], w['Kd'], w['Jd'])
+    assert np.array_equal(st['p'].data, g['rows64_p_data'])
+    assert np.array_equal(st['p'].indices, g['rows64_p_indices'])
+    for d in range(3):
+        assert np.array_equal(st['alpha'][d], g['rows64_alpha_%d' % d])
+        assert np.array_equal(st['snd'][d], g['rows64_snd_%d' % d])
+
+
+def test_workload_generators_stable():
+    g = golden('small3d')
+    assert np.array_equal(np.array(W.fingerprint(W.uniform_om(20000, 3, 303))), g['om_fp'])

@@ -1,0 +1,127 @@
+"""Pins of the oracle that do not come from the reference: closed forms,
+brute force on tiny inputs, and operator identities.  Each is chosen so a
+plausible mistake (dropped phase, transposed axis, wrong sign or scale) fails.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from pynufft_amd import workloads as W
+from conftest import golden, geometry, rel
+
+
+def brute_nudft(om, Nd, x):
+    """Triple loop: y[m] = sum_n x[n] prod_d exp(-i om[m,d] (n_d - N_d/2))."""
+    y = np.zeros(om.shape[0], dtype=np.complex128)
+    for m in range(om.shape[0]):
+        for n in np.ndindex(*Nd):
+            ph = sum(om[m, d] * (n[d] - Nd[d] / 2) for d in range(len(Nd)))
+            y[m] += x[n] * np.exp(-1j * ph)
+    return y
+
+
+def test_nudft_against_brute_force():
+    rng = np.random.default_rng(5)
+    Nd = (4, 5)
+    om = rng.uniform(-np.pi, np.pi, (7, 2))
+    x = rng.standard_normal(Nd) + 1j * rng.standard_normal(Nd)
+    D = oracle.OracleNUDFT()
+    D.plan(om, Nd)
+    assert rel(D.forward(x), brute_nudft(om, Nd, x)) < 1e-13
+    y = rng.standard_normal(7) + 1j * rng.standard_normal(7)
+    # adjoint = F^H y / prod(Nd): check <F x, y> = prod(Nd) <x, adj(y)>
+    lhs = np.vdot(y, D.forward(x))
+    rhs = np.prod(Nd) * np.vdot(D.adjoint(y), x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+def test_grid_aligned_j1_equals_nudft():
+    """Jd = 1, Kd = Nd, om on the grid: alpha = [1,0..], sn = 1, T = 1 and
+    u = e^{i om N/2}, so forward is exactly the NUDFT (up to complex64 FFT)."""
+    g = golden('grid_j1')
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    assert np.all(A.sn == 1)
+    for d in range(2):
+        assert A.st['alpha'][d][0] == 1 and np.all(A.st['alpha'][d][1:] == 0)
+    D = oracle.OracleNUDFT()
+    D.plan(g['om'], Nd)
+    assert rel(A.forward(g['x']), D.forward(g['x'])) < 1e-6
+
+
+@pytest.mark.parametrize('Nd,Kd', [((6,), (10,)), ((5, 3), (8, 6))])
+def test_xx2k_is_padded_dft(Nd, Kd):
+    """xx2k = DFT of the front-corner zero-padded image (brute-force matrix)."""
+    rng = np.random.default_rng(9)
+    om = rng.uniform(-np.pi, np.pi, (5, len(Nd)))
+    A = oracle.OracleNUFFT()
+    A.plan(om, Nd, Kd, tuple([2] * len(Nd)))
+    xx = (rng.standard_normal(Nd) + 1j * rng.standard_normal(Nd)).astype(np.complex64)
+    pad = np.zeros(Kd, dtype=np.complex128)
+    pad[tuple(slice(0, n) for n in Nd)] = xx
+    F = np.array([[1.0]])
+    for K in Kd:
+        k = np.arange(K)
+        F = np.kron(F, np.exp(-2j * np.pi * np.outer(k, k) / K))
+    ref = (F @ pad.ravel()).reshape(Kd)
+    assert rel(A.xx2k(xx), ref) < 1e-6
+    # k2xx inverts it on the kept corner
+    assert rel(A.k2xx(ref), xx) < 1e-6
+
+
+@pytest.mark.parametrize('name', ['tiny2d', 'tiny3d_mixed', 'plain2d'])
+def test_adjoint_identity(name):
+    """<forward(x), y> = prod(Kd) <x, adjoint(y)> (adjoint is A^H / prod(Kd))."""
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    rng = np.random.default_rng(3)
+    x = W.gaussian_image(Nd, 1)
+    y = (rng.standard_normal(g['om'].shape[0]) +
+         1j * rng.standard_normal(g['om'].shape[0])).astype(np.complex64)
+    lhs = np.vdot(y, A.forward(x))
+    rhs = np.prod(Kd) * np.vdot(A.adjoint(y), x)
+    assert abs(lhs - rhs) < 1e-5 * abs(lhs)
+
+
+def test_interpolator_accuracy_improves_with_J():
+    """Against the FP64 NUDFT: J=6 forward error ~5e-6 and monotone in J."""
+    w = W.plain2d()
+    D = oracle.OracleNUDFT()
+    D.plan(w['om'], w['Nd'])
+    yd = D.forward(w['x'])
+    errs = []
+    for J in (3, 4, 6):
+        A = oracle.OracleNUFFT()
+        A.plan(w['om'], w['Nd'], w['Kd'], (J, J))
+        errs.append(rel(A.forward(w['x']), yd))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 2e-5
+
+
+def test_cg_converges_to_min_norm_solution():
+    """CG from x0 = 0 on consistent P^H P k = P^H y reaches pinv(P^H P) P^H y."""
+    g = golden('tiny2d')
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    P = A.sp.toarray()
+    y = g['adjoint_in'].astype(np.complex128)
+    b = P.conj().T @ y
+    kstar = np.linalg.pinv(P.conj().T @ P) @ b
+    k, info = oracle.cg_scipy(lambda v: P.conj().T @ (P @ v), b, maxiter=500,
+                              rtol=1e-13)
+    assert rel(k, kstar) < 1e-8
+    assert rel(P.conj().T @ (P @ k), b) < 1e-11
+
+
+def test_plan_argument_errors():
+    om = np.zeros((3, 2))
+    with pytest.raises(TypeError):
+        oracle.plan(om, [8, 8], (16, 16), (4, 4))
+    with pytest.raises(KeyError):
+        oracle.plan(om, (8, 8), (16, 16), (4,))
+    with pytest.raises(TypeError):
+        oracle.kb_shape(17, 2.0)
