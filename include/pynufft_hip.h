@@ -64,8 +64,10 @@ typedef struct pnh_plan_desc {
   const void* u_c64;         /* complex64 per-dim factors u_d[row][j], concatenated */
   const void* u_c128;        /* complex128 factors (FP64 operators), may be NULL */
   int64_t u_off[PNH_MAXD];   /* element offset of dim d's block */
-  const void* csr_c64;       /* complex64 full coefficients, slices of 64 rows
-                                interleaved: [(M+63)/64][prod Jd][64]; may be NULL */
+  const void* csr_c64;       /* complex64 full coefficients (may be NULL), in
+                                slices of S rows: [ceil(M/S)][G][S][J_last],
+                                G = prod(Jd[:-1]), S = 64 / L where
+                                L = J_last (<= 4 or 6), 8 (<= 8) or 16 */
   const int32_t* csr_col;    /* int32 grid cell of each entry, same layout */
 
   const float* sn_f32;       /* [prod Nd] apodization (reference st['sn']) */

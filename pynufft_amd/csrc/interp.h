@@ -24,7 +24,9 @@ struct BinArgs {
   int tile[PNH_MAXD];          // tile extent per dim
   int tgrid[PNH_MAXD];         // tiles per dim
   int tile_cells;              // prod(tile)
-  int waves_per_block;
+  int waves_per_block;         // set per launch
+  int batch;                   // entries staged per step (set per launch)
+  int wave_lds_bytes;          // LDS bytes per wavefront (set per launch)
   size_t lds_budget;
   int64_t n_items;
   const int* items;            // [n_items][4]
@@ -32,6 +34,16 @@ struct BinArgs {
   int64_t n_merge;
   const int* merge;            // [n_merge][3]
 };
+
+// Lanes per sample used by k2y (and by the CSR table layout): one lane per
+// support offset on the last axis, rounded to a divisor-friendly width.
+__host__ __device__ inline int pnh_lanes_per_sample(int J_last) {
+  if (J_last <= 4) return J_last;
+  if (J_last == 6) return 6;
+  if (J_last <= 8) return 8;
+  if (J_last <= 16) return 16;
+  return 0;
+}
 
 int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s);

@@ -21,134 +21,173 @@
 
 namespace {
 
-template <int JC>
-struct Jdims {
-  __device__ __forceinline__ static int get(const int* J, int d) {
-    return JC > 0 ? JC : J[d];
-  }
-};
-
 __device__ __forceinline__ int wrap_add(int b, int j, int K) {
   int c = b + j;
   return c >= K ? c - K : c;
 }
 
+template <typename C>
+__device__ __forceinline__ C shfl_down_c(C v, int off) {
+  C z;
+  z.x = __shfl_down(v.x, off, 64);
+  z.y = __shfl_down(v.y, off, 64);
+  return z;
+}
+
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rl_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rl_f(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename C>
+__device__ __forceinline__ C rl_c(C v, int l) {
+  C z; z.x = rl_f(v.x, l); z.y = rl_f(v.y, l); return z;
+}
+
+// Offset of CSR entry (row r, flat support index j = g*JL + jl, g the index
+// over all but the last axis) in the slice-interleaved table: slices of SPW
+// rows, then g, then row-in-slice, then jl.
+__device__ __forceinline__ int64_t csr_index(int64_t r, int g, int jl, int G,
+                                             int JL, int SPW) {
+  const int64_t slice = r / SPW;
+  const int rl = (int)(r - slice * SPW);
+  return ((slice * G + g) * SPW + rl) * JL + jl;
+}
+
 // ---------------------------------------------------------------- k2y ----
+// JLC lanes per sample (one per last-axis support offset), SPW = 64/JLC
+// samples per wavefront; per-lane partial sums are combined with a segmented
+// shuffle reduction.  JLC == 0: generic, 16 lanes per sample (J_last <= 16).
 // y[perm[r]*nb + c0 + c] = sum_j coef(r, j) * grid[cell(r, j)*nb + c0 + c]
-template <typename T, int D, int JC, bool CSRV, int NBC>
+template <typename T, int D, int JLC, bool CSRV, int NBC>
 __global__ void __launch_bounds__(256)
 k2y_kernel(InterpArgs a, const void* __restrict__ grid_, void* __restrict__ y_,
            int c0, const int* __restrict__ skip) {
   typedef typename cplx<T>::t C;
   if (skip && *skip) return;
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= a.M) return;
+  constexpr int LPS = JLC > 0 ? JLC : 16;
+  constexpr int SPW = 64 / LPS;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int sl = lane / LPS;
+  const int jl = lane - sl * LPS;
+  const int JL = a.J[D - 1];
+  const int64_t r = wave * SPW + sl;
+  const bool valid = sl < SPW && jl < JL && r < a.M;
   const C* __restrict__ grid = (const C*)grid_;
-  C* __restrict__ y = (C*)y_;
-  const C* __restrict__ u = (const C*)a.u;
   const int nb = a.nb;
-  const int J0 = Jdims<JC>::get(a.J, 0);
-  const int J1 = D > 1 ? Jdims<JC>::get(a.J, 1) : 1;
-  const int J2 = D > 2 ? Jdims<JC>::get(a.J, 2) : 1;
 
   C acc[NBC];
 #pragma unroll
   for (int c = 0; c < NBC; ++c) acc[c] = mk<T>((T)0, (T)0);
 
-  if (CSRV) {
-    const int P = J0 * J1 * J2;
-    const int64_t slice = r >> 6, lane = r & 63;
-    const C* __restrict__ val = (const C*)a.csr_val + (slice * P) * 64 + lane;
-    const int* __restrict__ col = a.csr_col + (slice * P) * 64 + lane;
-    for (int j = 0; j < P; ++j) {
-      const C w = val[(int64_t)j * 64];
-      const int64_t cell = col[(int64_t)j * 64];
+  if (valid) {
+    const int J0 = a.J[0];
+    const int J1 = D > 1 ? a.J[1] : 1;
+    if (CSRV) {
+      const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
+      const C* __restrict__ val = (const C*)a.csr_val;
+      const int* __restrict__ col = a.csr_col;
+      const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
+      const int64_t gs = (int64_t)SPW * JL;
+      for (int g = 0; g < G; ++g) {
+        const C w = val[i0 + g * gs];
+        const int64_t cell = col[i0 + g * gs];
 #pragma unroll
-      for (int c = 0; c < NBC; ++c)
-        acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-    }
-  } else {
-    const int b0 = a.base[r];
-    const int b1 = D > 1 ? a.base[a.M + r] : 0;
-    const int b2 = D > 2 ? a.base[2 * a.M + r] : 0;
-    const C* u0 = u + a.u_off[0] + r * J0;
-    const C* u1 = u + (D > 1 ? a.u_off[1] + r * J1 : 0);
-    const C* u2 = u + (D > 2 ? a.u_off[2] + r * J2 : 0);
-    // With a static J the last-dimension factors and wrapped cell offsets
-    // are kept in registers; the product order stays ((u0*u1)*u2).
-    constexpr int JR = JC > 0 ? JC : 1;
-    C ul[JR];
-    int64_t cl[JR];
-    if (JC > 0 && D > 1) {
-      const C* ulast = D == 2 ? u1 : u2;
-      const int bl = D == 2 ? b1 : b2;
-#pragma unroll
-      for (int j = 0; j < JR; ++j) {
-        ul[j] = ulast[j];
-        cl[j] = wrap_add(bl, j, a.K[D - 1]);
+        for (int c = 0; c < NBC; ++c)
+          acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
       }
-    }
-    for (int j0 = 0; j0 < J0; ++j0) {
-      const int64_t i0 = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0];
-      const C w0 = u0[j0];
+    } else {
+      const C* __restrict__ u = (const C*)a.u;
+      const C ul = u[a.u_off[D - 1] + r * JL + jl];
+      const int bl = a.base[(D - 1) * a.M + r];
+      const int64_t cl = wrap_add(bl, jl, a.K[D - 1]);
       if (D == 1) {
 #pragma unroll
         for (int c = 0; c < NBC; ++c)
-          acc[c] = cadd(acc[c], cmul(w0, grid[i0 * nb + c0 + c]));
-        continue;
-      }
-      if (D == 2) {
-        if (JC > 0) {
-#pragma unroll
-          for (int j1 = 0; j1 < JR; ++j1) {
-            const C w = cmul(w0, ul[j1]);
-            const int64_t cell = i0 + cl[j1];
-#pragma unroll
-            for (int c = 0; c < NBC; ++c)
-              acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-          }
-        } else {
-          for (int j1 = 0; j1 < J1; ++j1) {
-            const C w = cmul(w0, u1[j1]);
-            const int64_t cell = i0 + wrap_add(b1, j1, a.K[1]);
-#pragma unroll
-            for (int c = 0; c < NBC; ++c)
-              acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-          }
-        }
-        continue;
-      }
-      for (int j1 = 0; j1 < J1; ++j1) {
-        const int64_t i1 = i0 + (int64_t)wrap_add(b1, j1, a.K[1]) * a.Kstride[1];
-        const C w01 = cmul(w0, u1[j1]);
-        if (JC > 0) {
-#pragma unroll
-          for (int j2 = 0; j2 < JR; ++j2) {
-            const C w = cmul(w01, ul[j2]);
-            const int64_t cell = i1 + cl[j2];
+          acc[c] = cmul(ul, grid[cl * nb + c0 + c]);
+      } else {
+        const int b0 = a.base[r];
+        const C* u0 = u + a.u_off[0] + r * J0;
+        if (D == 2) {
+          for (int j0 = 0; j0 < J0; ++j0) {
+            const int64_t cell = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
+            const C w = cmul(u0[j0], ul);
 #pragma unroll
             for (int c = 0; c < NBC; ++c)
               acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
           }
         } else {
-          for (int j2 = 0; j2 < J2; ++j2) {
-            const C w = cmul(w01, u2[j2]);
-            const int64_t cell = i1 + wrap_add(b2, j2, a.K[2]);
+          const int b1 = a.base[a.M + r];
+          const C* u1 = u + a.u_off[1] + r * J1;
+          for (int j0 = 0; j0 < J0; ++j0) {
+            const int64_t i0 = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
+            const C w0 = u0[j0];
+            for (int j1 = 0; j1 < J1; ++j1) {
+              const int64_t cell = i0 + (int64_t)wrap_add(b1, j1, a.K[1]) * a.Kstride[1];
+              const C w = cmul(cmul(w0, u1[j1]), ul);
 #pragma unroll
-            for (int c = 0; c < NBC; ++c)
-              acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
+              for (int c = 0; c < NBC; ++c)
+                acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
+            }
           }
         }
       }
     }
   }
-  const int64_t m = a.perm[r];
+  // segmented reduction over the LPS lanes of each sample (lane jl == 0 ends
+  // with the sum of lanes jl < JL of its group)
 #pragma unroll
-  for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
+  for (int off = 1; off < LPS; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < NBC; ++c) {
+      const C o = shfl_down_c(acc[c], off);
+      if (jl + off < LPS) acc[c] = cadd(acc[c], o);
+    }
+  }
+  if (valid && jl == 0) {
+    const int64_t m = a.perm[r];
+    C* __restrict__ y = (C*)y_;
+#pragma unroll
+    for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
+  }
 }
 
 // ---------------------------------------------------------------- y2k ----
+// Support offsets j in [0, J) whose cell (b + j) mod K lies in the tile
+// [T0, T0 + E).  Returns 1 and one interval [lo, lo + n) when the hit set is
+// a single interval (the common case), 0 when it is two intervals (a wrapped
+// support meeting both ends of a tile that nearly spans the axis).
+__device__ __forceinline__ int clip_axis(int b, int J, int K, int T0, int E,
+                                         int& lo, int& n) {
+  if (E == K) { lo = 0; n = J; return 1; }
+  const int T1 = T0 + E;
+  const int l1 = max(0, T0 - b), h1 = min(min(J, K - b), T1 - b);
+  const int l2 = max(K - b, T0 - b + K), h2 = min(J, T1 - b + K);
+  const bool a1 = h1 > l1, a2 = h2 > l2;
+  if (a1 && a2) { lo = 0; n = 0; return 0; }
+  if (a1) { lo = l1; n = h1 - l1; }
+  else if (a2) { lo = l2; n = h2 - l2; }
+  else { lo = 0; n = 0; }
+  return 1;
+}
+
+// q -> (q / n, q % n) for q < 64, n <= 64 via a 16-bit reciprocal
+__device__ __forceinline__ void divmod_small(int q, int n, int magic, int& d, int& m) {
+  d = (q * magic) >> 16;
+  m = q - d * n;
+}
+
 // One wavefront per work item (tile id, entry start, entry count, slot).
+// Entries are taken in batches: lanes load one entry's row, sample value and
+// first cells each (and stage its per-axis factors in LDS); the entries of
+// the batch are then applied one after another, lanes spread over the
+// support cells that fall inside the tile.
 template <typename T, int D, bool CSRV, int NBC>
 __global__ void __launch_bounds__(256)
 y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
@@ -162,13 +201,18 @@ y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
   const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
   if (item >= b.n_items) return;  // whole wave leaves together
   const int cells = b.tile_cells;
-  T* acc = reinterpret_cast<T*>(smem_raw) + (size_t)wave * cells * 2 * NBC;
+  const int J0 = a.J[0];
+  const int J1 = D > 1 ? a.J[1] : 1;
+  const int J2 = D > 2 ? a.J[2] : 1;
+  const int SJ = J0 + (D > 1 ? J1 : 0) + (D > 2 ? J2 : 0);
+  T* acc = reinterpret_cast<T*>(smem_raw + (size_t)wave * b.wave_lds_bytes);
+  C* stage = reinterpret_cast<C*>(acc + (size_t)cells * 2 * NBC);
+  int* srow = reinterpret_cast<int*>(stage + (size_t)b.batch * SJ);
   const C* __restrict__ y = (const C*)y_;
   const C* __restrict__ u = (const C*)a.u;
   const int nb = a.nb;
 
   const int4 it = reinterpret_cast<const int4*>(b.items)[item];
-  // tile origin and extent
   int tid = it.x;
   int T0[PNH_MAXD], E[PNH_MAXD];
 #pragma unroll
@@ -184,55 +228,148 @@ y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
 
   for (int i = lane; i < cells * 2 * NBC; i += 64) acc[i] = (T)0;
 
-  const int J0 = a.J[0];
-  const int J1 = D > 1 ? a.J[1] : 1;
-  const int J2 = D > 2 ? a.J[2] : 1;
-  const int P = J0 * J1 * J2;
-  const int J12 = J1 * J2;
-
+  const int JL = a.J[D - 1];
+  const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
+  const int LPS = pnh_lanes_per_sample(JL);
+  const int SPW = 64 / LPS;
   const int* __restrict__ rows = b.bin_rows + it.y;
   const int cnt = it.z;
-  for (int e = 0; e < cnt; ++e) {
-    const int64_t r = rows[e];
-    const int64_t m = a.perm[r];
-    int bb[PNH_MAXD];
+  const int BATCH = b.batch;
+
+  for (int e0 = 0; e0 < cnt; e0 += BATCH) {
+    const int ne = min(BATCH, cnt - e0);
+    // ---- lane-parallel setup: lane l prepares entry e0 + l ----
+    int my_r = 0;
+    int my_b[PNH_MAXD] = {0, 0, 0};
+    C my_y[NBC];
 #pragma unroll
-    for (int d = 0; d < D; ++d) bb[d] = a.base[d * a.M + r];
-    C yv[NBC];
+    for (int c = 0; c < NBC; ++c) my_y[c] = mk<T>((T)0, (T)0);
+    int my_geo = 0, my_start = 0, my_m1 = 0, my_m2 = 0, my_tot = 0;
+    if (lane < ne) {
+      my_r = rows[e0 + lane];
+      const int64_t m = a.perm[my_r];
 #pragma unroll
-    for (int c = 0; c < NBC; ++c) yv[c] = y[m * nb + c0 + c];
-    for (int q = lane; q < P; q += 64) {
-      const int j0 = q / J12;
-      const int rem = q - j0 * J12;
-      const int j1 = D > 1 ? rem / J2 : 0;
-      const int j2 = D > 2 ? rem - j1 * J2 : 0;
-      // tile-local coordinates (unsigned compare rejects cells outside)
-      const int l0 = wrap_add(bb[0], j0, a.K[0]) - T0[0];
-      if ((unsigned)l0 >= (unsigned)E[0]) continue;
-      int l1 = 0, l2 = 0;
-      if (D > 1) {
-        l1 = wrap_add(bb[1], j1, a.K[1]) - T0[1];
-        if ((unsigned)l1 >= (unsigned)E1) continue;
+      for (int d = 0; d < D; ++d) my_b[d] = a.base[d * a.M + my_r];
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) my_y[c] = y[m * nb + c0 + c];
+      // clip the support box to the tile: per axis first offset lo, count n
+      // and tile-local first cell s; flag 1 << 30 when some axis splits
+      int lo[PNH_MAXD] = {0, 0, 0}, n[PNH_MAXD] = {1, 1, 1}, st0[PNH_MAXD] = {0, 0, 0};
+      int single = 1;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        single &= clip_axis(my_b[d], a.J[d], a.K[d], T0[d], E[d], lo[d], n[d]);
+        st0[d] = wrap_add(my_b[d], lo[d], a.K[d]) - T0[d];
       }
-      if (D > 2) {
-        l2 = wrap_add(bb[2], j2, a.K[2]) - T0[2];
-        if ((unsigned)l2 >= (unsigned)E2) continue;
+      my_tot = single ? n[0] * n[1] * n[2] : -1;
+      my_geo = lo[0] | (lo[1] << 5) | (lo[2] << 10) | (n[1] << 20) | (n[2] << 25);
+      my_start = (st0[0] * E1 + st0[1]) * E2 + st0[2];
+      my_m2 = (65536 + n[2] - 1) / max(n[2], 1);
+      my_m1 = (65536 + n[1] - 1) / max(n[1], 1);
+    }
+    if (!CSRV) {
+      // rows of the batch through LDS (a lane may read any entry's row)
+      if (lane < ne) srow[lane] = my_r;
+      for (int idx = lane; idx < ne * SJ; idx += 64) {
+        const int ei = idx / SJ;
+        const int q = idx - ei * SJ;
+        const int64_t re = srow[ei];
+        int64_t src;
+        if (q < J0) src = a.u_off[0] + re * J0 + q;
+        else if (D > 1 && q < J0 + J1) src = a.u_off[1] + re * J1 + (q - J0);
+        else src = a.u_off[2] + re * J2 + (q - J0 - J1);
+        stage[idx] = u[src];
       }
-      C w;
-      if (CSRV) {
-        const int64_t slice = r >> 6, ln = r & 63;
-        w = ((const C*)a.csr_val)[(slice * P + q) * 64 + ln];
+    }
+    // ---- apply the entries in order ----
+    for (int i = 0; i < ne; ++i) {
+      const int tot = rl_i(my_tot, i);
+      C yv[NBC];
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) yv[c] = rl_c(my_y[c], i);
+      const C* st = stage + i * SJ;
+      const int r = rl_i(my_r, i);
+      if (tot >= 0) {
+        const int geo = rl_i(my_geo, i);
+        const int start = rl_i(my_start, i);
+        const int m1 = rl_i(my_m1, i), m2 = rl_i(my_m2, i);
+        const int lo0 = geo & 31, lo1 = (geo >> 5) & 31, lo2 = (geo >> 10) & 31;
+        const int n1 = (geo >> 20) & 31, n2 = (geo >> 25) & 31;
+        for (int q = lane; q < tot; q += 64) {
+          int t0, t1, t2;
+          if (tot <= 256) {
+            // exact for q < 256 and n <= 16 (16-bit reciprocals)
+            const int t = (q * m2) >> 16;
+            t2 = q - t * n2;
+            t0 = (t * m1) >> 16;
+            t1 = t - t0 * n1;
+          } else {
+            t2 = q % n2;
+            const int t = q / n2;
+            t1 = t % n1;
+            t0 = t / n1;
+          }
+          const int cell = start + (t0 * E1 + t1) * E2 + t2;
+          C w;
+          if (CSRV) {
+            const int j0 = lo0 + t0, j1 = lo1 + t1, j2 = lo2 + t2;
+            const int g = D == 1 ? 0 : (D == 2 ? j0 : j0 * J1 + j1);
+            const int jlast = D == 1 ? j0 : (D == 2 ? j1 : j2);
+            w = ((const C*)a.csr_val)[csr_index(r, g, jlast, G, JL, SPW)];
+          } else {
+            w = st[lo0 + t0];
+            if (D > 1) w = cmul(w, st[J0 + lo1 + t1]);
+            if (D > 2) w = cmul(w, st[J0 + J1 + lo2 + t2]);
+          }
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) {
+            const C v = cmulc(w, yv[c]);
+            atomicAdd(&acc[(c * 2 + 0) * cells + cell], v.x);
+            atomicAdd(&acc[(c * 2 + 1) * cells + cell], v.y);
+          }
+        }
       } else {
-        w = u[a.u_off[0] + r * J0 + j0];
-        if (D > 1) w = cmul(w, u[a.u_off[1] + r * J1 + j1]);
-        if (D > 2) w = cmul(w, u[a.u_off[2] + r * J2 + j2]);
-      }
-      const int cell = (l0 * E1 + l1) * E2 + l2;
+        // a wrapped support meets both ends of this tile: masked walk over
+        // the whole support box
+        int bb[PNH_MAXD];
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) {
-        const C v = cmulc(w, yv[c]);
-        atomicAdd(&acc[(c * 2 + 0) * cells + cell], v.x);
-        atomicAdd(&acc[(c * 2 + 1) * cells + cell], v.y);
+        for (int d = 0; d < D; ++d) bb[d] = rl_i(my_b[d], i);
+        const int P = J0 * J1 * J2;
+        const int J12 = J1 * J2;
+        for (int q = lane; q < P; q += 64) {
+          const int j0 = q / J12;
+          const int rem = q - j0 * J12;
+          const int j1 = D > 1 ? rem / J2 : 0;
+          const int j2 = D > 2 ? rem - j1 * J2 : 0;
+          const int l0 = wrap_add(bb[0], j0, a.K[0]) - T0[0];
+          if ((unsigned)l0 >= (unsigned)E[0]) continue;
+          int l1 = 0, l2 = 0;
+          if (D > 1) {
+            l1 = wrap_add(bb[1], j1, a.K[1]) - T0[1];
+            if ((unsigned)l1 >= (unsigned)E1) continue;
+          }
+          if (D > 2) {
+            l2 = wrap_add(bb[2], j2, a.K[2]) - T0[2];
+            if ((unsigned)l2 >= (unsigned)E2) continue;
+          }
+          C w;
+          if (CSRV) {
+            const int g = D == 1 ? 0 : (D == 2 ? j0 : j0 * J1 + j1);
+            const int jlast = D == 1 ? j0 : (D == 2 ? j1 : j2);
+            w = ((const C*)a.csr_val)[csr_index(r, g, jlast, G, JL, SPW)];
+          } else {
+            w = st[j0];
+            if (D > 1) w = cmul(w, st[J0 + j1]);
+            if (D > 2) w = cmul(w, st[J0 + J1 + j2]);
+          }
+          const int cell = (l0 * E1 + l1) * E2 + l2;
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) {
+            const C v = cmulc(w, yv[c]);
+            atomicAdd(&acc[(c * 2 + 0) * cells + cell], v.x);
+            atomicAdd(&acc[(c * 2 + 1) * cells + cell], v.y);
+          }
+        }
       }
     }
   }
@@ -303,11 +440,13 @@ y2k_merge_kernel(InterpArgs a, BinArgs b, const void* __restrict__ part_,
   }
 }
 
-template <typename T, int D, int JC, bool CSRV, int NBC>
+template <typename T, int D, int LPS, bool CSRV, int NBC>
 int launch_k2y_t(const InterpArgs& a, const void* grid, void* y, int c0,
                  const int* skip, hipStream_t s) {
-  const int64_t nblk = (a.M + 255) / 256;
-  hipLaunchKernelGGL((k2y_kernel<T, D, JC, CSRV, NBC>), dim3((unsigned)nblk),
+  constexpr int SPW = 64 / LPS;
+  const int64_t waves = (a.M + SPW - 1) / SPW;
+  const int64_t nblk = (waves + 3) / 4;
+  hipLaunchKernelGGL((k2y_kernel<T, D, LPS, CSRV, NBC>), dim3((unsigned)nblk),
                      dim3(256), 0, s, a, grid, y, c0, skip);
   PNH_LAUNCH_CHECK();
   return 0;
@@ -316,11 +455,18 @@ int launch_k2y_t(const InterpArgs& a, const void* grid, void* y, int c0,
 template <typename T, int D, bool CSRV, int NBC>
 int launch_k2y_j(const InterpArgs& a, const void* grid, void* y, int c0,
                  const int* skip, hipStream_t s) {
-  bool allj6 = true;
-  for (int d = 0; d < D; ++d) allj6 = allj6 && a.J[d] == 6;
-  if (!CSRV && allj6 && D > 1)
-    return launch_k2y_t<T, D, 6, CSRV, NBC>(a, grid, y, c0, skip, s);
-  return launch_k2y_t<T, D, 0, CSRV, NBC>(a, grid, y, c0, skip, s);
+  switch (pnh_lanes_per_sample(a.J[D - 1])) {
+    case 1: return launch_k2y_t<T, D, 1, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 2: return launch_k2y_t<T, D, 2, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 3: return launch_k2y_t<T, D, 3, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 4: return launch_k2y_t<T, D, 4, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 6: return launch_k2y_t<T, D, 6, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 8: return launch_k2y_t<T, D, 8, CSRV, NBC>(a, grid, y, c0, skip, s);
+    case 16: return launch_k2y_t<T, D, 16, CSRV, NBC>(a, grid, y, c0, skip, s);
+    default:
+      pnh_set_error("interpolator size %d on the last axis is above 16", a.J[D - 1]);
+      return -2;
+  }
 }
 
 template <typename T, int D, bool CSRV>
@@ -328,11 +474,8 @@ int launch_k2y_nb(const InterpArgs& a, const void* grid, void* y,
                   const int* skip, hipStream_t s) {
   int c0 = 0;
   while (c0 < a.nb) {
-    const int left = a.nb - c0;
     int rc;
-    if (left >= 8) { rc = launch_k2y_j<T, D, CSRV, 8>(a, grid, y, c0, skip, s); c0 += 8; }
-    else if (left >= 4) { rc = launch_k2y_j<T, D, CSRV, 4>(a, grid, y, c0, skip, s); c0 += 4; }
-    else if (left >= 2) { rc = launch_k2y_j<T, D, CSRV, 2>(a, grid, y, c0, skip, s); c0 += 2; }
+    if (a.nb - c0 >= 4) { rc = launch_k2y_j<T, D, CSRV, 4>(a, grid, y, c0, skip, s); c0 += 4; }
     else { rc = launch_k2y_j<T, D, CSRV, 1>(a, grid, y, c0, skip, s); c0 += 1; }
     if (rc) return rc;
   }
@@ -340,22 +483,39 @@ int launch_k2y_nb(const InterpArgs& a, const void* grid, void* y,
 }
 
 template <typename T, int D, bool CSRV>
-int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
+int launch_y2k_nb(const InterpArgs& a, const BinArgs& b0, const void* y,
                   void* grid, void* part, const int* skip, hipStream_t s) {
-  const int waves = b.waves_per_block;
-  const int64_t nblk = (b.n_items + waves - 1) / waves;
+  typedef typename cplx<T>::t C;
+  int SJ = 0;
+  for (int d = 0; d < D; ++d) SJ += a.J[d];
   int c0 = 0;
   while (c0 < a.nb) {
-    // LDS per wave: tile_cells * 2 * NBC * sizeof(T); keep it within budget
     const int left = a.nb - c0;
     int nbc = left >= 4 ? 4 : (left >= 2 ? 2 : 1);
-    while (nbc > 1 && (size_t)waves * b.tile_cells * 2 * nbc * sizeof(T) > b.lds_budget)
-      nbc >>= 1;
-    const size_t lds = (size_t)waves * b.tile_cells * 2 * nbc * sizeof(T);
-    if (lds > b.lds_budget) {
-      pnh_set_error("y2k tile needs %zu bytes of LDS (budget %zu)", lds, b.lds_budget);
-      return -2;
+    BinArgs b = b0;
+    // per-wave LDS: accumulators (2 * nbc * cells reals) + factor staging
+    // (batch * SJ complex) + batch row ids; shrink until one wave fits
+    int batch = 32;
+    size_t wl = 0;
+    for (;;) {
+      wl = (size_t)b.tile_cells * 2 * nbc * sizeof(T) + (size_t)batch * SJ * sizeof(C) +
+           (size_t)batch * sizeof(int);
+      wl = (wl + 15) & ~(size_t)15;
+      if (wl <= b.lds_budget) break;
+      if (nbc > 1) nbc >>= 1;
+      else if (batch > 4) batch >>= 1;
+      else {
+        pnh_set_error("y2k tile needs %zu bytes of LDS (budget %zu)", wl, b.lds_budget);
+        return -2;
+      }
     }
+    int waves = 4;
+    while (waves > 1 && (size_t)waves * wl > b.lds_budget) --waves;
+    b.batch = batch;
+    b.wave_lds_bytes = (int)wl;
+    b.waves_per_block = waves;
+    const size_t lds = (size_t)waves * wl;
+    const int64_t nblk = (b.n_items + waves - 1) / waves;
     dim3 grid_dim((unsigned)nblk), blk(64 * waves);
     switch (nbc) {
       case 4: hipLaunchKernelGGL((y2k_binned_kernel<T, D, CSRV, 4>), grid_dim, blk, lds, s, a, b, y, grid, part, c0, skip); break;
@@ -386,12 +546,15 @@ int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
 #define K2Y_CASE(T, D)                                                       \
   return csrv ? launch_k2y_nb<T, D, true>(a, grid, y, skip, s)               \
               : launch_k2y_nb<T, D, false>(a, grid, y, skip, s);
+#define K2Y_CASE64(D) return launch_k2y_nb<double, D, false>(a, grid, y, skip, s);
   if (fp64) {
-    switch (a.D) { case 1: K2Y_CASE(double, 1) case 2: K2Y_CASE(double, 2) case 3: K2Y_CASE(double, 3) }
+    if (csrv) { pnh_set_error("FP64 operators use the per-axis factors"); return -2; }
+    switch (a.D) { case 1: K2Y_CASE64(1) case 2: K2Y_CASE64(2) case 3: K2Y_CASE64(3) }
   } else {
     switch (a.D) { case 1: K2Y_CASE(float, 1) case 2: K2Y_CASE(float, 2) case 3: K2Y_CASE(float, 3) }
   }
 #undef K2Y_CASE
+#undef K2Y_CASE64
   pnh_set_error("unsupported ndim %d", a.D);
   return -2;
 }
@@ -403,12 +566,15 @@ int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
 #define Y2K_CASE(T, D)                                                       \
   return csrv ? launch_y2k_nb<T, D, true>(a, b, y, grid, part, skip, s)      \
               : launch_y2k_nb<T, D, false>(a, b, y, grid, part, skip, s);
+#define Y2K_CASE64(D) return launch_y2k_nb<double, D, false>(a, b, y, grid, part, skip, s);
   if (fp64) {
-    switch (a.D) { case 1: Y2K_CASE(double, 1) case 2: Y2K_CASE(double, 2) case 3: Y2K_CASE(double, 3) }
+    if (csrv) { pnh_set_error("FP64 operators use the per-axis factors"); return -2; }
+    switch (a.D) { case 1: Y2K_CASE64(1) case 2: Y2K_CASE64(2) case 3: Y2K_CASE64(3) }
   } else {
     switch (a.D) { case 1: Y2K_CASE(float, 1) case 2: Y2K_CASE(float, 2) case 3: Y2K_CASE(float, 3) }
   }
 #undef Y2K_CASE
+#undef Y2K_CASE64
   pnh_set_error("unsupported ndim %d", a.D);
   return -2;
 }

@@ -24,6 +24,19 @@ _c64 = torch.complex64
 _c128 = torch.complex128
 
 
+def lanes_per_sample(J_last):
+    """Lanes per sample of the k2y kernel (fixes the CSR table layout)."""
+    if J_last <= 4:
+        return J_last
+    if J_last == 6:
+        return 6
+    if J_last <= 8:
+        return 8
+    if J_last <= 16:
+        return 16
+    raise NotImplementedError('Jd above 16 on the last axis is not supported')
+
+
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
@@ -66,7 +79,8 @@ class NUFFT:
         self._ws = {}
 
     # ------------------------------------------------------------ plan --
-    def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None):
+    def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
+             tile=None, chunk=None):
         """Design the interpolator on the host in FP64 and upload it.
 
         Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
@@ -74,10 +88,14 @@ class NUFFT:
         follows the deprecated ``NUFFT_hsa.plan`` (``linalg/nufft_hsa.py:
         152-242``): coils share the trajectory, coil index fastest.
         ``radix == len(Nd)`` stores the full coefficient table like
-        ``legacy=True``.  Returns 0.
+        ``legacy=True``.  ``tile`` (gridding tile per axis) and ``chunk``
+        (bin entries per gridding work item) are tuning knobs; the defaults
+        come from ``helper.tile_shape`` and ``helper.sort_and_bin``.
+        Returns 0.
         """
         self.release()
-        st = helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes)
+        st = helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile,
+                         chunk_entries=chunk)
         dd = len(Nd)
         self.ndims = dd
         self.ft_axes = tuple(range(dd)) if ft_axes is None else tuple(ft_axes)
@@ -141,17 +159,20 @@ class NUFFT:
 
     def _build_csr(self):
         """Full prod(Jd) coefficient table from the FP64 factors (rounded to
-        complex64) and its int32 cell indices, rows in slices of 64 with the
-        entry index outermost inside a slice."""
+        complex64) and its int32 grid cells, laid out for k2y's lanes: slices
+        of S rows, [slice][g][row in slice][j_last] with g the flat index over
+        all but the last axis and S = 64 / lanes_per_sample(J_last)."""
         st = self._host
         dd = self.ndims
         M = self.M[0]
-        P = self.Jdprod
+        JL = self.Jd[-1]
+        G = self.Jdprod // JL
+        S = 64 // lanes_per_sample(JL)
         perm = st['perm']
         dev = self.device
-        Mp = (M + 63) // 64 * 64
-        val = torch.zeros((Mp // 64, P, 64), dtype=_c64, device=dev)
-        col = torch.zeros((Mp // 64, P, 64), dtype=torch.int32, device=dev)
+        ns = (M + S - 1) // S
+        val = torch.zeros((ns, G, S, JL), dtype=_c64, device=dev)
+        col = torch.zeros((ns, G, S, JL), dtype=torch.int32, device=dev)
         kstride = [int(np.prod(self.Kd[a + 1:])) for a in range(dd)]
         step = 1 << 16
         for s in range(0, M, step):
@@ -167,8 +188,8 @@ class NUFFT:
                 coef = (coef[:, :, None] * ua[:, None, :]).reshape(e - s, -1)
                 cell = (cell[:, :, None] + ca[:, None, :]).reshape(e - s, -1)
             r = torch.arange(s, e, device=dev)
-            val[r // 64, :, r % 64] = coef.to(_c64)
-            col[r // 64, :, r % 64] = cell.to(torch.int32)
+            val[r // S, :, r % S, :] = coef.to(_c64).reshape(e - s, G, JL)
+            col[r // S, :, r % S, :] = cell.to(torch.int32).reshape(e - s, G, JL)
         self._t['csr_val'] = val
         self._t['csr_col'] = col
 

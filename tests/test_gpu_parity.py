@@ -202,3 +202,49 @@ def test_solver_argument_errors(hip):
         A.solve(y, 'bicgstab')
     with pytest.raises(ValueError):
         A.forward(np.zeros((3, 3), np.complex64))
+
+
+def _random_geometries(n, seed=77):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        d = int(rng.integers(1, 4))
+        Jd, Nd, Kd = [], [], []
+        for _ in range(d):
+            J = int(rng.integers(1, 9))
+            N = int(rng.integers(max(2, J), 24))
+            ratio = rng.choice([1.0, 1.5, 2.0])
+            K = max(int(round(N * ratio)), J)
+            # FFT lengths with factors 2, 3, 5, 7 only
+            while any(K % p == 0 for p in (11, 13, 17, 19, 23)) or K < N:
+                K += 1
+            if K == 2 * N and J > 16:
+                J = 16
+            Jd.append(J)
+            Nd.append(N)
+            Kd.append(K)
+        M = int(rng.integers(1, 200))
+        om = rng.uniform(-np.pi, np.pi, (M, d))
+        out.append((tuple(Nd), tuple(Kd), tuple(Jd), om))
+    return out
+
+
+@pytest.mark.parametrize('case', range(24))
+def test_random_geometries(hip, case):
+    """Small random 1-3D problems (odd sizes, K/N in {1, 1.5, 2}, J 1..8,
+    tiles that nearly span an axis) against the oracle."""
+    Nd, Kd, Jd, om = _random_geometries(24)[case]
+    try:
+        ref = oracle.OracleNUFFT()
+        ref.plan(om, Nd, Kd, Jd)
+    except TypeError:
+        pytest.skip('the reference rejects this J at K/N = 2')
+    A = make(dict(om=om, Nd=Nd, Kd=Kd, Jd=Jd), chunk=4 if case % 2 else None)
+    rng = np.random.default_rng(case)
+    x = W.gaussian_image(Nd, case)
+    y = (rng.standard_normal(om.shape[0]) + 1j * rng.standard_normal(om.shape[0]))
+    assert rel(A.y2k(y.astype(np.complex128)), ref.y2k(y)) < TOL64
+    k = ref.xx2k(ref.x2xx(x)).astype(np.complex128)
+    assert rel(A.k2y(k), ref.k2y(k)) < TOL64
+    assert rel(A.forward(x), ref.forward(x)) < TOL
+    assert rel(A.adjoint(y.astype(np.complex64)), ref.adjoint(y.astype(np.complex64))) < TOL
