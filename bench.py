@@ -116,6 +116,9 @@ def main():
                     help='full coefficient table (CSR) instead of factors')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample', type=int, default=131072)
+    ap.add_argument('--copy-probe-mib', type=int, default=0,
+                    help='after timing, copy this many MiB device-to-device '
+                         '(a known-size kernel to calibrate HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the operator does not shard across '
@@ -170,6 +173,16 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get('%s_%s_%s' % (args.workload, fmt, kname))
+
+    if args.copy_probe_mib:
+        src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -63,6 +63,8 @@ typedef struct pnh_plan_desc {
   const int32_t* base;       /* [ndim][M] first support cell per dim (row order) */
   const void* u_c64;         /* complex64 per-dim factors u_d[row][j], concatenated */
   const void* u_c128;        /* complex128 factors (FP64 operators), may be NULL */
+  const void* du_c64;        /* complex64 d u_d / d om_d, same layout as u_c64;
+                                needed only by pnh_forward_grad_om, may be NULL */
   int64_t u_off[PNH_MAXD];   /* element offset of dim d's block */
   const void* csr_c64;       /* complex64 full coefficients (may be NULL), in
                                 slices of S rows: [ceil(M/S)][G][S][J_last],
@@ -134,6 +136,16 @@ int pnh_adjoint(const pnh_plan* plan, const void* y, void* x, void* ws, void* st
  * Reference nufft/_nufft_class_methods_cpu.py:201-214 (_selfadjoint_cpu). */
 int pnh_selfadjoint(const pnh_plan* plan, const void* x, void* x_out, void* ws,
                     void* stream);
+/* Gradient of a real loss with respect to the sample coordinates, for
+ * y = k2y(k): om_grad[m][d] = sum_c Re(conj(g[m][c]) * dy[m][c] / d om[m][d]),
+ * where g = dL/d conj(y) (torch's convention), k is the oversampled spectrum
+ * xx2k(x2xx(x)) of the forward pass and om_grad is float32 [M][ndim] in the
+ * original sample order.  The derivative goes through the per-axis min-max
+ * factors (product rule across axes); the support cells are piecewise
+ * constant in om.  No counterpart in the reference (new); it backs the
+ * trajectory gradient of pynufft_amd.autograd. */
+int pnh_forward_grad_om(const pnh_plan* plan, const void* k, const void* g,
+                        float* om_grad, void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

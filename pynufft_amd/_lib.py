@@ -32,6 +32,7 @@ class PlanDesc(ctypes.Structure):
         ('base', _vp),
         ('u_c64', _vp),
         ('u_c128', _vp),
+        ('du_c64', _vp),
         ('u_off', ctypes.c_int64 * MAXD),
         ('csr_c64', _vp),
         ('csr_col', _vp),
@@ -73,6 +74,7 @@ EXPORTS = {
     'pnh_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_selfadjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_forward_grad_om': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, _vp, _vp, _vp]),
 }

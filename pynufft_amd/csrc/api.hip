@@ -62,6 +62,7 @@ InterpArgs interp_args(const pnh_plan* p, int fp64) {
   a.perm = p->d.perm;
   a.base = p->d.base;
   a.u = fp64 ? p->d.u_c128 : p->d.u_c64;
+  a.du = fp64 ? nullptr : p->d.du_c64;
   for (int d = 0; d < PNH_MAXD; ++d) a.u_off[d] = p->d.u_off[d];
   // The stored coefficient table is complex64; FP64 operators always build
   // coefficients from the FP64 factors.
@@ -419,6 +420,14 @@ int pnh_selfadjoint(const pnh_plan* p, const void* x, void* x_out, void* ws, voi
   if (p->M > 0 && (rc = do_k2y(p, 0, grid, ytmp, nullptr, s))) return rc;
   if ((rc = do_y2k(p, 0, ytmp, grid, part, nullptr, s))) return rc;
   return run_passes_inverse(p, grid, x_out, true, 0, nullptr, s);
+}
+
+int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* om_grad,
+                        void* stream) {
+  REQUIRE(p && k && (g || p->M == 0) && (om_grad || p->M == 0), "null argument");
+  REQUIRE(p->d.du_c64 && p->d.u_c64, "plan has no factor derivatives (du_c64)");
+  InterpArgs ia = interp_args(p, 0);
+  return pnh_k2y_dom_launch(ia, 0, k, g, om_grad, (hipStream_t)stream);
 }
 
 int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,

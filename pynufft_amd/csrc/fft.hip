@@ -182,12 +182,10 @@ __device__ __forceinline__ int64_t line_base(const FftSide& s, int64_t o0,
 // in_ and out_ may alias (in-place passes): every input of the block is read
 // into LDS before the first store.
 template <typename T, bool POW2>
-__global__ void __launch_bounds__(256)
-fft_lines_kernel(FftDesc d, const void* in_, void* out_,
-                 const void* __restrict__ tw_, const int* __restrict__ skip) {
+__device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void* out_,
+                                         const void* __restrict__ tw_, int64_t blk,
+                                         unsigned char* smem_raw) {
   typedef typename cplx<T>::t C;
-  if (skip && *skip) return;
-  extern __shared__ unsigned char smem_raw[];
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   const int n = d.n;
   const int ld = n + d.lds_pad;           // padded LDS line length
@@ -197,7 +195,7 @@ fft_lines_kernel(FftDesc d, const void* in_, void* out_,
   const C* __restrict__ tw = (const C*)tw_;
 
   const int LB = d.lines_per_block;
-  const int64_t line0 = (int64_t)blockIdx.x * LB;
+  const int64_t line0 = blk * LB;
   const int64_t n_lines = d.n_outer * d.n_inner;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x;
@@ -268,6 +266,15 @@ fft_lines_kernel(FftDesc d, const void* in_, void* out_,
     v.x *= s; v.y *= s;
     out[line_base(d.dst, o0, o1, b) + (int64_t)t * d.dst.t_stride] = v;
   }
+}
+
+template <typename T, bool POW2>
+__global__ void __launch_bounds__(256)
+fft_lines_kernel(FftDesc d, const void* in_, void* out_,
+                 const void* __restrict__ tw_, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ unsigned char smem_raw[];
+  fft_body<T, POW2>(d, in_, out_, tw_, blockIdx.x, smem_raw);
 }
 
 }  // namespace

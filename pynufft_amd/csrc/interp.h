@@ -16,6 +16,7 @@ struct InterpArgs {
   const int* base;             // [D][M] first support cell per dim
   const void* u;               // per-dim factors (complex, dtype of the op)
   int64_t u_off[PNH_MAXD];     // element offset of dim d's [M][J_d] block
+  const void* du;              // d u_d / d om_d, same layout as u (or NULL)
   const void* csr_val;         // complex64 slice-interleaved table or NULL
   const int* csr_col;          // int32 slice-interleaved table or NULL
 };
@@ -47,6 +48,9 @@ __host__ __device__ inline int pnh_lanes_per_sample(int J_last) {
 
 int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s);
+// out[m*D + d] (+)= sum_c Re(conj(g[m, c]) * sum_j d_d coef(m, j) * grid[cell(m, j), c])
+int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
+                       const void* g, void* out, hipStream_t s);
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);

@@ -65,15 +65,13 @@ __device__ __forceinline__ int64_t csr_index(int64_t r, int g, int jl, int G,
 // shuffle reduction.  JLC == 0: generic, 16 lanes per sample (J_last <= 16).
 // y[perm[r]*nb + c0 + c] = sum_j coef(r, j) * grid[cell(r, j)*nb + c0 + c]
 template <typename T, int D, int JLC, bool CSRV, int NBC>
-__global__ void __launch_bounds__(256)
-k2y_kernel(InterpArgs a, const void* __restrict__ grid_, void* __restrict__ y_,
-           int c0, const int* __restrict__ skip) {
+__device__ __forceinline__ void k2y_body(const InterpArgs& a, int64_t wave,
+                                         const void* __restrict__ grid_,
+                                         void* __restrict__ y_, int c0) {
   typedef typename cplx<T>::t C;
-  if (skip && *skip) return;
   constexpr int LPS = JLC > 0 ? JLC : 16;
   constexpr int SPW = 64 / LPS;
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int sl = lane / LPS;
   const int jl = lane - sl * LPS;
   const int JL = a.J[D - 1];
@@ -158,6 +156,119 @@ k2y_kernel(InterpArgs a, const void* __restrict__ grid_, void* __restrict__ y_,
   }
 }
 
+
+template <typename T, int D, int JLC, bool CSRV, int NBC>
+__global__ void __launch_bounds__(256)
+k2y_kernel(InterpArgs a, const void* __restrict__ grid_, void* __restrict__ y_,
+           int c0, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  k2y_body<T, D, JLC, CSRV, NBC>(a, wave, grid_, y_, c0);
+}
+
+// ------------------------------------------------------------- dy/dom ----
+// Same lane layout as k2y.  For every sample, the D partial derivatives of
+// y with respect to its coordinates, d_d y = sum_j (prod_{a != d} u_a)
+// du_d * grid, contracted with the incoming gradient g as
+// out[m, d] = sum_c Re(conj(g[m, c]) d_d y[m, c]).
+template <typename T, int D, int LPS, int NBC>
+__global__ void __launch_bounds__(256)
+k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
+               const void* __restrict__ g_, T* __restrict__ out, int c0,
+               int accumulate) {
+  typedef typename cplx<T>::t C;
+  constexpr int SPW = 64 / LPS;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int sl = lane / LPS;
+  const int jl = lane - sl * LPS;
+  const int JL = a.J[D - 1];
+  const int64_t r = wave * SPW + sl;
+  const bool valid = sl < SPW && jl < JL && r < a.M;
+  const C* __restrict__ grid = (const C*)grid_;
+  const int nb = a.nb;
+  C acc[D][NBC];
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+#pragma unroll
+    for (int c = 0; c < NBC; ++c) acc[d][c] = mk<T>((T)0, (T)0);
+  if (valid) {
+    const C* __restrict__ u = (const C*)a.u;
+    const C* __restrict__ du = (const C*)a.du;
+    const int J0 = a.J[0];
+    const int J1 = D > 1 ? a.J[1] : 1;
+    const int64_t ol = a.u_off[D - 1] + r * JL + jl;
+    const C ul = u[ol], dul = du[ol];
+    const int64_t cl = wrap_add(a.base[(D - 1) * a.M + r], jl, a.K[D - 1]);
+    if constexpr (D == 1) {
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) acc[0][c] = cmul(dul, grid[cl * nb + c0 + c]);
+    } else {
+      const int b0 = a.base[r];
+      const int64_t o0 = a.u_off[0] + r * J0;
+      if constexpr (D == 2) {
+        for (int j0 = 0; j0 < J0; ++j0) {
+          const int64_t cell = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
+          const C w0 = cmul(du[o0 + j0], ul);
+          const C w1 = cmul(u[o0 + j0], dul);
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) {
+            const C gv = grid[cell * nb + c0 + c];
+            acc[0][c] = cadd(acc[0][c], cmul(w0, gv));
+            acc[D - 1][c] = cadd(acc[D - 1][c], cmul(w1, gv));
+          }
+        }
+      } else {
+        const int b1 = a.base[a.M + r];
+        const int64_t o1 = a.u_off[1] + r * J1;
+        for (int j0 = 0; j0 < J0; ++j0) {
+          const int64_t i0 = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
+          const C u0 = u[o0 + j0], du0 = du[o0 + j0];
+          for (int j1 = 0; j1 < J1; ++j1) {
+            const int64_t cell = i0 + (int64_t)wrap_add(b1, j1, a.K[1]) * a.Kstride[1];
+            const C u1 = u[o1 + j1], du1 = du[o1 + j1];
+            const C w0 = cmul(cmul(du0, u1), ul);
+            const C w1 = cmul(cmul(u0, du1), ul);
+            const C w2 = cmul(cmul(u0, u1), dul);
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) {
+              const C gv = grid[cell * nb + c0 + c];
+              acc[0][c] = cadd(acc[0][c], cmul(w0, gv));
+              acc[1][c] = cadd(acc[1][c], cmul(w1, gv));
+              acc[D - 1][c] = cadd(acc[D - 1][c], cmul(w2, gv));
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < LPS; off <<= 1) {
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) {
+        const C o = shfl_down_c(acc[d][c], off);
+        if (jl + off < LPS) acc[d][c] = cadd(acc[d][c], o);
+      }
+  }
+  if (valid && jl == 0) {
+    const int64_t m = a.perm[r];
+    const C* __restrict__ g = (const C*)g_;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      T s = 0;
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) {
+        const C gv = g[m * nb + c0 + c];
+        s += gv.x * acc[d][c].x + gv.y * acc[d][c].y;
+      }
+      if (accumulate) out[m * D + d] += s;
+      else out[m * D + d] = s;
+    }
+  }
+}
+
 // ---------------------------------------------------------------- y2k ----
 // Support offsets j in [0, J) whose cell (b + j) mod K lies in the tile
 // [T0, T0 + E).  Returns 1 and one interval [lo, lo + n) when the hit set is
@@ -183,36 +294,32 @@ __device__ __forceinline__ void divmod_small(int q, int n, int magic, int& d, in
   m = q - d * n;
 }
 
-// One wavefront per work item (tile id, entry start, entry count, slot).
+// Gridding of one work item (tile id, entry start, entry count, slot) by one
+// wavefront with its own LDS accumulator.
 // Entries are taken in batches: lanes load one entry's row, sample value and
 // first cells each (and stage its per-axis factors in LDS); the entries of
 // the batch are then applied one after another, lanes spread over the
 // support cells that fall inside the tile.
 template <typename T, int D, bool CSRV, int NBC>
-__global__ void __launch_bounds__(256)
-y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
-                  void* __restrict__ grid_, void* __restrict__ part_, int c0,
-                  const int* __restrict__ skip) {
+__device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
+                                         const int4 it, unsigned char* wave_lds,
+                                         const void* __restrict__ y_,
+                                         void* __restrict__ grid_,
+                                         void* __restrict__ part_, int c0) {
   typedef typename cplx<T>::t C;
-  if (skip && *skip) return;
-  extern __shared__ unsigned char smem_raw[];
-  const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (item >= b.n_items) return;  // whole wave leaves together
   const int cells = b.tile_cells;
   const int J0 = a.J[0];
   const int J1 = D > 1 ? a.J[1] : 1;
   const int J2 = D > 2 ? a.J[2] : 1;
   const int SJ = J0 + (D > 1 ? J1 : 0) + (D > 2 ? J2 : 0);
-  T* acc = reinterpret_cast<T*>(smem_raw + (size_t)wave * b.wave_lds_bytes);
+  T* acc = reinterpret_cast<T*>(wave_lds);
   C* stage = reinterpret_cast<C*>(acc + (size_t)cells * 2 * NBC);
   int* srow = reinterpret_cast<int*>(stage + (size_t)b.batch * SJ);
   const C* __restrict__ y = (const C*)y_;
   const C* __restrict__ u = (const C*)a.u;
   const int nb = a.nb;
 
-  const int4 it = reinterpret_cast<const int4*>(b.items)[item];
   int tid = it.x;
   int T0[PNH_MAXD], E[PNH_MAXD];
 #pragma unroll
@@ -395,6 +502,22 @@ y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
   }
 }
 
+// One wavefront per work item (tile id, entry start, entry count, slot).
+template <typename T, int D, bool CSRV, int NBC>
+__global__ void __launch_bounds__(256)
+y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
+                  void* __restrict__ grid_, void* __restrict__ part_, int c0,
+                  const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ unsigned char smem_raw[];
+  const int wave = threadIdx.x >> 6;
+  const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (item >= b.n_items) return;  // whole wave leaves together
+  const int4 it = reinterpret_cast<const int4*>(b.items)[item];
+  y2k_body<T, D, CSRV, NBC>(a, b, it, smem_raw + (size_t)wave * b.wave_lds_bytes,
+                            y_, grid_, part_, c0);
+}
+
 // Sum the partial tiles of every split tile in chunk order.
 template <typename T, int D, int NBC>
 __global__ void __launch_bounds__(256)
@@ -538,6 +661,45 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b0, const void* y,
   return 0;
 }
 
+
+template <typename T, int D, int LPS>
+int launch_dom_t(const InterpArgs& a, const void* grid, const void* g, void* out,
+                 hipStream_t s) {
+  constexpr int SPW = 64 / LPS;
+  const int64_t waves = (a.M + SPW - 1) / SPW;
+  const int64_t nblk = (waves + 3) / 4;
+  for (int c0 = 0; c0 < a.nb;) {
+    if (a.nb - c0 >= 4) {
+      hipLaunchKernelGGL((k2y_dom_kernel<T, D, LPS, 4>), dim3((unsigned)nblk), dim3(256), 0,
+                         s, a, grid, g, (T*)out, c0, c0 > 0);
+      c0 += 4;
+    } else {
+      hipLaunchKernelGGL((k2y_dom_kernel<T, D, LPS, 1>), dim3((unsigned)nblk), dim3(256), 0,
+                         s, a, grid, g, (T*)out, c0, c0 > 0);
+      c0 += 1;
+    }
+    PNH_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+template <typename T, int D>
+int launch_dom_j(const InterpArgs& a, const void* grid, const void* g, void* out,
+                 hipStream_t s) {
+  switch (pnh_lanes_per_sample(a.J[D - 1])) {
+    case 1: return launch_dom_t<T, D, 1>(a, grid, g, out, s);
+    case 2: return launch_dom_t<T, D, 2>(a, grid, g, out, s);
+    case 3: return launch_dom_t<T, D, 3>(a, grid, g, out, s);
+    case 4: return launch_dom_t<T, D, 4>(a, grid, g, out, s);
+    case 6: return launch_dom_t<T, D, 6>(a, grid, g, out, s);
+    case 8: return launch_dom_t<T, D, 8>(a, grid, g, out, s);
+    case 16: return launch_dom_t<T, D, 16>(a, grid, g, out, s);
+    default:
+      pnh_set_error("interpolator size %d on the last axis is above 16", a.J[D - 1]);
+      return -2;
+  }
+}
+
 }  // namespace
 
 int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
@@ -575,6 +737,27 @@ int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
   }
 #undef Y2K_CASE
 #undef Y2K_CASE64
+  pnh_set_error("unsupported ndim %d", a.D);
+  return -2;
+}
+
+int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
+                       const void* g, void* out, hipStream_t s) {
+  if (!a.du) { pnh_set_error("plan has no factor derivatives"); return -1; }
+  if (a.M == 0) return 0;
+  if (fp64) {
+    switch (a.D) {
+      case 1: return launch_dom_j<double, 1>(a, grid, g, out, s);
+      case 2: return launch_dom_j<double, 2>(a, grid, g, out, s);
+      case 3: return launch_dom_j<double, 3>(a, grid, g, out, s);
+    }
+  } else {
+    switch (a.D) {
+      case 1: return launch_dom_j<float, 1>(a, grid, g, out, s);
+      case 2: return launch_dom_j<float, 2>(a, grid, g, out, s);
+      case 3: return launch_dom_j<float, 3>(a, grid, g, out, s);
+    }
+  }
   pnh_set_error("unsupported ndim %d", a.D);
   return -2;
 }

@@ -126,7 +126,16 @@ def overlap_pinv(N, J, K, alpha, beta):
     return scipy.linalg.pinv(acc)
 
 
-def _minmax_chunk(o, N, J, K, alpha, beta, Tinv):
+def _dsinc(x):
+    """d/dx sin(pi x)/(pi x) = (cos(pi x) - sinc(x)) / x, 0 at x = 0."""
+    out = np.zeros_like(x)
+    nz = x != 0
+    xn = x[nz]
+    out[nz] = (np.cos(np.pi * xn) - np.sinc(xn)) / xn
+    return out
+
+
+def _minmax_chunk(o, N, J, K, alpha, beta, Tinv, grad=False):
     gam = 2.0 * np.pi / (K * 1.0)
     L = np.size(alpha) - 1
     ratio = 1.0 * K / N
@@ -136,28 +145,44 @@ def _minmax_chunk(o, N, J, K, alpha, beta, Tinv):
     for l in range(-L, L + 1):
         r = r + (alpha[abs(l)] * 1.0) * np.sinc((arg + 1.0 * l * beta) / ratio)
     c = Tinv.dot(r)
+    ph = np.exp(-1.0j * o * N / 2.0) * np.exp(1.0j * gam * (N * 1.0 - 1.0) / 2.0 * arg)
     v = np.exp(1.0j * gam * (N * 1.0 - 1.0) / 2.0 * arg) * c
     v = np.exp(-1.0j * o * N / 2.0) * v
-    return v.T.conj(), np.mod(koff.astype(np.int64) + 1, K)
+    base = np.mod(koff.astype(np.int64) + 1, K)
+    if not grad:
+        return v.T.conj(), base
+    # d/d(om): arg moves by 1/gam (koff is piecewise constant), the two
+    # phases contribute i(N-1)/2 - iN/2 = -i/2, and dr = sum_l alpha sinc'.
+    dr = np.zeros(arg.shape)
+    for l in range(-L, L + 1):
+        dr = dr + (alpha[abs(l)] * 1.0) * _dsinc((arg + 1.0 * l * beta) / ratio)
+    dr = dr / (gam * ratio)
+    dv = ph * (Tinv.dot(dr) - 0.5j * c)
+    return v.T.conj(), base, dv.T.conj()
 
 
-def minmax_axis(om, N, J, K, alpha, beta, Tinv, chunk=1 << 16, pool=None):
+def minmax_axis(om, N, J, K, alpha, beta, Tinv, chunk=1 << 16, pool=None,
+                grad=False):
     """Per-sample min-max factors along one axis.
 
     Returns u (M, J) complex128 -- the reference's ``ud[d]`` (helper.py:606-618
     with OMEGA_u :148-162) -- and base (M,) int64, the first support cell
-    mod(floor(om/gam - J/2) + 1, K) (OMEGA_k :164-210).  Sample chunks are
-    independent and run on ``pool`` (numpy releases the GIL); the arithmetic
-    per sample is the same as unchunked.
+    mod(floor(om/gam - J/2) + 1, K) (OMEGA_k :164-210); with ``grad`` also
+    du/d(om) (M, J) complex128.  Sample chunks are independent and run on
+    ``pool`` (numpy releases the GIL); the arithmetic per sample is the same
+    as unchunked.
     """
     M = om.shape[0]
     u = np.empty((M, J), dtype=np.complex128)
     base = np.empty(M, dtype=np.int64)
+    du = np.empty((M, J), dtype=np.complex128) if grad else None
 
     def job(s):
-        uu, bb = _minmax_chunk(om[s:s + chunk], N, J, K, alpha, beta, Tinv)
-        u[s:s + chunk] = uu
-        base[s:s + chunk] = bb
+        res = _minmax_chunk(om[s:s + chunk], N, J, K, alpha, beta, Tinv, grad)
+        u[s:s + chunk] = res[0]
+        base[s:s + chunk] = res[1]
+        if grad:
+            du[s:s + chunk] = res[2]
 
     starts = range(0, M, chunk)
     if pool is None:
@@ -165,7 +190,23 @@ def minmax_axis(om, N, J, K, alpha, beta, Tinv, chunk=1 << 16, pool=None):
             job(s)
     else:
         list(pool.map(job, starts))
-    return u, base
+    return (u, base, du) if grad else (u, base)
+
+
+def factor_gradients(st):
+    """d u_d / d om_d (M, J_d) complex128 for every axis of a plan."""
+    om = st['om']
+    out = []
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for a in range(len(st['Nd'])):
+            N, J, K = st['Nd'][a], st['Jd'][a], st['Kd'][a]
+            alpha, beta = st['alpha'][a], st['beta'][a]
+            Tinv = overlap_pinv(N, J, K, alpha, beta)
+            res = minmax_axis(om[:, a], N, J, K, alpha, beta, Tinv, pool=pool,
+                              grad=True)
+            out.append(res[2])
+    return out
 
 
 def tile_shape(Kd):

@@ -152,6 +152,18 @@ class NUFFT:
         self._create()
         return 0
 
+    def _ensure_du(self):
+        """Upload d u_d / d om_d (FP64 on the host, stored complex64)."""
+        if 'du_c64' not in self._t:
+            st = self._host
+            perm = st['perm']
+            du = helper.factor_gradients(st)
+            flat = np.concatenate([np.ascontiguousarray(du[a][perm]).ravel()
+                                   for a in range(self.ndims)]) if self.M[0] else \
+                np.zeros(1, np.complex128)
+            self._t['du_c64'] = torch.from_numpy(flat.astype(np.complex64)).to(self.device)
+            self._create()
+
     def _ensure_fp64(self):
         if 'u_c128' not in self._t:
             self._t['u_c128'] = torch.from_numpy(self._u128_host).to(self.device)
@@ -215,6 +227,7 @@ class NUFFT:
         d.base = t['base'].data_ptr()
         d.u_c64 = t['u_c64'].data_ptr()
         d.u_c128 = t['u_c128'].data_ptr() if 'u_c128' in t else None
+        d.du_c64 = t['du_c64'].data_ptr() if 'du_c64' in t else None
         d.csr_c64 = t['csr_val'].data_ptr() if 'csr_val' in t else None
         d.csr_col = t['csr_col'].data_ptr() if 'csr_col' in t else None
         d.sn_f32 = t['sn_f32'].data_ptr()
@@ -382,6 +395,22 @@ class NUFFT:
         out = self._empty('k', kt.dtype)
         ws = self._workspace(_lib.OP_K2Y2K, dt)
         self._call(self._libh.pnh_k2y2k, dt, _ptr(kt), _ptr(out), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def forward_grad_om(self, k, g):
+        """d(loss)/d(om) for y = k2y(k): Re(conj(g) dy/dom) summed over coils.
+
+        ``k`` is the oversampled spectrum ``xx2k(x2xx(x))``, ``g`` the incoming
+        gradient dL/d conj(y) (torch's convention).  Returns (M, ndim) float32.
+        """
+        self._need_plan()
+        self._ensure_du()
+        kt, is_np = self._in(k, 'k')
+        gt, _ = self._in(g, 'y')
+        out = torch.empty((self.M[0], self.ndims), dtype=torch.float32,
+                          device=self.device)
+        self._call(self._libh.pnh_forward_grad_om, _ptr(kt), _ptr(gt), _ptr(out),
                    self._stream())
         return self._out(out, is_np)
 
