@@ -105,6 +105,46 @@ def cpu_baseline(w, m_sample, seed=7):
     }
 
 
+def bench_batched(torch, W, maxiter=20, reps=3):
+    """Batched uneven 2-D CG (64 problems) vs looping the plain operator."""
+    from pynufft_amd.batch import BatchNUFFT
+    ps = W.batched2d()
+    t0 = time.perf_counter()
+    B = BatchNUFFT()
+    B.plan(ps)
+    torch.cuda.synchronize()
+    t_plan = time.perf_counter() - t0
+    xs = [torch.from_numpy(p['x']).to(B.device) for p in ps]
+    y = B.forward(xs)
+    ys = B.unpack_samples(y)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps, out
+
+    t_plain, plain = timed(lambda: [op.solve(ys[i], 'cg', maxiter=maxiter, rtol=0.0)
+                                    for i, op in enumerate(B.ops)])
+    t_batch, xb = timed(lambda: B.solve(y, maxiter=maxiter, rtol=0.0))
+    t_graph, xg = timed(lambda: B.solve_graph(y, maxiter=maxiter, rtol=0.0))
+    xgl = B.unpack_images(xg)
+    diff = max(float(torch.linalg.vector_norm(xgl[i] - plain[i]) /
+                     torch.linalg.vector_norm(plain[i])) for i in range(B.n))
+    n = B.n
+    return {
+        'problems': n, 'cg_iterations': maxiter, 'samples_total': int(B.n_samples),
+        'plain_loop_solves_per_s': n / t_plain,
+        'batched_solves_per_s': n / t_batch,
+        'batched_graph_solves_per_s': n / t_graph,
+        'speedup_graph_vs_plain': t_plain / t_graph,
+        'max_rel_diff_vs_plain': diff, 'plan_s': t_plan,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -116,6 +156,8 @@ def main():
                     help='full coefficient table (CSR) instead of factors')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample', type=int, default=131072)
+    ap.add_argument('--no-batched', action='store_true',
+                    help='skip the batched uneven-problem CG comparison')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='after timing, copy this many MiB device-to-device '
                          '(a known-size kernel to calibrate HBM byte counters)')
@@ -184,6 +226,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    batched = None
+    if not args.no_batched:
+        del A, k, y, x
+        torch.cuda.empty_cache()
+        batched = bench_batched(torch, W)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(w, min(args.cpu_sample, M))
@@ -206,6 +254,7 @@ def main():
         'pair_hbm_frac': pair_bytes * value / 1e9 / HBM_PEAK_GBS,
         'pair_algorithmic_bytes': pair_bytes,
         'plan_s': t_plan,
+        'batched_cg': batched,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -157,6 +157,40 @@ int pnh_solve_cg(const pnh_plan* plan, const void* y, void* x, int maxiter,
                  double rtol, double atol, void* ws, int32_t* iters_out,
                  void* stream);
 
+/* ---- many independent problems (uneven Nd, Kd, M; same ndim and Jd) ----
+ * The reference handles one problem per NUFFT object; a batch here runs every
+ * stage of all problems in one launch.  Vectors are packed: problem p's image,
+ * samples and grid follow problem p-1's (complex64 for forward/adjoint and the
+ * solve's y/x).  Each problem is described by the same pnh_plan_desc as a
+ * single plan (batch == 1, factor format, complex64 and complex128 factors);
+ * the per-problem and per-stage work tables live in `table`, a device buffer
+ * of pnh_batch_table_bytes() bytes written once by pnh_batch_create (which
+ * synchronizes `stream`).  No counterpart in the reference (new). */
+typedef struct pnh_batch pnh_batch;
+typedef struct pnh_graph pnh_graph;
+int pnh_batch_table_bytes(const pnh_plan_desc* descs, int n, size_t* bytes);
+int pnh_batch_create(const pnh_plan_desc* descs, int n, void* table, void* stream,
+                     pnh_batch** out);
+int pnh_batch_destroy(pnh_batch* batch);
+/* Packed totals: sum of prod(Nd), of M and of prod(Kd). */
+int pnh_batch_sizes(const pnh_batch* batch, int64_t* n_image, int64_t* n_samples,
+                    int64_t* n_grid);
+/* Scratch for PNH_OP_FORWARD, PNH_OP_ADJOINT or PNH_OP_SOLVE_CG. */
+int pnh_batch_workspace_bytes(const pnh_batch* batch, int op, int dtype, size_t* bytes);
+/* Batched forward / adjoint (complex64), one launch per stage for all problems. */
+int pnh_batch_forward(const pnh_batch* batch, const void* x, void* y, void* ws, void* stream);
+int pnh_batch_adjoint(const pnh_batch* batch, const void* y, void* x, void* ws, void* stream);
+/* Batched FP64 CG, semantics of pnh_solve_cg per problem (iters_out: int32[n]). */
+int pnh_batch_solve_cg(const pnh_batch* batch, const void* y, void* x, int maxiter,
+                       double rtol, double atol, void* ws, int32_t* iters_out, void* stream);
+/* Capture pnh_batch_solve_cg with these buffers into a HIP graph, and replay
+ * it on any stream; the buffers must outlive the graph. */
+int pnh_batch_cg_graph_create(const pnh_batch* batch, const void* y, void* x, int maxiter,
+                              double rtol, double atol, void* ws, int32_t* iters_out,
+                              pnh_graph** out);
+int pnh_graph_launch(pnh_graph* graph, void* stream);
+int pnh_graph_destroy(pnh_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

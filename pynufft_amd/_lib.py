@@ -77,6 +77,25 @@ EXPORTS = {
     'pnh_forward_grad_om': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, _vp, _vp, _vp]),
+    'pnh_batch_table_bytes': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_batch_create': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int, _vp, _vp,
+                                        ctypes.POINTER(_vp)]),
+    'pnh_batch_destroy': (ctypes.c_int, [_vp]),
+    'pnh_batch_sizes': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64)]),
+    'pnh_batch_workspace_bytes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_batch_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_batch_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    'pnh_batch_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, _vp, _vp, _vp]),
+    'pnh_batch_cg_graph_create': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, _vp, _vp,
+                                                 ctypes.POINTER(_vp)]),
+    'pnh_graph_launch': (ctypes.c_int, [_vp, _vp]),
+    'pnh_graph_destroy': (ctypes.c_int, [_vp]),
 }
 
 

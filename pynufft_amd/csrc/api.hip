@@ -222,6 +222,60 @@ int do_k2y(const pnh_plan* p, int fp64, const void* k, void* y, const int* skip,
   return pnh_k2y_launch(ia, fp64, k, y, skip, s);
 }
 
+
+// Validate a descriptor and fill a host-side plan (no device work).
+int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
+  REQUIRE(d.ndim >= 1 && d.ndim <= PNH_MAXD, "ndim must be 1..%d (got %d)", PNH_MAXD, d.ndim);
+  REQUIRE(d.batch >= 1, "batch must be >= 1");
+  REQUIRE(d.M >= 0 && d.M < (1LL << 31), "M out of range");
+  REQUIRE(d.perm && d.base, "perm/base pointers are required");
+  REQUIRE(d.u_c64 || d.u_c128 || d.csr_c64, "no interpolator data");
+  REQUIRE(!d.csr_c64 || d.csr_col, "csr_col required with csr_c64");
+  REQUIRE(d.sn_f32, "sn_f32 is required");
+  memset(p, 0, sizeof(*p));
+  p->d = d;
+  p->D = d.ndim;
+  p->nb = d.batch;
+  p->M = d.M;
+  p->Ntot = 1;
+  p->Ktot = 1;
+  int64_t cells = 1;
+  for (int a = 0; a < p->D; ++a) {
+    p->N[a] = d.Nd[a];
+    p->K[a] = d.Kd[a];
+    p->J[a] = d.Jd[a];
+    REQUIRE(d.Nd[a] >= 1 && d.Kd[a] >= d.Nd[a] && d.Kd[a] < (1LL << 30) && d.Jd[a] >= 1 &&
+                d.Jd[a] <= d.Kd[a] && d.tile[a] >= 1,
+            "bad geometry on axis %d: Nd=%lld Kd=%lld Jd=%d tile=%d", a,
+            (long long)d.Nd[a], (long long)d.Kd[a], d.Jd[a], d.tile[a]);
+    p->Ntot *= d.Nd[a];
+    p->Ktot *= d.Kd[a];
+    p->bins.tile[a] = d.tile[a];
+    p->bins.tgrid[a] = (int)((d.Kd[a] + d.tile[a] - 1) / d.tile[a]);
+    cells *= d.tile[a];
+    int radix[PNH_FFT_MAX_PASS], ns[PNH_FFT_MAX_PASS], np;
+    if (pnh_fft_factor((int)d.Kd[a], radix, ns, &np)) return -2;
+    REQUIRE(d.tw_c64[a], "tw_c64[%d] missing", a);
+  }
+  REQUIRE(pnh_lanes_per_sample(d.Jd[d.ndim - 1]) > 0, "Jd above 16 on the last axis");
+  p->bins.tile_cells = (int)cells;
+  p->bins.waves_per_block = 4;
+  p->bins.lds_budget = kLdsBudget;
+  if ((size_t)cells * 2 * sizeof(double) > kLdsBudget) {
+    pnh_set_error("gridding tile of %lld cells exceeds the LDS budget", (long long)cells);
+    return -2;
+  }
+  p->bins.n_items = d.n_items;
+  p->bins.items = d.items;
+  p->bins.bin_rows = d.bin_rows;
+  p->bins.n_merge = d.n_merge;
+  p->bins.merge = d.merge;
+  p->slot_nbc = p->nb >= 4 ? 4 : (p->nb >= 2 ? 2 : 1);
+  REQUIRE(d.n_items > 0 && d.items && (d.bin_rows || d.M == 0),
+          "gridding work items are required");
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -243,63 +297,10 @@ int pnh_device_count(void) {
 int pnh_plan_create(const pnh_plan_desc* desc, pnh_plan** out) {
   REQUIRE(desc && out, "null argument");
   *out = nullptr;
-  const pnh_plan_desc& d = *desc;
-  REQUIRE(d.ndim >= 1 && d.ndim <= PNH_MAXD, "ndim must be 1..%d (got %d)", PNH_MAXD, d.ndim);
-  REQUIRE(d.batch >= 1, "batch must be >= 1");
-  REQUIRE(d.M >= 0 && d.M < (1LL << 31), "M out of range");
-  REQUIRE(d.perm && d.base, "perm/base pointers are required");
-  REQUIRE(d.u_c64 || d.u_c128 || d.csr_c64, "no interpolator data");
-  REQUIRE(!d.csr_c64 || d.csr_col, "csr_col required with csr_c64");
-  REQUIRE(d.sn_f32, "sn_f32 is required");
   pnh_plan* p = new (std::nothrow) pnh_plan;
   if (!p) { pnh_set_error("out of host memory"); return -1; }
-  memset(p, 0, sizeof(*p));
-  p->d = d;
-  p->D = d.ndim;
-  p->nb = d.batch;
-  p->M = d.M;
-  p->Ntot = 1;
-  p->Ktot = 1;
-  int64_t cells = 1;
-  for (int a = 0; a < p->D; ++a) {
-    p->N[a] = d.Nd[a];
-    p->K[a] = d.Kd[a];
-    p->J[a] = d.Jd[a];
-    if (d.Nd[a] < 1 || d.Kd[a] < d.Nd[a] || d.Kd[a] >= (1LL << 30) || d.Jd[a] < 1 ||
-        d.Jd[a] > d.Kd[a] || d.tile[a] < 1) {
-      pnh_set_error("bad geometry on axis %d: Nd=%lld Kd=%lld Jd=%d tile=%d", a,
-                    (long long)d.Nd[a], (long long)d.Kd[a], d.Jd[a], d.tile[a]);
-      delete p;
-      return -1;
-    }
-    p->Ntot *= d.Nd[a];
-    p->Ktot *= d.Kd[a];
-    p->bins.tile[a] = d.tile[a];
-    p->bins.tgrid[a] = (int)((d.Kd[a] + d.tile[a] - 1) / d.tile[a]);
-    cells *= d.tile[a];
-    int radix[PNH_FFT_MAX_PASS], ns[PNH_FFT_MAX_PASS], np;
-    if (pnh_fft_factor((int)d.Kd[a], radix, ns, &np)) { delete p; return -2; }
-    if (!d.tw_c64[a]) { pnh_set_error("tw_c64[%d] missing", a); delete p; return -1; }
-  }
-  p->bins.tile_cells = (int)cells;
-  p->bins.waves_per_block = 4;
-  p->bins.lds_budget = kLdsBudget;
-  while (p->bins.waves_per_block > 1 &&
-         (size_t)p->bins.waves_per_block * cells * 2 * sizeof(double) > kLdsBudget)
-    p->bins.waves_per_block >>= 1;
-  if ((size_t)cells * 2 * sizeof(double) > kLdsBudget) {
-    pnh_set_error("gridding tile of %lld cells exceeds the LDS budget", (long long)cells);
-    delete p;
-    return -2;
-  }
-  p->bins.n_items = d.n_items;
-  p->bins.items = d.items;
-  p->bins.bin_rows = d.bin_rows;
-  p->bins.n_merge = d.n_merge;
-  p->bins.merge = d.merge;
-  p->slot_nbc = p->nb >= 4 ? 4 : (p->nb >= 2 ? 2 : 1);
-  REQUIRE(d.n_items > 0 && d.items && (d.bin_rows || d.M == 0),
-          "gridding work items are required");
+  const int rc = plan_init(*desc, p);
+  if (rc) { delete p; return rc; }
   *out = p;
   return 0;
 }
@@ -479,6 +480,474 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
   if (iters_out)
     PNH_CHECK_HIP(hipMemcpyAsync(iters_out, sc->iters, sizeof(int) * nb,
                                  hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+}  // extern "C"
+
+// ================================================================ batches ==
+#include <algorithm>
+#include <vector>
+
+#include "batch.h"
+
+struct pnh_batch {
+  int n, D, J_last, SJ, tile_cells;
+  int64_t N_total, M_total, K_total, slots_total;
+  int fft_threads;
+  BatchProblem* probs;                 // device (inside the caller's table)
+  BatchTable k2y_t;
+  const int2* items;
+  int64_t n_items;
+  const int2* merges;
+  int64_t n_merge;
+  BatchTable fft_t[3][PNH_MAXD];       // [kind][axis]
+  size_t fft_lds[3][PNH_MAXD];
+  int fft_pow2[3][PNH_MAXD];
+  BatchTable vec_t[3];                 // [0] grid cells, [1] samples, [2] image
+  int64_t vec_chunk;
+  size_t table_bytes;
+};
+
+struct pnh_graph {
+  hipGraphExec_t exec;
+};
+
+namespace {
+
+const int64_t kVecChunk = 4096;
+
+struct Blob {
+  std::vector<unsigned char> bytes;
+  size_t add(const void* src, size_t n) {
+    const size_t off = (bytes.size() + 255) & ~(size_t)255;
+    bytes.resize(off + n);
+    if (n) memcpy(bytes.data() + off, src, n);
+    return off;
+  }
+};
+
+void add_table(Blob& blob, const std::vector<int64_t>& blocks_per_problem,
+               size_t* off_tab, size_t* off_first, int64_t* n_blocks) {
+  std::vector<int> tab, first;
+  int64_t acc = 0;
+  for (size_t p = 0; p < blocks_per_problem.size(); ++p) {
+    first.push_back((int)acc);
+    for (int64_t b = 0; b < blocks_per_problem[p]; ++b) tab.push_back((int)p);
+    acc += blocks_per_problem[p];
+  }
+  first.push_back((int)acc);
+  if (tab.empty()) tab.push_back(0);
+  *off_tab = blob.add(tab.data(), tab.size() * sizeof(int));
+  *off_first = blob.add(first.data(), first.size() * sizeof(int));
+  *n_blocks = acc;
+}
+
+size_t y2k_wave_lds(int cells, int SJ, int fp64, int batch_entries) {
+  const size_t esz = fp64 ? 8 : 4;
+  size_t wl = (size_t)cells * 2 * esz + (size_t)batch_entries * SJ * 2 * esz +
+              (size_t)batch_entries * sizeof(int);
+  return (wl + 15) & ~(size_t)15;
+}
+
+// Build every host-side table of a batch.  With `table` NULL only sizes are
+// computed; otherwise the blob is copied to `table` and device pointers set.
+int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stream,
+                pnh_batch* b) {
+  REQUIRE(descs && n >= 1, "need at least one problem");
+  memset(b, 0, sizeof(*b));
+  std::vector<pnh_plan> plans(n);
+  for (int i = 0; i < n; ++i) {
+    int rc = plan_init(descs[i], &plans[i]);
+    if (rc) return rc;
+    const pnh_plan& p = plans[i];
+    REQUIRE(p.nb == 1, "batched problems are single-coil (problem %d has batch %d)", i, p.nb);
+    REQUIRE(p.d.u_c64 && p.d.u_c128 && p.d.tw_c128[0], "problem %d lacks c64/c128 factors", i);
+    REQUIRE(!p.d.csr_c64, "batched problems use the per-axis factors");
+    if (i > 0) {
+      REQUIRE(p.D == plans[0].D, "all problems must have the same ndim");
+      for (int a = 0; a < p.D; ++a) {
+        REQUIRE(p.J[a] == plans[0].J[a], "all problems must have the same Jd");
+        REQUIRE(p.bins.tile[a] == plans[0].bins.tile[a], "all problems must share the tile shape");
+      }
+    }
+  }
+  const int D = plans[0].D;
+  b->n = n;
+  b->D = D;
+  b->J_last = plans[0].J[D - 1];
+  b->SJ = 0;
+  for (int a = 0; a < D; ++a) b->SJ += plans[0].J[a];
+  b->tile_cells = plans[0].bins.tile_cells;
+  b->fft_threads = 256;
+  b->vec_chunk = kVecChunk;
+  std::vector<BatchProblem> probs(n);
+  int64_t xo = 0, yo = 0, ko = 0, so = 0;
+  for (int i = 0; i < n; ++i) {
+    const pnh_plan& p = plans[i];
+    BatchProblem& q = probs[i];
+    memset(&q, 0, sizeof(q));
+    q.ia = interp_args(&p, 0);
+    q.ia64 = interp_args(&p, 1);
+    q.bins = p.bins;
+    q.x_off = xo; q.y_off = yo; q.k_off = ko; q.part_off = so;
+    q.n_img = p.Ntot;
+    q.n_grid = p.Ktot;
+    q.sn32 = p.d.sn_f32;
+    for (int a = 0; a < D; ++a) {
+      int rc = build_pass(&p, a, false, a == D - 1, false, a == D - 1 ? 1 : 0, 0, 1.0, &q.fwd[a]);
+      const double sc = a == D - 1 ? 1.0 / (double)p.Ktot : 1.0;
+      if (!rc) rc = build_pass(&p, a, true, false, a == D - 1, a == D - 1 ? 2 : 0, 0, sc, &q.inv[a]);
+      if (!rc) rc = build_pass(&p, a, true, false, a == D - 1, 0, 1, sc, &q.inv64[a]);
+      if (rc) return rc;
+      q.tw32[a] = p.d.tw_c64[a];
+      q.tw64[a] = p.d.tw_c128[a];
+    }
+    xo += p.Ntot; yo += p.M; ko += p.Ktot; so += p.d.n_slots;
+  }
+  b->N_total = xo; b->M_total = yo; b->K_total = ko; b->slots_total = so;
+
+  Blob blob;
+  const size_t off_probs = blob.add(probs.data(), probs.size() * sizeof(BatchProblem));
+  // k2y: 4 wavefronts of SPW samples per block
+  const int SPW = 64 / pnh_lanes_per_sample(b->J_last);
+  std::vector<int64_t> nb_k2y(n);
+  for (int i = 0; i < n; ++i) nb_k2y[i] = ((plans[i].M + SPW - 1) / SPW + 3) / 4;
+  size_t ok_tab, ok_first;
+  add_table(blob, nb_k2y, &ok_tab, &ok_first, &b->k2y_t.n_blocks);
+  // FFT passes per kind and axis
+  size_t of_tab[3][PNH_MAXD], of_first[3][PNH_MAXD];
+  for (int kind = 0; kind < 3; ++kind)
+    for (int a = 0; a < D; ++a) {
+      std::vector<int64_t> nbk(n);
+      size_t lds = 0;
+      int pow2 = 1;
+      for (int i = 0; i < n; ++i) {
+        FftDesc& f = kind == BFFT_FWD ? probs[i].fwd[a]
+                                      : (kind == BFFT_INV ? probs[i].inv[a] : probs[i].inv64[a]);
+        const int64_t lines = f.n_outer * f.n_inner;
+        nbk[i] = (lines + f.lines_per_block - 1) / f.lines_per_block;
+        const size_t esz = kind == BFFT_INV64 ? 16 : 8;
+        lds = std::max(lds, 2 * (size_t)f.lines_per_block * (f.n + f.lds_pad) * esz);
+        for (int q = 0; q < f.npass; ++q)
+          pow2 &= (f.radix[q] == 2 || f.radix[q] == 4 || f.radix[q] == 8);
+      }
+      add_table(blob, nbk, &of_tab[kind][a], &of_first[kind][a], &b->fft_t[kind][a].n_blocks);
+      b->fft_lds[kind][a] = lds;
+      b->fft_pow2[kind][a] = pow2;
+    }
+  // vector tables: grid cells, samples, image
+  size_t ov_tab[3], ov_first[3];
+  for (int e = 0; e < 3; ++e) {
+    std::vector<int64_t> nbv(n);
+    for (int i = 0; i < n; ++i) {
+      const int64_t len = e == 0 ? plans[i].Ktot : (e == 1 ? plans[i].M : plans[i].Ntot);
+      nbv[i] = (len + kVecChunk - 1) / kVecChunk;
+    }
+    add_table(blob, nbv, &ov_tab[e], &ov_first[e], &b->vec_t[e].n_blocks);
+  }
+  // gridding items of all problems, heaviest first (needs the counts)
+  std::vector<std::pair<int, int2>> all;  // (count, (problem, item))
+  std::vector<int2> merges;
+  for (int i = 0; i < n; ++i) {
+    std::vector<int4> it(plans[i].d.n_items);
+    if (table) {
+      PNH_CHECK_HIP(hipMemcpy(it.data(), plans[i].d.items, it.size() * sizeof(int4),
+                              hipMemcpyDeviceToHost));
+    }
+    for (int64_t k = 0; k < plans[i].d.n_items; ++k)
+      all.push_back(std::make_pair(table ? it[k].z : 0, make_int2(i, (int)k)));
+    for (int64_t k = 0; k < plans[i].d.n_merge; ++k) merges.push_back(make_int2(i, (int)k));
+  }
+  std::stable_sort(all.begin(), all.end(), [](const std::pair<int, int2>& x,
+                                              const std::pair<int, int2>& y) {
+    return x.first > y.first;
+  });
+  std::vector<int2> items(all.size());
+  for (size_t k = 0; k < all.size(); ++k) items[k] = all[k].second;
+  if (items.empty()) items.push_back(make_int2(0, 0));
+  if (merges.empty()) merges.push_back(make_int2(0, 0));
+  const size_t oi = blob.add(items.data(), items.size() * sizeof(int2));
+  const size_t om = blob.add(merges.data(), merges.size() * sizeof(int2));
+  b->n_items = (int64_t)all.size();
+  int64_t nm = 0;
+  for (int i = 0; i < n; ++i) nm += plans[i].d.n_merge;
+  b->n_merge = nm;
+  b->table_bytes = blob.bytes.size();
+  if (!table) return 0;
+
+  unsigned char* base = (unsigned char*)table;
+  PNH_CHECK_HIP(hipMemcpyAsync(base, blob.bytes.data(), blob.bytes.size(),
+                               hipMemcpyHostToDevice, stream));
+  PNH_CHECK_HIP(hipStreamSynchronize(stream));
+  b->probs = (BatchProblem*)(base + off_probs);
+  b->k2y_t.tab = (const int*)(base + ok_tab);
+  b->k2y_t.first = (const int*)(base + ok_first);
+  for (int kind = 0; kind < 3; ++kind)
+    for (int a = 0; a < D; ++a) {
+      b->fft_t[kind][a].tab = (const int*)(base + of_tab[kind][a]);
+      b->fft_t[kind][a].first = (const int*)(base + of_first[kind][a]);
+    }
+  for (int e = 0; e < 3; ++e) {
+    b->vec_t[e].tab = (const int*)(base + ov_tab[e]);
+    b->vec_t[e].first = (const int*)(base + ov_first[e]);
+  }
+  b->items = (const int2*)(base + oi);
+  b->merges = (const int2*)(base + om);
+  return 0;
+}
+
+struct BatchY2kCfg {
+  int wave_lds, batch_entries, waves;
+};
+
+BatchY2kCfg y2k_cfg(const pnh_batch* b, int fp64) {
+  BatchY2kCfg c;
+  c.batch_entries = 32;
+  size_t wl = y2k_wave_lds(b->tile_cells, b->SJ, fp64, c.batch_entries);
+  while (wl > kLdsBudget && c.batch_entries > 4) {
+    c.batch_entries >>= 1;
+    wl = y2k_wave_lds(b->tile_cells, b->SJ, fp64, c.batch_entries);
+  }
+  c.wave_lds = (int)wl;
+  c.waves = 4;
+  while (c.waves > 1 && (size_t)c.waves * wl > kLdsBudget) --c.waves;
+  return c;
+}
+
+// Forward kind: image -> grid, last axis first (image read by the first
+// pass).  Inverse kinds: grid -> image, first axis first (image written by
+// the last pass).  Intermediate passes work in place on the grid.
+int batch_passes(const pnh_batch* b, int kind, int fp64, const void* image_in,
+                 void* image_out, void* grid, const int* done, const int* all_done,
+                 hipStream_t s) {
+  for (int step = 0; step < b->D; ++step) {
+    const int a = kind == BFFT_FWD ? b->D - 1 - step : step;
+    const bool img_in = kind == BFFT_FWD && a == b->D - 1;
+    const bool img_out = kind != BFFT_FWD && a == b->D - 1;
+    int rc = pnh_batch_fft_launch(b->probs, b->fft_t[kind][a], a, kind, fp64,
+                                  b->fft_pow2[kind][a], b->fft_lds[kind][a], b->fft_threads,
+                                  img_in ? image_in : grid, img_in, img_out ? image_out : grid,
+                                  img_out, done, all_done, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int batch_y2k(const pnh_batch* b, int fp64, const void* y, void* k, void* part,
+              const int* done, const int* all_done, hipStream_t s) {
+  const BatchY2kCfg c = y2k_cfg(b, fp64);
+  return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->merges, b->n_merge, b->D,
+                              fp64, b->tile_cells, c.wave_lds, c.batch_entries, c.waves, y, k,
+                              part, done, all_done, s);
+}
+
+struct BatchWs {
+  void *x, *r, *p, *q, *yt, *part;
+  BatchCg cg;
+};
+
+// Carve the CG workspace; with ws == NULL only the size is computed.
+size_t batch_cg_layout(const pnh_batch* b, void* ws, BatchWs* w) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    void* ptr = ws ? (void*)((char*)ws + off) : nullptr;
+    off += align_up(bytes);
+    return ptr;
+  };
+  const size_t g2 = (size_t)b->K_total * 16;
+  const size_t n = b->n;
+  w->x = take(g2);
+  w->r = take(g2);
+  w->p = take(g2);
+  w->q = take(g2);
+  w->yt = take((size_t)b->M_total * 16);
+  w->part = take((size_t)b->slots_total * b->tile_cells * 16);
+  w->cg.rr = (double*)take(n * 8);
+  w->cg.rr_prev = (double*)take(n * 8);
+  w->cg.tol = (double*)take(n * 8);
+  w->cg.pq = (double2*)take(n * 16);
+  w->cg.done = (int*)take(n * 4);
+  w->cg.iters = (int*)take(n * 4);
+  w->cg.all_done = (int*)take(4);
+  const int64_t nblk = std::max(b->vec_t[0].n_blocks,
+                                std::max(b->vec_t[1].n_blocks, b->vec_t[2].n_blocks));
+  w->cg.part = (double2*)take((size_t)nblk * 16);
+  return off;
+}
+
+__global__ void copy_iters_kernel(const int* __restrict__ src, int* __restrict__ dst, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double rtol,
+                double atol, void* ws, int32_t* iters_out, hipStream_t s) {
+  BatchWs w;
+  batch_cg_layout(b, ws, &w);
+  const BatchCg& cg = w.cg;
+  const int64_t ch = b->vec_chunk;
+  int rc;
+  // b = P^H y in FP64 (complex64 samples upcast exactly)
+  if ((rc = pnh_batch_vec_launch(BVEC_CAST_Y, b->probs, b->vec_t[1], ch, (void*)y, w.yt,
+                                 nullptr, nullptr, cg, 0, s))) return rc;
+  if ((rc = batch_y2k(b, 1, w.yt, w.r, w.part, nullptr, nullptr, s))) return rc;
+  if ((rc = pnh_batch_vec_launch(BVEC_ZERO_X, b->probs, b->vec_t[0], ch, w.x, nullptr, nullptr,
+                                 nullptr, cg, 0, s))) return rc;
+  if ((rc = pnh_batch_vec_launch(BVEC_DOT_RR, b->probs, b->vec_t[0], ch, w.r, nullptr, nullptr,
+                                 nullptr, cg, 0, s))) return rc;
+  if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_INIT, 0, maxiter, rtol,
+                                      atol, s))) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, b->vec_t[0], ch, w.p, w.r, nullptr,
+                                   nullptr, cg, it == 0, s))) return rc;
+    if ((rc = pnh_batch_k2y_launch(b->probs, b->k2y_t, b->D, b->J_last, 1, w.p, w.yt, cg.done,
+                                   cg.all_done, s))) return rc;
+    if ((rc = batch_y2k(b, 1, w.yt, w.q, w.part, cg.done, cg.all_done, s))) return rc;
+    if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, b->vec_t[0], ch, w.p, w.q, nullptr,
+                                   nullptr, cg, 0, s))) return rc;
+    if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_PQ, it, maxiter, rtol,
+                                        atol, s))) return rc;
+    if ((rc = pnh_batch_vec_launch(BVEC_XR_UPDATE, b->probs, b->vec_t[0], ch, w.x, w.r, w.p,
+                                   w.q, cg, 0, s))) return rc;
+    if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_RR, it, maxiter, rtol,
+                                        atol, s))) return rc;
+  }
+  // image = crop(ifftn(k)) in FP64 into q (packed image layout), then
+  // x = complex64(image) / sn
+  if ((rc = batch_passes(b, BFFT_INV64, 1, nullptr, w.q, w.x, nullptr, nullptr, s))) return rc;
+  if ((rc = pnh_batch_vec_launch(BVEC_CAST_DIV, b->probs, b->vec_t[2], ch, w.q, x, nullptr,
+                                 nullptr, cg, 0, s))) return rc;
+  if (iters_out) {
+    hipLaunchKernelGGL(copy_iters_kernel, dim3(1), dim3(256), 0, s, cg.iters, iters_out, b->n);
+    PNH_LAUNCH_CHECK();
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pnh_batch_table_bytes(const pnh_plan_desc* descs, int n, size_t* bytes) {
+  REQUIRE(bytes, "null argument");
+  pnh_batch b;
+  int rc = build_batch(descs, n, nullptr, nullptr, &b);
+  if (rc) return rc;
+  *bytes = b.table_bytes;
+  return 0;
+}
+
+int pnh_batch_create(const pnh_plan_desc* descs, int n, void* table, void* stream,
+                     pnh_batch** out) {
+  REQUIRE(out && table, "null argument");
+  *out = nullptr;
+  pnh_batch* b = new (std::nothrow) pnh_batch;
+  if (!b) { pnh_set_error("out of host memory"); return -1; }
+  int rc = build_batch(descs, n, table, (hipStream_t)stream, b);
+  if (rc) { delete b; return rc; }
+  *out = b;
+  return 0;
+}
+
+int pnh_batch_destroy(pnh_batch* b) {
+  delete b;
+  return 0;
+}
+
+int pnh_batch_sizes(const pnh_batch* b, int64_t* n_image, int64_t* n_samples, int64_t* n_grid) {
+  REQUIRE(b && n_image && n_samples && n_grid, "null argument");
+  *n_image = b->N_total;
+  *n_samples = b->M_total;
+  *n_grid = b->K_total;
+  return 0;
+}
+
+int pnh_batch_workspace_bytes(const pnh_batch* b, int op, int dtype, size_t* bytes) {
+  REQUIRE(b && bytes, "null argument");
+  (void)dtype;
+  const size_t grid = align_up((size_t)b->K_total * 8);
+  const size_t part = align_up((size_t)b->slots_total * b->tile_cells * 8);
+  switch (op) {
+    case PNH_OP_FORWARD: *bytes = grid; break;
+    case PNH_OP_ADJOINT: *bytes = grid + part; break;
+    case PNH_OP_SOLVE_CG: { BatchWs w; *bytes = batch_cg_layout(b, nullptr, &w); break; }
+    default: pnh_set_error("op %d is not provided for batches", op); return -1;
+  }
+  return 0;
+}
+
+int pnh_batch_forward(const pnh_batch* b, const void* x, void* y, void* ws, void* stream) {
+  REQUIRE(b && x && ws && (y || b->M_total == 0), "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = batch_passes(b, BFFT_FWD, 0, x, nullptr, ws, nullptr, nullptr, s);
+  if (rc) return rc;
+  return pnh_batch_k2y_launch(b->probs, b->k2y_t, b->D, b->J_last, 0, ws, y, nullptr, nullptr, s);
+}
+
+int pnh_batch_adjoint(const pnh_batch* b, const void* y, void* x, void* ws, void* stream) {
+  REQUIRE(b && x && ws && (y || b->M_total == 0), "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  char* w = (char*)ws;
+  void* grid = w;
+  void* part = w + align_up((size_t)b->K_total * 8);
+  int rc = batch_y2k(b, 0, y, grid, part, nullptr, nullptr, s);
+  if (rc) return rc;
+  return batch_passes(b, BFFT_INV, 0, nullptr, x, grid, nullptr, nullptr, s);
+}
+
+int pnh_batch_solve_cg(const pnh_batch* b, const void* y, void* x, int maxiter, double rtol,
+                       double atol, void* ws, int32_t* iters_out, void* stream) {
+  REQUIRE(b && x && ws && (y || b->M_total == 0), "null argument");
+  REQUIRE(maxiter >= 0, "maxiter must be >= 0");
+  return batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, (hipStream_t)stream);
+}
+
+int pnh_batch_cg_graph_create(const pnh_batch* b, const void* y, void* x, int maxiter,
+                              double rtol, double atol, void* ws, int32_t* iters_out,
+                              pnh_graph** out) {
+  REQUIRE(b && x && ws && out, "null argument");
+  *out = nullptr;
+  hipStream_t cs;
+  PNH_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    pnh_set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+    return -3;
+  }
+  const int rc = batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, cs);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) {
+    pnh_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+    return -3;
+  }
+  pnh_graph* gr = new (std::nothrow) pnh_graph;
+  if (!gr) { (void)hipGraphDestroy(g); pnh_set_error("out of host memory"); return -1; }
+  e = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    delete gr;
+    pnh_set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return -3;
+  }
+  *out = gr;
+  return 0;
+}
+
+int pnh_graph_launch(pnh_graph* g, void* stream) {
+  REQUIRE(g, "null argument");
+  PNH_CHECK_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return 0;
+}
+
+int pnh_graph_destroy(pnh_graph* g) {
+  if (g) {
+    (void)hipGraphExecDestroy(g->exec);
+    delete g;
+  }
   return 0;
 }
 

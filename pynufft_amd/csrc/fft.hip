@@ -343,3 +343,46 @@ int pnh_fft_factor(int n, int* radix, int* ns, int* npass) {
   *npass = cnt;
   return 0;
 }
+
+// ------------------------------------------------------------- batches ----
+#include "batch.h"
+
+namespace {
+
+template <typename T, bool POW2>
+__global__ void __launch_bounds__(256)
+fft_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t, int axis, int kind,
+                 const void* in_, int in_image, void* out_, int out_image,
+                 const int* __restrict__ done, const int* __restrict__ all_done) {
+  typedef typename cplx<T>::t C;
+  if (all_done && *all_done) return;
+  const int p = t.tab[blockIdx.x];
+  if (done && done[p]) return;
+  const int64_t lb = (int64_t)blockIdx.x - t.first[p];
+  extern __shared__ unsigned char smem_raw[];
+  const BatchProblem& pr = probs[p];
+  const FftDesc d = kind == BFFT_FWD ? pr.fwd[axis] : (kind == BFFT_INV ? pr.inv[axis] : pr.inv64[axis]);
+  const void* tw = kind == BFFT_INV64 ? pr.tw64[axis] : pr.tw32[axis];
+  const C* in = (const C*)in_ + (in_image ? pr.x_off : pr.k_off);
+  C* out = (C*)out_ + (out_image ? pr.x_off : pr.k_off);
+  fft_body<T, POW2>(d, in, out, tw, lb, smem_raw);
+}
+
+}  // namespace
+
+int pnh_batch_fft_launch(const BatchProblem* probs, const BatchTable& t, int axis,
+                         int kind, int fp64, int pow2, size_t lds, int threads,
+                         const void* in, int in_image, void* out, int out_image,
+                         const int* done, const int* all_done, hipStream_t s) {
+  if (t.n_blocks == 0) return 0;
+  dim3 g((unsigned)t.n_blocks), b(threads);
+  if (fp64) {
+    if (pow2) hipLaunchKernelGGL((fft_batch_kernel<double, true>), g, b, lds, s, probs, t, axis, kind, in, in_image, out, out_image, done, all_done);
+    else hipLaunchKernelGGL((fft_batch_kernel<double, false>), g, b, lds, s, probs, t, axis, kind, in, in_image, out, out_image, done, all_done);
+  } else {
+    if (pow2) hipLaunchKernelGGL((fft_batch_kernel<float, true>), g, b, lds, s, probs, t, axis, kind, in, in_image, out, out_image, done, all_done);
+    else hipLaunchKernelGGL((fft_batch_kernel<float, false>), g, b, lds, s, probs, t, axis, kind, in, in_image, out, out_image, done, all_done);
+  }
+  PNH_LAUNCH_CHECK();
+  return 0;
+}

@@ -761,3 +761,167 @@ int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
   pnh_set_error("unsupported ndim %d", a.D);
   return -2;
 }
+
+// ------------------------------------------------------------- batches ----
+// The same kernels over many independent problems in one launch: each block
+// (k2y) or wavefront (y2k) looks up its problem and runs the single-problem
+// body on that problem's slice of the packed vectors.
+#include "batch.h"
+
+namespace {
+
+template <typename T, int D, int LPS>
+__global__ void __launch_bounds__(256)
+k2y_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
+                 const void* __restrict__ k_, void* __restrict__ y_,
+                 const int* __restrict__ done, const int* __restrict__ all_done) {
+  typedef typename cplx<T>::t C;
+  if (all_done && *all_done) return;
+  const int p = t.tab[blockIdx.x];
+  if (done && done[p]) return;
+  const int64_t lb = (int64_t)blockIdx.x - t.first[p];
+  const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
+  const int64_t wave = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  k2y_body<T, D, LPS, false, 1>(a, wave, (const C*)k_ + probs[p].k_off,
+                                (C*)y_ + probs[p].y_off, 0);
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+y2k_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
+                 int64_t n_items, int wave_lds, int batch_entries,
+                 const void* __restrict__ y_, void* __restrict__ k_, void* __restrict__ part_,
+                 const int* __restrict__ done, const int* __restrict__ all_done) {
+  typedef typename cplx<T>::t C;
+  if (all_done && *all_done) return;
+  extern __shared__ unsigned char smem_raw[];
+  const int wave = threadIdx.x >> 6;
+  const int64_t gi = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  if (gi >= n_items) return;
+  const int2 pi = items[gi];
+  const int p = pi.x;
+  if (done && done[p]) return;
+  const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
+  BinArgs b = probs[p].bins;
+  b.batch = batch_entries;
+  b.wave_lds_bytes = wave_lds;
+  const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
+  y2k_body<T, D, false, 1>(a, b, it, smem_raw + (size_t)wave * wave_lds,
+                           (const C*)y_ + probs[p].y_off, (C*)k_ + probs[p].k_off,
+                           (C*)part_ + probs[p].part_off * b.tile_cells, 0);
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+merge_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ merges,
+                   int64_t n_merge, const void* __restrict__ part_, void* __restrict__ k_,
+                   const int* __restrict__ done, const int* __restrict__ all_done) {
+  typedef typename cplx<T>::t C;
+  if (all_done && *all_done) return;
+  for (int64_t g = blockIdx.y; g < n_merge; g += gridDim.y) {
+    const int2 pm = merges[g];
+    const int p = pm.x;
+    if (done && done[p]) continue;
+    const InterpArgs& a = probs[p].ia;
+    const BinArgs& b = probs[p].bins;
+    const int3 me = reinterpret_cast<const int3*>(b.merge)[pm.y];
+    const int cells = b.tile_cells;
+    int tid = me.x;
+    int T0[PNH_MAXD], E[PNH_MAXD];
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) {
+      const int tg = b.tgrid[d];
+      const int td = tid % tg;
+      tid /= tg;
+      T0[d] = td * b.tile[d];
+      E[d] = min(b.tile[d], a.K[d] - T0[d]);
+    }
+    const int E1 = D > 1 ? E[1] : 1;
+    const int E2 = D > 2 ? E[2] : 1;
+    const int ncell = E[0] * E1 * E2;
+    const C* part = (const C*)part_ + probs[p].part_off * cells;
+    C* grid = (C*)k_ + probs[p].k_off;
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < ncell;
+         cell += gridDim.x * blockDim.x) {
+      const int l2 = D > 2 ? cell % E2 : 0;
+      const int t = D > 2 ? cell / E2 : cell;
+      const int l1 = D > 1 ? t % E1 : 0;
+      const int l0 = D > 1 ? t / E1 : t;
+      int64_t gcell = (int64_t)(T0[0] + l0) * a.Kstride[0];
+      if (D > 1) gcell += (int64_t)(T0[1] + l1) * a.Kstride[1];
+      if (D > 2) gcell += (int64_t)(T0[2] + l2);
+      C s = part[(int64_t)me.y * cells + cell];
+      for (int k = 1; k < me.z; ++k) s = cadd(s, part[(int64_t)(me.y + k) * cells + cell]);
+      grid[gcell] = s;
+    }
+  }
+}
+
+template <typename T, int D>
+int batch_k2y_d(const BatchProblem* probs, const BatchTable& t, int J_last, const void* k,
+                void* y, const int* done, const int* all_done, hipStream_t s) {
+  dim3 g((unsigned)t.n_blocks), b(256);
+  switch (pnh_lanes_per_sample(J_last)) {
+    case 1: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 1>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 2: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 2>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 3: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 3>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 4: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 4>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 6: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 6>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 8: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 8>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    case 16: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 16>), g, b, 0, s, probs, t, k, y, done, all_done); break;
+    default: pnh_set_error("interpolator size above 16"); return -2;
+  }
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+int pnh_batch_k2y_launch(const BatchProblem* probs, const BatchTable& t, int D, int J_last,
+                         int fp64, const void* k, void* y, const int* done,
+                         const int* all_done, hipStream_t s) {
+  if (t.n_blocks == 0) return 0;
+  if (fp64) {
+    switch (D) {
+      case 1: return batch_k2y_d<double, 1>(probs, t, J_last, k, y, done, all_done, s);
+      case 2: return batch_k2y_d<double, 2>(probs, t, J_last, k, y, done, all_done, s);
+      case 3: return batch_k2y_d<double, 3>(probs, t, J_last, k, y, done, all_done, s);
+    }
+  } else {
+    switch (D) {
+      case 1: return batch_k2y_d<float, 1>(probs, t, J_last, k, y, done, all_done, s);
+      case 2: return batch_k2y_d<float, 2>(probs, t, J_last, k, y, done, all_done, s);
+      case 3: return batch_k2y_d<float, 3>(probs, t, J_last, k, y, done, all_done, s);
+    }
+  }
+  pnh_set_error("unsupported ndim %d", D);
+  return -2;
+}
+
+int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
+                         const int2* merges, int64_t n_merge, int D, int fp64,
+                         int tile_cells, int wave_lds, int batch_entries, int waves,
+                         const void* y, void* k, void* part, const int* done,
+                         const int* all_done, hipStream_t s) {
+  const int64_t nblk = (n_items + waves - 1) / waves;
+  const size_t lds = (size_t)waves * wave_lds;
+  dim3 g((unsigned)nblk), b(64 * waves);
+#define YB(T, DD) hipLaunchKernelGGL((y2k_batch_kernel<T, DD>), g, b, lds, s, probs, items, \
+                                     n_items, wave_lds, batch_entries, y, k, part, done, all_done)
+#define MB(T, DD) hipLaunchKernelGGL((merge_batch_kernel<T, DD>), mg, dim3(256), 0, s, probs, \
+                                     merges, n_merge, part, k, done, all_done)
+  if (nblk > 0) {
+    if (fp64) { if (D == 1) YB(double, 1); else if (D == 2) YB(double, 2); else YB(double, 3); }
+    else { if (D == 1) YB(float, 1); else if (D == 2) YB(float, 2); else YB(float, 3); }
+    PNH_LAUNCH_CHECK();
+  }
+  if (n_merge > 0) {
+    dim3 mg((unsigned)((tile_cells + 255) / 256), (unsigned)(n_merge < 65535 ? n_merge : 65535));
+    if (fp64) { if (D == 1) MB(double, 1); else if (D == 2) MB(double, 2); else MB(double, 3); }
+    else { if (D == 1) MB(float, 1); else if (D == 2) MB(float, 2); else MB(float, 3); }
+    PNH_LAUNCH_CHECK();
+  }
+#undef YB
+#undef MB
+  return 0;
+}

@@ -80,7 +80,7 @@ class NUFFT:
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
-             tile=None, chunk=None):
+             tile=None, chunk=None, host_plan=None):
         """Design the interpolator on the host in FP64 and upload it.
 
         Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
@@ -91,11 +91,12 @@ class NUFFT:
         ``legacy=True``.  ``tile`` (gridding tile per axis) and ``chunk``
         (bin entries per gridding work item) are tuning knobs; the defaults
         come from ``helper.tile_shape`` and ``helper.sort_and_bin``.
-        Returns 0.
+        ``host_plan`` may pass a ``helper.plan`` result computed elsewhere
+        (e.g. in a thread pool).  Returns 0.
         """
         self.release()
-        st = helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile,
-                         chunk_entries=chunk)
+        st = host_plan if host_plan is not None else \
+            helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_entries=chunk)
         dd = len(Nd)
         self.ndims = dd
         self.ft_axes = tuple(range(dd)) if ft_axes is None else tuple(ft_axes)

@@ -115,3 +115,29 @@ def fingerprint(a):
     a = a.astype(np.float64).ravel()
     w = np.cos(np.arange(a.size) * 0.7071067811865476)
     return float(np.dot(a, w)), float(np.dot(a, a))
+
+
+def batched2d(n=64, seed0=400):
+    """``n`` independent 2-D problems of uneven size (the batched workload).
+
+    Problem i (seed 400 + i): Nd = (N, N) with N drawn from {128, 192, 256},
+    Kd = 2 Nd, Jd = (6, 6); trajectory either golden-angle radial with
+    64-402 spokes x 2N readout samples, or a variable-density spiral with
+    density ~ 1/(1 + r^2) and M drawn in [16384, 205824]; image = 10-ellipse
+    phantom.
+    """
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed0 + i)
+        N = int(rng.choice([128, 192, 256]))
+        if rng.uniform() < 0.5:
+            spokes = int(rng.integers(64, 403))
+            om = radial_om(spokes, 2 * N)
+            kind = 'radial'
+        else:
+            M = int(rng.integers(16384, 205825))
+            om = spiral_om(M, N, seed0 + i)
+            kind = 'spiral'
+        out.append(dict(om=om, Nd=(N, N), Kd=(2 * N, 2 * N), Jd=(6, 6),
+                        x=ellipse_phantom((N, N), seed0 + i), kind=kind))
+    return out

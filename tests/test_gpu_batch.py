@@ -1,0 +1,83 @@
+"""Batched uneven problems: one launch per stage for all problems must give
+each problem's single-plan result; the captured CG graph must reproduce the
+eager batched solve on every replay."""
+import numpy as np
+import pytest
+
+from pynufft_amd import workloads as W
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def problems():
+    out = []
+    specs = [((16, 16), 900, 'uniform'), ((24, 24), 2000, 'radial'),
+             ((20, 20), 300, 'spiral'), ((32, 32), 3000, 'uniform'),
+             ((12, 12), 100, 'radial'), ((24, 24), 1500, 'spiral')]
+    for i, (Nd, M, kind) in enumerate(specs):
+        if kind == 'uniform':
+            om = W.uniform_om(M, 2, 900 + i)
+        elif kind == 'radial':
+            spokes = max(2, M // (2 * Nd[0]))
+            om = W.radial_om(spokes, 2 * Nd[0])
+        else:
+            om = W.spiral_om(M, Nd[0], 900 + i)
+        Kd = tuple(2 * n for n in Nd)
+        out.append((om, Nd, Kd, (6, 6)))
+    return out
+
+
+@pytest.fixture(scope='module')
+def batch(hip):
+    from pynufft_amd.batch import BatchNUFFT
+    B = BatchNUFFT()
+    B.plan(problems(), chunk=64)
+    return B
+
+
+def test_batch_operators_match_single(batch):
+    import torch
+    B = batch
+    xs = [torch.from_numpy(W.gaussian_image(Nd, 50 + i)).to(B.device)
+          for i, Nd in enumerate(B.Nd)]
+    y = B.forward(xs)
+    ys = B.unpack_samples(y)
+    for i, op in enumerate(B.ops):
+        assert rel(ys[i].cpu().numpy(), op.forward(xs[i]).cpu().numpy()) < 1e-6
+    x = B.adjoint(y)
+    xa = B.unpack_images(x)
+    for i, op in enumerate(B.ops):
+        assert rel(xa[i].cpu().numpy(), op.adjoint(ys[i]).cpu().numpy()) < 1e-6
+
+
+def test_batch_cg_matches_single(batch):
+    import torch
+    B = batch
+    xs = [torch.from_numpy(W.gaussian_image(Nd, 60 + i)).to(B.device)
+          for i, Nd in enumerate(B.Nd)]
+    ys = B.unpack_samples(B.forward(xs))
+    for rtol in (1e-5, 3e-2):
+        x = B.unpack_images(B.solve(ys, maxiter=12, rtol=rtol))
+        its = B.cg_iters.cpu().numpy()
+        for i, op in enumerate(B.ops):
+            xi = op.solve(ys[i], 'cg', maxiter=12, rtol=rtol)
+            assert rel(x[i].cpu().numpy(), xi.cpu().numpy()) < 1e-6
+            assert its[i] == op.cg_iters
+
+
+def test_batch_cg_graph_replays(batch):
+    import torch
+    B = batch
+    xs = [torch.from_numpy(W.gaussian_image(Nd, 70 + i)).to(B.device)
+          for i, Nd in enumerate(B.Nd)]
+    y = B.forward(xs)
+    ref = B.solve(y, maxiter=8)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        out = B.solve_graph(y, maxiter=8)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), rep
+        # eager work between replays
+        B.adjoint(B.forward(xs))
+        torch.cuda.synchronize()
