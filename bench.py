@@ -28,6 +28,10 @@ METRIC = ("3D 128³ NUFFT fwd+adj pairs/s (% HBM peak); "
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    print('[bench] %s' % msg, file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(kernel, fmt, M, Jd, Kd, nb=1):
     """Bytes one launch must move at minimum (per-unit figures x units).
 
@@ -127,6 +131,7 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps, out
 
+    progress('batched: planned %d problems in %.1f s' % (len(ps), t_plan))
     t_plain, plain = timed(lambda: [op.solve(ys[i], 'cg', maxiter=maxiter, rtol=0.0)
                                     for i, op in enumerate(B.ops)])
     t_batch, xb = timed(lambda: B.solve(y, maxiter=maxiter, rtol=0.0))
@@ -171,6 +176,7 @@ def main():
     from pynufft_amd import workloads as W
     from pynufft_amd.nufft import NUFFT
 
+    progress('workload %s' % args.workload)
     w = getattr(W, args.workload)()
     t0 = time.perf_counter()
     A = NUFFT(legacy=args.legacy)
@@ -185,6 +191,7 @@ def main():
         y = A.forward(x)
         return A.adjoint(y)
 
+    progress('planned in %.1f s' % t_plan)
     for _ in range(args.warmup):
         pair()
     torch.cuda.synchronize()
@@ -196,6 +203,7 @@ def main():
     ms_per_step = 1000.0 * dt / args.steps
     value = args.steps / dt
 
+    progress('timed %d pairs: %.3f ms each' % (args.steps, ms_per_step))
     # dominant kernel: k2y vs y2k timed alone with HIP events
     k = A.xx2k(x)
     y = A.k2y(k)
@@ -230,10 +238,13 @@ def main():
     if not args.no_batched:
         del A, k, y, x
         torch.cuda.empty_cache()
+        progress('batched CG workload')
         batched = bench_batched(torch, W)
+        progress('batched done')
 
     cpu = None
     if not args.no_cpu_baseline:
+        progress('CPU oracle baseline')
         cpu = cpu_baseline(w, min(args.cpu_sample, M))
 
     pair_bytes = (2 * algorithmic_bytes('k2y', fmt, M, w['Jd'], w['Kd']) +

@@ -271,12 +271,18 @@ k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
 
 // ---------------------------------------------------------------- y2k ----
 // Support offsets j in [0, J) whose cell (b + j) mod K lies in the tile
-// [T0, T0 + E).  Returns 1 and one interval [lo, lo + n) when the hit set is
-// a single interval (the common case), 0 when it is two intervals (a wrapped
-// support meeting both ends of a tile that nearly spans the axis).
+// [T0, T0 + E).  Returns 1 and one interval [lo, lo + n) when the hit cells
+// are one contiguous run (the common case), 0 when they are two runs (a
+// wrapped support meeting both ends of a tile that spans or nearly spans the
+// axis).
 __device__ __forceinline__ int clip_axis(int b, int J, int K, int T0, int E,
                                          int& lo, int& n) {
-  if (E == K) { lo = 0; n = J; return 1; }
+  if (E == K) {
+    // the tile spans the axis: the hit cells are contiguous from b unless
+    // the support wraps past K - 1
+    lo = 0; n = J;
+    return b + J <= K ? 1 : 0;
+  }
   const int T1 = T0 + E;
   const int l1 = max(0, T0 - b), h1 = min(min(J, K - b), T1 - b);
   const int l2 = max(K - b, T0 - b + K), h2 = min(J, T1 - b + K);

@@ -486,6 +486,38 @@ class NUFFT:
         it = self._cg_iters_dev.cpu().numpy()
         return int(it[0]) if self.batch is None else it
 
+    # ------------------------------------------------------ multi-coil --
+    def set_sense(self, coil_profile):
+        """Coil sensitivities S, shape Nd + (batch,) (reference
+        ``linalg/nufft_cpu.py:163-175``, ``set_sense``)."""
+        if self.batch is None:
+            raise ValueError('plan with batch=<number of coils> first')
+        S = torch.as_tensor(coil_profile).to(self.device, torch.complex64)
+        if tuple(S.shape) != self.Nd + (self.batch,):
+            raise ValueError('coil profile must have shape %s' % (self.Nd + (self.batch,),))
+        self._sense = S.contiguous()
+
+    def forward_one2many(self, x):
+        """y[:, c] = forward(S_c x) (reference ``linalg/nufft_cpu.py:177-189``)."""
+        is_np = not isinstance(x, torch.Tensor)
+        xi = torch.as_tensor(x).to(self.device, torch.complex64).reshape(self.Nd)
+        y = self.forward(xi[..., None] * self._sense)
+        return y.cpu().numpy() if is_np else y
+
+    def adjoint_many2one(self, y):
+        """sum_c conj(S_c) adjoint(y[:, c]) (reference ``linalg/nufft_cpu.py:191-203``)."""
+        is_np = not isinstance(y, torch.Tensor)
+        xt = self.adjoint(torch.as_tensor(y).to(self.device, torch.complex64))
+        out = (xt * self._sense.conj()).sum(dim=-1)
+        return out.cpu().numpy() if is_np else out
+
+    def selfadjoint_one2many2one(self, x):
+        """adjoint_many2one(forward_one2many(x))."""
+        is_np = not isinstance(x, torch.Tensor)
+        xi = torch.as_tensor(x).to(self.device, torch.complex64)
+        out = self.adjoint_many2one(self.forward_one2many(xi))
+        return out.cpu().numpy() if is_np else out
+
     # ----------------------------------------------------------- misc --
     def to_device(self, x, shape=None):
         """complex64 copy on the plan's device (reference ``to_device``)."""

@@ -141,3 +141,31 @@ def batched2d(n=64, seed0=400):
         out.append(dict(om=om, Nd=(N, N), Kd=(2 * N, 2 * N), Jd=(6, 6),
                         x=ellipse_phantom((N, N), seed0 + i), kind=kind))
     return out
+
+
+def coil_maps(Nd, ncoil, seed):
+    """Smooth complex coil sensitivities: degree-2 polynomials in (y, x) over
+    [-1, 1)^2 with random complex coefficients, normalised so that
+    sum_c |S_c|^2 = 1 at every pixel.  Shape Nd + (ncoil,), complex64."""
+    rng = np.random.default_rng(seed)
+    yy = -1.0 + 2.0 * np.arange(Nd[0]) / Nd[0]
+    xx = -1.0 + 2.0 * np.arange(Nd[1]) / Nd[1]
+    Y, X = np.meshgrid(yy, xx, indexing='ij')
+    basis = np.stack([np.ones_like(X), X, Y, X * X, X * Y, Y * Y], axis=-1)
+    coef = rng.standard_normal((6, ncoil)) + 1j * rng.standard_normal((6, ncoil))
+    coef[0] += 3.0  # keep every map away from zero
+    S = basis @ coef
+    S /= np.sqrt(np.sum(np.abs(S) ** 2, axis=-1, keepdims=True))
+    return S.astype(np.complex64)
+
+
+def multicoil2d(perturb=1e-3, seed=501):
+    """Differentiable multi-coil workload: Nd 256^2, Kd 512^2, Jd 6^2, 256
+    radial spokes x 512 samples (M 131 072), 8 coils, ellipse image; the
+    data trajectory is om + N(0, perturb) rad."""
+    rng = np.random.default_rng(seed)
+    om = radial_om(256, 512)
+    om_data = np.clip(om + rng.normal(0.0, perturb, om.shape), -np.pi, np.pi - 1e-9)
+    Nd = (256, 256)
+    return dict(om=om, om_data=om_data, Nd=Nd, Kd=(512, 512), Jd=(6, 6),
+                x=ellipse_phantom(Nd, seed + 1), S=coil_maps(Nd, 8, seed + 2))

@@ -10,42 +10,120 @@ from pynufft_amd import workloads as W
 from conftest import TINY_CASES, golden, geometry, rel
 
 
+def _wrap(b, j, K):
+    c = b + j
+    return c - K if c >= K else c
+
+
+def clip_axis(b, J, K, T0, E):
+    """Mirror of the kernel's clip_axis (csrc/interp.hip)."""
+    if E == K:
+        return (1 if b + J <= K else 0), 0, J
+    T1 = T0 + E
+    l1, h1 = max(0, T0 - b), min(min(J, K - b), T1 - b)
+    l2, h2 = max(K - b, T0 - b + K), min(J, T1 - b + K)
+    a1, a2 = h1 > l1, h2 > l2
+    if a1 and a2:
+        return 0, 0, 0
+    if a1:
+        return 1, l1, h1 - l1
+    if a2:
+        return 1, l2, h2 - l2
+    return 1, 0, 0
+
+
 def emulate_y2k(st, Kd, Jd, y):
-    """The binned gridding algorithm in numpy: one accumulator per work item,
-    samples visited in bin order, support cells clipped to the tile, split
-    tiles summed in chunk order."""
+    """The y2k kernel's algorithm in numpy, step for step: one accumulator per
+    work item, entries in bin order, support boxes clipped to the tile
+    (packed start cell, 16-bit reciprocal division) or, for a wrapped support
+    in a tile that (nearly) spans an axis, the masked walk; split tiles summed
+    in chunk order."""
     D = len(Kd)
     tile = st['tile']
     tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
     perm = st['perm']
     base = st['base'][:, perm]
     u = [st['ud'][a][perm] for a in range(D)]
+    cells = int(np.prod(tile))
+    J = list(Jd) + [1] * (3 - D)
+    K = list(Kd) + [1] * (3 - D)
     out = np.full(Kd, np.nan, dtype=complex)
-    part = {}
+    parts = {}
     for tid, s, cnt, slot in st['items']:
-        t, T0 = tid, [0] * D
-        for a in range(D - 1, -1, -1):
-            T0[a] = (t % tgrid[a]) * tile[a]
-            t //= tgrid[a]
-        E = [min(tile[a], Kd[a] - T0[a]) for a in range(D)]
-        acc = np.zeros(E, complex)
+        t, T0, E = tid, [0, 0, 0], [1, 1, 1]
+        for d in range(D - 1, -1, -1):
+            T0[d] = (t % tgrid[d]) * tile[d]
+            t //= tgrid[d]
+            E[d] = min(tile[d], Kd[d] - T0[d])
+        E1, E2 = E[1], E[2]
+        acc = np.zeros(cells, complex)
         for r in st['bin_rows'][s:s + cnt]:
-            for j in np.ndindex(*Jd):
-                l = [(base[a][r] + j[a]) % Kd[a] - T0[a] for a in range(D)]
-                if all(0 <= l[a] < E[a] for a in range(D)):
-                    w = 1
-                    for a in range(D):
-                        w = w * u[a][r][j[a]]
-                    acc[tuple(l)] += np.conj(w) * y[perm[r]]
-        sl = tuple(slice(T0[a], T0[a] + E[a]) for a in range(D))
+            bb = [int(base[d][r]) for d in range(D)] + [0] * (3 - D)
+            lo, n, st0, single = [0, 0, 0], [1, 1, 1], [0, 0, 0], 1
+            for d in range(D):
+                ok, lo[d], n[d] = clip_axis(bb[d], J[d], K[d], T0[d], E[d])
+                single &= ok
+                st0[d] = _wrap(bb[d], lo[d], K[d]) - T0[d]
+            yv = y[perm[r]]
+            if single:
+                tot = n[0] * n[1] * n[2]
+                start = (st0[0] * E1 + st0[1]) * E2 + st0[2]
+                m2 = (65536 + n[2] - 1) // max(n[2], 1)
+                m1 = (65536 + n[1] - 1) // max(n[1], 1)
+                for q in range(tot):
+                    if tot <= 256:
+                        tt = (q * m2) >> 16
+                        t2 = q - tt * n[2]
+                        t0 = (tt * m1) >> 16
+                        t1 = tt - t0 * n[1]
+                    else:
+                        t2, tt = q % n[2], q // n[2]
+                        t1, t0 = tt % n[1], tt // n[1]
+                    w = u[0][r][lo[0] + t0]
+                    if D > 1:
+                        w = w * u[1][r][lo[1] + t1]
+                    if D > 2:
+                        w = w * u[2][r][lo[2] + t2]
+                    acc[start + (t0 * E1 + t1) * E2 + t2] += np.conj(w) * yv
+            else:
+                for j in np.ndindex(*Jd):
+                    l = [_wrap(bb[d], j[d], K[d]) - T0[d] for d in range(D)]
+                    if all(0 <= l[d] < E[d] for d in range(D)):
+                        l = l + [0] * (3 - D)
+                        w = 1
+                        for d in range(D):
+                            w = w * u[d][r][j[d]]
+                        acc[(l[0] * E1 + l[1]) * E2 + l[2]] += np.conj(w) * yv
+        blk = acc[:E[0] * E1 * E2].reshape([E[d] for d in range(D)])
+        sl = tuple(slice(T0[d], T0[d] + E[d]) for d in range(D))
         if slot < 0:
-            out[sl] = acc
+            out[sl] = blk
         else:
-            part[slot] = (sl, acc)
-    for tid, s0, n in st['merge']:
-        sl = part[s0][0]
-        out[sl] = sum(part[s0 + k][1] for k in range(n))
+            parts[slot] = (sl, blk)
+    for tid, s0, nn in st['merge']:
+        sl = parts[s0][0]
+        out[sl] = sum(parts[s0 + k][1] for k in range(nn))
     return out
+
+
+def test_clip_axis_exhaustive():
+    """clip_axis returns exactly the hit offsets, as one contiguous run of
+    cells, or flags the case (two runs) for the masked walk."""
+    for K in range(1, 26):
+        for J in range(1, min(K, 8) + 1):
+            for tile in range(1, K + 1):
+                for T0 in range(0, K, tile):
+                    E = min(tile, K - T0)
+                    for b in range(K):
+                        hits = [j for j in range(J) if T0 <= (b + j) % K < T0 + E]
+                        single, lo, n = clip_axis(b, J, K, T0, E)
+                        if single:
+                            assert hits == list(range(lo, lo + n))
+                            cells = [(b + j) % K for j in hits]
+                            assert cells == list(range(cells[0], cells[0] + n)) if n else True
+                        else:
+                            runs = [(b + j) % K for j in hits]
+                            assert any(runs[i + 1] != runs[i] + 1 for i in range(len(runs) - 1))
 
 
 @pytest.mark.parametrize('name', TINY_CASES + ['plain2d'])
