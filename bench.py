@@ -150,6 +150,37 @@ def bench_batched(torch, W, maxiter=20, reps=3):
     }
 
 
+def bench_cg_radial(torch, W, maxiter=30, reps=3, cpu=True):
+    """Single-problem FP64 CG (radial 2-D 256^2/512^2, M 205 824, 30
+    iterations, rtol 0) on the GPU, and the CPU oracle's scipy CG."""
+    from pynufft_amd.nufft import NUFFT
+    w = W.radial2d()
+    A = NUFFT()
+    A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
+    x = torch.from_numpy(w['x']).to(A.device)
+    y = A.forward(x)
+    A.solve(y, 'cg', maxiter=maxiter, rtol=0.0)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        A.solve(y, 'cg', maxiter=maxiter, rtol=0.0)
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t) / reps
+    out = {'workload': 'radial2d', 'cg_iterations': maxiter, 'gpu_solves_per_s': 1.0 / t_gpu,
+           'gpu_ms_per_solve': 1000.0 * t_gpu}
+    if cpu:
+        import oracle
+        B = oracle.OracleNUFFT()
+        B.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
+        yc = y.cpu().numpy()
+        t = time.perf_counter()
+        B.solve(yc, 'cg', maxiter=maxiter, rtol=0.0)
+        t_cpu = time.perf_counter() - t
+        out['cpu_oracle_solves_per_s'] = 1.0 / t_cpu
+        out['cpu_cores'] = 1
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -242,6 +273,10 @@ def main():
         batched = bench_batched(torch, W)
         progress('batched done')
 
+    progress('single-problem CG (radial 2-D)')
+    cg_single = bench_cg_radial(torch, W, cpu=not args.no_cpu_baseline) \
+        if not args.no_batched else None
+
     cpu = None
     if not args.no_cpu_baseline:
         progress('CPU oracle baseline')
@@ -266,6 +301,7 @@ def main():
         'pair_algorithmic_bytes': pair_bytes,
         'plan_s': t_plan,
         'batched_cg': batched,
+        'cg_single': cg_single,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

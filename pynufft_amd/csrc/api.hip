@@ -259,9 +259,13 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   }
   REQUIRE(pnh_lanes_per_sample(d.Jd[d.ndim - 1]) > 0, "Jd above 16 on the last axis");
   p->bins.tile_cells = (int)cells;
+  // LDS accumulator: last axis contiguous, rows padded by one cell
+  if (p->D == 1) p->bins.acc_cells = d.tile[0];
+  else if (p->D == 2) p->bins.acc_cells = d.tile[0] * (d.tile[1] + 1);
+  else p->bins.acc_cells = d.tile[0] * d.tile[1] * (d.tile[2] + 1);
   p->bins.waves_per_block = 4;
   p->bins.lds_budget = kLdsBudget;
-  if ((size_t)cells * 2 * sizeof(double) > kLdsBudget) {
+  if ((size_t)p->bins.acc_cells * 2 * sizeof(double) > kLdsBudget) {
     pnh_set_error("gridding tile of %lld cells exceeds the LDS budget", (long long)cells);
     return -2;
   }
@@ -492,7 +496,7 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 #include "batch.h"
 
 struct pnh_batch {
-  int n, D, J_last, SJ, tile_cells;
+  int n, D, J_last, SJ, tile_cells, acc_cells;
   int64_t N_total, M_total, K_total, slots_total;
   int fft_threads;
   BatchProblem* probs;                 // device (inside the caller's table)
@@ -544,6 +548,7 @@ void add_table(Blob& blob, const std::vector<int64_t>& blocks_per_problem,
 }
 
 size_t y2k_wave_lds(int cells, int SJ, int fp64, int batch_entries) {
+  // `cells` is the padded accumulator size
   const size_t esz = fp64 ? 8 : 4;
   size_t wl = (size_t)cells * 2 * esz + (size_t)batch_entries * SJ * 2 * esz +
               (size_t)batch_entries * sizeof(int);
@@ -579,6 +584,7 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   b->SJ = 0;
   for (int a = 0; a < D; ++a) b->SJ += plans[0].J[a];
   b->tile_cells = plans[0].bins.tile_cells;
+  b->acc_cells = plans[0].bins.acc_cells;
   b->fft_threads = 256;
   b->vec_chunk = kVecChunk;
   std::vector<BatchProblem> probs(n);
@@ -704,10 +710,10 @@ struct BatchY2kCfg {
 BatchY2kCfg y2k_cfg(const pnh_batch* b, int fp64) {
   BatchY2kCfg c;
   c.batch_entries = 32;
-  size_t wl = y2k_wave_lds(b->tile_cells, b->SJ, fp64, c.batch_entries);
+  size_t wl = y2k_wave_lds(b->acc_cells, b->SJ, fp64, c.batch_entries);
   while (wl > kLdsBudget && c.batch_entries > 4) {
     c.batch_entries >>= 1;
-    wl = y2k_wave_lds(b->tile_cells, b->SJ, fp64, c.batch_entries);
+    wl = y2k_wave_lds(b->acc_cells, b->SJ, fp64, c.batch_entries);
   }
   c.wave_lds = (int)wl;
   c.waves = 4;

@@ -25,6 +25,7 @@ struct BinArgs {
   int tile[PNH_MAXD];          // tile extent per dim
   int tgrid[PNH_MAXD];         // tiles per dim
   int tile_cells;              // prod(tile)
+  int acc_cells;               // LDS accumulator cells (last-axis rows padded by one)
   int waves_per_block;         // set per launch
   int batch;                   // entries staged per step (set per launch)
   int wave_lds_bytes;          // LDS bytes per wavefront (set per launch)

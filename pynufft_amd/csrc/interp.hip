@@ -314,13 +314,14 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
                                          void* __restrict__ part_, int c0) {
   typedef typename cplx<T>::t C;
   const int lane = threadIdx.x & 63;
-  const int cells = b.tile_cells;
+  const int cells = b.tile_cells;      // compact tile (partials, output)
+  const int acells = b.acc_cells;      // LDS accumulator (padded rows)
   const int J0 = a.J[0];
   const int J1 = D > 1 ? a.J[1] : 1;
   const int J2 = D > 2 ? a.J[2] : 1;
   const int SJ = J0 + (D > 1 ? J1 : 0) + (D > 2 ? J2 : 0);
   T* acc = reinterpret_cast<T*>(wave_lds);
-  C* stage = reinterpret_cast<C*>(acc + (size_t)cells * 2 * NBC);
+  C* stage = reinterpret_cast<C*>(acc + (size_t)acells * 2 * NBC);
   int* srow = reinterpret_cast<int*>(stage + (size_t)b.batch * SJ);
   const C* __restrict__ y = (const C*)y_;
   const C* __restrict__ u = (const C*)a.u;
@@ -338,8 +339,13 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
   }
   const int E1 = D > 1 ? E[1] : 1;
   const int E2 = D > 2 ? E[2] : 1;
+  // LDS strides: the last axis is contiguous, rows are padded by one cell so
+  // that a support box's rows fall on different banks
+  const int S2 = 1;
+  const int S1 = D == 3 ? E2 + 1 : 1;
+  const int S0 = D == 3 ? E1 * (E2 + 1) : (D == 2 ? E1 + 1 : 1);
 
-  for (int i = lane; i < cells * 2 * NBC; i += 64) acc[i] = (T)0;
+  for (int i = lane; i < acells * 2 * NBC; i += 64) acc[i] = (T)0;
 
   const int JL = a.J[D - 1];
   const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
@@ -376,7 +382,7 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
       }
       my_tot = single ? n[0] * n[1] * n[2] : -1;
       my_geo = lo[0] | (lo[1] << 5) | (lo[2] << 10) | (n[1] << 20) | (n[2] << 25);
-      my_start = (st0[0] * E1 + st0[1]) * E2 + st0[2];
+      my_start = st0[0] * S0 + st0[1] * S1 + st0[2] * S2;
       my_m2 = (65536 + n[2] - 1) / max(n[2], 1);
       my_m1 = (65536 + n[1] - 1) / max(n[1], 1);
     }
@@ -422,7 +428,7 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
             t1 = t % n1;
             t0 = t / n1;
           }
-          const int cell = start + (t0 * E1 + t1) * E2 + t2;
+          const int cell = start + t0 * S0 + t1 * S1 + t2 * S2;
           C w;
           if (CSRV) {
             const int j0 = lo0 + t0, j1 = lo1 + t1, j2 = lo2 + t2;
@@ -437,8 +443,8 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
 #pragma unroll
           for (int c = 0; c < NBC; ++c) {
             const C v = cmulc(w, yv[c]);
-            atomicAdd(&acc[(c * 2 + 0) * cells + cell], v.x);
-            atomicAdd(&acc[(c * 2 + 1) * cells + cell], v.y);
+            atomicAdd(&acc[(c * 2 + 0) * acells + cell], v.x);
+            atomicAdd(&acc[(c * 2 + 1) * acells + cell], v.y);
           }
         }
       } else {
@@ -475,12 +481,12 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
             if (D > 1) w = cmul(w, st[J0 + j1]);
             if (D > 2) w = cmul(w, st[J0 + J1 + j2]);
           }
-          const int cell = (l0 * E1 + l1) * E2 + l2;
+          const int cell = l0 * S0 + l1 * S1 + l2 * S2;
 #pragma unroll
           for (int c = 0; c < NBC; ++c) {
             const C v = cmulc(w, yv[c]);
-            atomicAdd(&acc[(c * 2 + 0) * cells + cell], v.x);
-            atomicAdd(&acc[(c * 2 + 1) * cells + cell], v.y);
+            atomicAdd(&acc[(c * 2 + 0) * acells + cell], v.x);
+            atomicAdd(&acc[(c * 2 + 1) * acells + cell], v.y);
           }
         }
       }
@@ -499,9 +505,10 @@ __device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
     int64_t g = (int64_t)(T0[0] + l0) * a.Kstride[0];
     if (D > 1) g += (int64_t)(T0[1] + l1) * a.Kstride[1];
     if (D > 2) g += (int64_t)(T0[2] + l2);
+    const int ac = l0 * S0 + l1 * S1 + l2 * S2;
 #pragma unroll
     for (int c = 0; c < NBC; ++c) {
-      const C v = mk<T>(acc[(c * 2 + 0) * cells + cell], acc[(c * 2 + 1) * cells + cell]);
+      const C v = mk<T>(acc[(c * 2 + 0) * acells + ac], acc[(c * 2 + 1) * acells + ac]);
       if (it.w < 0) grid[g * nb + c0 + c] = v;
       else part[((int64_t)it.w * cells + cell) * NBC + c] = v;
     }
@@ -627,7 +634,7 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b0, const void* y,
     int batch = 32;
     size_t wl = 0;
     for (;;) {
-      wl = (size_t)b.tile_cells * 2 * nbc * sizeof(T) + (size_t)batch * SJ * sizeof(C) +
+      wl = (size_t)b.acc_cells * 2 * nbc * sizeof(T) + (size_t)batch * SJ * sizeof(C) +
            (size_t)batch * sizeof(int);
       wl = (wl + 15) & ~(size_t)15;
       if (wl <= b.lds_budget) break;

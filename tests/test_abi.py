@@ -82,3 +82,37 @@ def test_operator_requires_gpu_or_raises_loudly():
         pytest.skip('device present')
     with pytest.raises(RuntimeError):
         NUFFT()
+
+
+def _fake_desc(ndim, Jd, tile, K=64, N=32, M=100):
+    """A descriptor whose pointers are non-null placeholders: only host-side
+    validation runs (the size query touches no device memory)."""
+    from pynufft_amd import _lib
+    d = _lib.PlanDesc()
+    d.ndim = ndim
+    d.batch = 1
+    d.M = M
+    for a in range(ndim):
+        d.Nd[a] = N
+        d.Kd[a] = K
+        d.Jd[a] = Jd
+        d.tile[a] = tile
+        d.tw_c64[a] = 0x1000
+        d.tw_c128[a] = 0x1000
+    for f in ('perm', 'base', 'u_c64', 'u_c128', 'sn_f32', 'items', 'bin_rows'):
+        setattr(d, f, 0x1000)
+    d.n_items = 4
+    return d
+
+
+def test_batch_validation_on_host(lib):
+    from pynufft_amd import _lib
+    n = ctypes.c_size_t()
+    ok = (_lib.PlanDesc * 2)(_fake_desc(2, 6, 32), _fake_desc(2, 6, 32, K=96, N=48))
+    assert lib.pnh_batch_table_bytes(ok, 2, ctypes.byref(n)) == 0 and n.value > 0
+    mixed = (_lib.PlanDesc * 2)(_fake_desc(2, 6, 32), _fake_desc(3, 6, 8))
+    assert lib.pnh_batch_table_bytes(mixed, 2, ctypes.byref(n)) == -1
+    assert b'ndim' in lib.pnh_last_error()
+    jd = (_lib.PlanDesc * 2)(_fake_desc(2, 6, 32), _fake_desc(2, 4, 32))
+    assert lib.pnh_batch_table_bytes(jd, 2, ctypes.byref(n)) == -1
+    assert lib.pnh_batch_table_bytes(ok, 0, ctypes.byref(n)) == -1
