@@ -33,21 +33,27 @@ def progress(msg):
 
 
 def algorithmic_bytes(kernel, fmt, M, Jd, Kd, nb=1):
-    """Bytes one launch must move at minimum (per-unit figures x units).
-
-    factor format, per sample: sum(Jd) complex64 factors + ndim int32 first
-    cells + int32 row->sample + nb complex64 samples; plus the Kd grid once.
-    CSR format, per sample: prod(Jd) complex64 coefficients + prod(Jd) int32
-    cells + int32 row->sample + nb complex64 samples; plus the grid once.
-    """
+    """Algorithmic bytes of one interpolation launch (k2y or y2k):
+    B = M*P*8 (complex64 coefficients) + I + M*8*nb (samples) + K*8*nb (the
+    grid once), P = prod(Jd), I = the index stream the format reads: M*P*4
+    (int32 cells, CSR table) or M*d*4 (int32 first cells, factor format)."""
     d = len(Kd)
     K = int(np.prod(Kd))
     P = int(np.prod(Jd))
-    if fmt == 'CSR':
-        per_sample = P * 8 + P * 4 + 4 + nb * 8
-    else:
-        per_sample = int(np.sum(Jd)) * 8 + d * 4 + 4 + nb * 8
-    return M * per_sample + K * 8 * nb
+    index = M * P * 4 if fmt == 'CSR' else M * d * 4
+    return M * P * 8 + index + M * 8 * nb + K * 8 * nb
+
+
+def physical_bytes(fmt, M, Jd, Kd, nb=1):
+    """Bytes the format must at least move per launch: the factor format
+    reads sum(Jd) complex64 factors per sample instead of prod(Jd)
+    coefficients, plus d first cells, the row->sample index, the samples
+    and the grid once."""
+    d = len(Kd)
+    K = int(np.prod(Kd))
+    P = int(np.prod(Jd))
+    coef = P * 12 if fmt == 'CSR' else int(np.sum(Jd)) * 8 + d * 4
+    return M * (coef + 4 + 8 * nb) + K * 8 * nb
 
 
 def fft_bytes(Nd, Kd, nb=1):
@@ -295,7 +301,8 @@ def main():
         'roofline': {'bound': 'hbm', 'kernel': kname, 'achieved': achieved,
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                     'algorithmic_bytes': nbytes, 'launch_ms': kms},
+                     'algorithmic_bytes': nbytes, 'launch_ms': kms,
+                     'physical_bytes': physical_bytes(fmt, M, w['Jd'], w['Kd'])},
         'kernels_ms': {'k2y': t_k2y, 'y2k': t_y2k, 'xx2k': t_fft},
         'pair_hbm_frac': pair_bytes * value / 1e9 / HBM_PEAK_GBS,
         'pair_algorithmic_bytes': pair_bytes,

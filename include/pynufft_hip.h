@@ -77,14 +77,21 @@ typedef struct pnh_plan_desc {
   const void* tw_c64[PNH_MAXD];   /* exp(-2 pi i t / Kd[d]), t < Kd[d] */
   const void* tw_c128[PNH_MAXD];  /* same in complex128, may be NULL */
 
+  /* Gridding (y2k) work list.  Rows are sorted by the tile that holds their
+   * first support cell (C order over the tile grid), so a tile owns one
+   * contiguous run of rows, cut into items of at most 4096 rows.  Per axis
+   * either tile >= Kd (one tile spans the axis) or tile >= Jd - 1 and the
+   * last tile's extent Kd - (ceil(Kd / tile) - 1) * tile >= Jd - 1. */
   int32_t tile[PNH_MAXD];    /* gridding tile extent per dim */
+  double u_bound;            /* >= |coefficient| for every row and support offset */
   int64_t n_items;           /* gridding work items */
-  const int32_t* items;      /* [n_items][4]: tile id, entry start, entry count,
-                                partial slot (-1 = writes the grid directly) */
-  const int32_t* bin_rows;   /* work item entries (sorted row ids) */
-  int64_t n_merge;           /* split tiles */
-  const int32_t* merge;      /* [n_merge][3]: tile id, first slot, slot count */
-  int64_t n_slots;           /* partial tile slots used by split items */
+  const int32_t* items;      /* [n_items][4]: tile id, first row, rows, partial slot */
+  int64_t n_tiles;           /* prod over dims of ceil(Kd / tile) */
+  const int32_t* tile_slots; /* [n_tiles][2]: first partial slot, slot count; the
+                                slots of one tile are consecutive, in row order */
+  int64_t n_slots;           /* partial slots (one per item) */
+  int32_t threads;           /* gridding workgroup size: multiple of 64 in
+                                64..1024, or 0 to let the library choose */
 } pnh_plan_desc;
 
 typedef struct pnh_plan pnh_plan;

@@ -41,12 +41,13 @@ class PlanDesc(ctypes.Structure):
         ('tw_c64', _vp * MAXD),
         ('tw_c128', _vp * MAXD),
         ('tile', ctypes.c_int32 * MAXD),
+        ('u_bound', ctypes.c_double),
         ('n_items', ctypes.c_int64),
         ('items', _vp),
-        ('bin_rows', _vp),
-        ('n_merge', ctypes.c_int64),
-        ('merge', _vp),
+        ('n_tiles', ctypes.c_int64),
+        ('tile_slots', _vp),
         ('n_slots', ctypes.c_int64),
+        ('threads', ctypes.c_int32),
     ]
 
 

@@ -33,26 +33,25 @@ class BatchNUFFT:
         self._graphs = {}
         self._ws = {}
 
-    def plan(self, problems, tile=None, chunk=None):
-        """``problems``: sequence of (om, Nd, Kd, Jd) (or dicts with those keys)."""
+    def plan(self, problems, tile=None, chunk=None, threads=0):
+        """``problems``: sequence of (om, Nd, Kd, Jd) (or dicts with those keys).
+        ``tile``/``chunk``/``threads``: gridding knobs applied to every
+        problem (default: each problem's own tile, see ``NUFFT.plan``)."""
         self.release()
         probs = [(p['om'], p['Nd'], p['Kd'], p['Jd']) if isinstance(p, dict) else p
                  for p in problems]
         if not probs:
             raise ValueError('need at least one problem')
-        if tile is None:
-            tiles = [helper.tile_shape(Kd) for (_, _, Kd, _) in probs]
-            tile = tuple(min(t[a] for t in tiles) for a in range(len(tiles[0])))
         # host FP64 designs in parallel (numpy releases the GIL), then upload
         with concurrent.futures.ThreadPoolExecutor(8) as pool:
             sts = list(pool.map(
                 lambda p: helper.plan(p[0], tuple(p[1]), tuple(p[2]), tuple(p[3]),
-                                      tile=tile, chunk_entries=chunk), probs))
+                                      tile=tile, chunk_rows=chunk), probs))
         self.ops = []
         for (om, Nd, Kd, Jd), st in zip(probs, sts):
             op = NUFFT(self.device)
             op.plan(om, tuple(Nd), tuple(Kd), tuple(Jd), tile=tile, chunk=chunk,
-                    host_plan=st)
+                    host_plan=st, threads=threads)
             op._ensure_fp64()
             self.ops.append(op)
         n = len(self.ops)

@@ -30,7 +30,8 @@ void pnh_set_error(const char* fmt, ...) {
     }                                 \
   } while (0)
 
-static const size_t kLdsBudget = 65536;
+static const size_t kLdsBudget = 65536;      // FFT line buffers
+static const int64_t kGridLdsMax = 163840;   // largest gridding accumulator + scratch
 
 struct pnh_plan {
   pnh_plan_desc d;
@@ -39,7 +40,7 @@ struct pnh_plan {
   int64_t N[PNH_MAXD], K[PNH_MAXD];
   int J[PNH_MAXD];
   BinArgs bins;
-  int slot_nbc;            // coils per y2k launch chunk used to size partial slots
+  int slot_nbc;            // coils per partial slot element (min(nb, 4))
 };
 
 namespace {
@@ -73,7 +74,7 @@ InterpArgs interp_args(const pnh_plan* p, int fp64) {
 
 size_t partial_bytes(const pnh_plan* p, int fp64) {
   const size_t esz = fp64 ? 16 : 8;
-  return (size_t)p->d.n_slots * p->bins.tile_cells * p->slot_nbc * esz;
+  return (size_t)p->d.n_slots * p->bins.pad_cells * p->slot_nbc * esz;
 }
 
 // Build the line-FFT descriptor for axis `a`.
@@ -239,7 +240,7 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   p->M = d.M;
   p->Ntot = 1;
   p->Ktot = 1;
-  int64_t cells = 1;
+  int64_t pad_cells = 1, n_tiles = 1;
   for (int a = 0; a < p->D; ++a) {
     p->N[a] = d.Nd[a];
     p->K[a] = d.Kd[a];
@@ -250,33 +251,53 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
             (long long)d.Nd[a], (long long)d.Kd[a], d.Jd[a], d.tile[a]);
     p->Ntot *= d.Nd[a];
     p->Ktot *= d.Kd[a];
-    p->bins.tile[a] = d.tile[a];
-    p->bins.tgrid[a] = (int)((d.Kd[a] + d.tile[a] - 1) / d.tile[a]);
-    cells *= d.tile[a];
+    const int64_t tile = d.tile[a] < d.Kd[a] ? d.tile[a] : d.Kd[a];
+    const int64_t tg = (d.Kd[a] + tile - 1) / tile;
+    p->bins.tile[a] = (int)tile;
+    p->bins.tgrid[a] = (int)tg;
+    if (tg == 1) {
+      p->bins.pad[a] = (int)d.Kd[a];
+    } else {
+      REQUIRE(tile >= d.Jd[a] - 1 && d.Kd[a] - (tg - 1) * tile >= d.Jd[a] - 1,
+              "gridding tile %lld on axis %d leaves a tile narrower than Jd - 1 = %d "
+              "(Kd = %lld)", (long long)tile, a, d.Jd[a] - 1, (long long)d.Kd[a]);
+      p->bins.pad[a] = (int)(tile + d.Jd[a] - 1);
+    }
+    pad_cells *= p->bins.pad[a];
+    n_tiles *= tg;
     int radix[PNH_FFT_MAX_PASS], ns[PNH_FFT_MAX_PASS], np;
     if (pnh_fft_factor((int)d.Kd[a], radix, ns, &np)) return -2;
     REQUIRE(d.tw_c64[a], "tw_c64[%d] missing", a);
   }
   REQUIRE(pnh_lanes_per_sample(d.Jd[d.ndim - 1]) > 0, "Jd above 16 on the last axis");
-  p->bins.tile_cells = (int)cells;
-  // LDS accumulator: last axis contiguous, rows padded by one cell
-  if (p->D == 1) p->bins.acc_cells = d.tile[0];
-  else if (p->D == 2) p->bins.acc_cells = d.tile[0] * (d.tile[1] + 1);
-  else p->bins.acc_cells = d.tile[0] * d.tile[1] * (d.tile[2] + 1);
-  p->bins.waves_per_block = 4;
-  p->bins.lds_budget = kLdsBudget;
-  if ((size_t)p->bins.acc_cells * 2 * sizeof(double) > kLdsBudget) {
-    pnh_set_error("gridding tile of %lld cells exceeds the LDS budget", (long long)cells);
+  for (int a = 0; a + 1 < p->D; ++a)
+    REQUIRE(d.Jd[a] <= 16, "Jd above 16 on axis %d", a);
+  if (pad_cells * 16 + 256 > kGridLdsMax) {
+    pnh_set_error("gridding tile needs %lld cells of LDS accumulator (limit %lld)",
+                  (long long)pad_cells, (long long)((kGridLdsMax - 256) / 16));
     return -2;
   }
+  p->bins.pad_cells = pad_cells;
+  p->bins.n_tiles = n_tiles;
+  REQUIRE(d.n_tiles == n_tiles, "n_tiles is %lld, the tile grid has %lld tiles",
+          (long long)d.n_tiles, (long long)n_tiles);
+  REQUIRE(d.tile_slots, "tile_slots is required");
+  REQUIRE(d.n_items >= 0 && (d.n_items == 0 || d.items), "gridding work items are required");
+  REQUIRE(d.n_slots >= d.n_items, "n_slots below n_items");
+  REQUIRE(d.M == 0 || (d.u_bound > 0.0 && d.u_bound < 1e300), "u_bound must be positive");
+  p->bins.u_bound = d.u_bound;
   p->bins.n_items = d.n_items;
   p->bins.items = d.items;
-  p->bins.bin_rows = d.bin_rows;
-  p->bins.n_merge = d.n_merge;
-  p->bins.merge = d.merge;
+  p->bins.tile_slots = d.tile_slots;
+  if (d.threads == 0) {
+    const int64_t bytes = pad_cells * 16;
+    p->bins.threads = bytes > 81920 ? 1024 : (bytes > 40960 ? 512 : 256);
+  } else {
+    REQUIRE(d.threads >= 64 && d.threads <= 1024 && d.threads % 64 == 0,
+            "threads must be a multiple of 64 in 64..1024 (got %d)", d.threads);
+    p->bins.threads = d.threads;
+  }
   p->slot_nbc = p->nb >= 4 ? 4 : (p->nb >= 2 ? 2 : 1);
-  REQUIRE(d.n_items > 0 && d.items && (d.bin_rows || d.M == 0),
-          "gridding work items are required");
   return 0;
 }
 
@@ -496,15 +517,13 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 #include "batch.h"
 
 struct pnh_batch {
-  int n, D, J_last, SJ, tile_cells, acc_cells;
-  int64_t N_total, M_total, K_total, slots_total;
+  int n, D, J_last, J_max, grid_threads;
+  int64_t N_total, M_total, K_total, part_elems, pad_cells_max;
   int fft_threads;
   BatchProblem* probs;                 // device (inside the caller's table)
   BatchTable k2y_t;
   const int2* items;
   int64_t n_items;
-  const int2* merges;
-  int64_t n_merge;
   BatchTable fft_t[3][PNH_MAXD];       // [kind][axis]
   size_t fft_lds[3][PNH_MAXD];
   int fft_pow2[3][PNH_MAXD];
@@ -547,14 +566,6 @@ void add_table(Blob& blob, const std::vector<int64_t>& blocks_per_problem,
   *n_blocks = acc;
 }
 
-size_t y2k_wave_lds(int cells, int SJ, int fp64, int batch_entries) {
-  // `cells` is the padded accumulator size
-  const size_t esz = fp64 ? 8 : 4;
-  size_t wl = (size_t)cells * 2 * esz + (size_t)batch_entries * SJ * 2 * esz +
-              (size_t)batch_entries * sizeof(int);
-  return (wl + 15) & ~(size_t)15;
-}
-
 // Build every host-side table of a batch.  With `table` NULL only sizes are
 // computed; otherwise the blob is copied to `table` and device pointers set.
 int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stream,
@@ -573,7 +584,6 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
       REQUIRE(p.D == plans[0].D, "all problems must have the same ndim");
       for (int a = 0; a < p.D; ++a) {
         REQUIRE(p.J[a] == plans[0].J[a], "all problems must have the same Jd");
-        REQUIRE(p.bins.tile[a] == plans[0].bins.tile[a], "all problems must share the tile shape");
       }
     }
   }
@@ -581,10 +591,14 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   b->n = n;
   b->D = D;
   b->J_last = plans[0].J[D - 1];
-  b->SJ = 0;
-  for (int a = 0; a < D; ++a) b->SJ += plans[0].J[a];
-  b->tile_cells = plans[0].bins.tile_cells;
-  b->acc_cells = plans[0].bins.acc_cells;
+  b->J_max = 1;
+  for (int a = 0; a < D; ++a) b->J_max = std::max(b->J_max, plans[0].J[a]);
+  b->pad_cells_max = 1;
+  b->grid_threads = 64;
+  for (int i = 0; i < n; ++i) {
+    b->pad_cells_max = std::max(b->pad_cells_max, plans[i].bins.pad_cells);
+    b->grid_threads = std::max(b->grid_threads, plans[i].bins.threads);
+  }
   b->fft_threads = 256;
   b->vec_chunk = kVecChunk;
   std::vector<BatchProblem> probs(n);
@@ -609,9 +623,9 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
       q.tw32[a] = p.d.tw_c64[a];
       q.tw64[a] = p.d.tw_c128[a];
     }
-    xo += p.Ntot; yo += p.M; ko += p.Ktot; so += p.d.n_slots;
+    xo += p.Ntot; yo += p.M; ko += p.Ktot; so += p.d.n_slots * p.bins.pad_cells;
   }
-  b->N_total = xo; b->M_total = yo; b->K_total = ko; b->slots_total = so;
+  b->N_total = xo; b->M_total = yo; b->K_total = ko; b->part_elems = so;
 
   Blob blob;
   const size_t off_probs = blob.add(probs.data(), probs.size() * sizeof(BatchProblem));
@@ -654,7 +668,6 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   }
   // gridding items of all problems, heaviest first (needs the counts)
   std::vector<std::pair<int, int2>> all;  // (count, (problem, item))
-  std::vector<int2> merges;
   for (int i = 0; i < n; ++i) {
     std::vector<int4> it(plans[i].d.n_items);
     if (table) {
@@ -663,7 +676,6 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
     }
     for (int64_t k = 0; k < plans[i].d.n_items; ++k)
       all.push_back(std::make_pair(table ? it[k].z : 0, make_int2(i, (int)k)));
-    for (int64_t k = 0; k < plans[i].d.n_merge; ++k) merges.push_back(make_int2(i, (int)k));
   }
   std::stable_sort(all.begin(), all.end(), [](const std::pair<int, int2>& x,
                                               const std::pair<int, int2>& y) {
@@ -672,13 +684,8 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   std::vector<int2> items(all.size());
   for (size_t k = 0; k < all.size(); ++k) items[k] = all[k].second;
   if (items.empty()) items.push_back(make_int2(0, 0));
-  if (merges.empty()) merges.push_back(make_int2(0, 0));
   const size_t oi = blob.add(items.data(), items.size() * sizeof(int2));
-  const size_t om = blob.add(merges.data(), merges.size() * sizeof(int2));
   b->n_items = (int64_t)all.size();
-  int64_t nm = 0;
-  for (int i = 0; i < n; ++i) nm += plans[i].d.n_merge;
-  b->n_merge = nm;
   b->table_bytes = blob.bytes.size();
   if (!table) return 0;
 
@@ -699,26 +706,7 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
     b->vec_t[e].first = (const int*)(base + ov_first[e]);
   }
   b->items = (const int2*)(base + oi);
-  b->merges = (const int2*)(base + om);
   return 0;
-}
-
-struct BatchY2kCfg {
-  int wave_lds, batch_entries, waves;
-};
-
-BatchY2kCfg y2k_cfg(const pnh_batch* b, int fp64) {
-  BatchY2kCfg c;
-  c.batch_entries = 32;
-  size_t wl = y2k_wave_lds(b->acc_cells, b->SJ, fp64, c.batch_entries);
-  while (wl > kLdsBudget && c.batch_entries > 4) {
-    c.batch_entries >>= 1;
-    wl = y2k_wave_lds(b->acc_cells, b->SJ, fp64, c.batch_entries);
-  }
-  c.wave_lds = (int)wl;
-  c.waves = 4;
-  while (c.waves > 1 && (size_t)c.waves * wl > kLdsBudget) --c.waves;
-  return c;
 }
 
 // Forward kind: image -> grid, last axis first (image read by the first
@@ -742,9 +730,8 @@ int batch_passes(const pnh_batch* b, int kind, int fp64, const void* image_in,
 
 int batch_y2k(const pnh_batch* b, int fp64, const void* y, void* k, void* part,
               const int* done, const int* all_done, hipStream_t s) {
-  const BatchY2kCfg c = y2k_cfg(b, fp64);
-  return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->merges, b->n_merge, b->D,
-                              fp64, b->tile_cells, c.wave_lds, c.batch_entries, c.waves, y, k,
+  return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->vec_t[0], b->vec_chunk,
+                              b->D, b->J_max, fp64, b->pad_cells_max, b->grid_threads, y, k,
                               part, done, all_done, s);
 }
 
@@ -768,7 +755,7 @@ size_t batch_cg_layout(const pnh_batch* b, void* ws, BatchWs* w) {
   w->p = take(g2);
   w->q = take(g2);
   w->yt = take((size_t)b->M_total * 16);
-  w->part = take((size_t)b->slots_total * b->tile_cells * 16);
+  w->part = take((size_t)b->part_elems * 16);
   w->cg.rr = (double*)take(n * 8);
   w->cg.rr_prev = (double*)take(n * 8);
   w->cg.tol = (double*)take(n * 8);
@@ -872,7 +859,7 @@ int pnh_batch_workspace_bytes(const pnh_batch* b, int op, int dtype, size_t* byt
   REQUIRE(b && bytes, "null argument");
   (void)dtype;
   const size_t grid = align_up((size_t)b->K_total * 8);
-  const size_t part = align_up((size_t)b->slots_total * b->tile_cells * 8);
+  const size_t part = align_up((size_t)b->part_elems * 8);
   switch (op) {
     case PNH_OP_FORWARD: *bytes = grid; break;
     case PNH_OP_ADJOINT: *bytes = grid + part; break;

@@ -11,7 +11,7 @@ struct BatchProblem {
   InterpArgs ia64;           // complex128 factors
   BinArgs bins;              // this problem's gridding work list
   int64_t x_off, y_off, k_off;
-  int64_t part_off;          // first partial-tile slot of this problem
+  int64_t part_off;          // first element of this problem's partial slots
   int64_t n_img, n_grid;     // prod(Nd), prod(Kd)
   const float* sn32;
   FftDesc fwd[PNH_MAXD];     // forward passes, fp32, x2xx fused in the first
@@ -44,11 +44,12 @@ struct BatchCg {
 int pnh_batch_k2y_launch(const BatchProblem* probs, const BatchTable& t, int D, int J_last,
                          int fp64, const void* k, void* y, const int* done,
                          const int* all_done, hipStream_t s);
+// Grids every problem's items into partial slots, then gathers the grids
+// (blocks of `chunk` cells by table `gather`).
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const int2* merges, int64_t n_merge, int D, int fp64,
-                         int tile_cells, int wave_lds, int batch_entries, int waves,
-                         const void* y, void* k, void* part, const int* done,
-                         const int* all_done, hipStream_t s);
+                         const BatchTable& gather, int64_t chunk, int D, int jmax, int fp64,
+                         int64_t pad_cells, int threads, const void* y, void* k, void* part,
+                         const int* done, const int* all_done, hipStream_t s);
 int pnh_batch_fft_launch(const BatchProblem* probs, const BatchTable& t, int axis,
                          int kind, int fp64, int pow2, size_t lds, int threads,
                          const void* in, int in_image, void* out, int out_image,

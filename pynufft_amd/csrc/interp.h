@@ -24,17 +24,15 @@ struct InterpArgs {
 struct BinArgs {
   int tile[PNH_MAXD];          // tile extent per dim
   int tgrid[PNH_MAXD];         // tiles per dim
-  int tile_cells;              // prod(tile)
-  int acc_cells;               // LDS accumulator cells (last-axis rows padded by one)
-  int waves_per_block;         // set per launch
-  int batch;                   // entries staged per step (set per launch)
-  int wave_lds_bytes;          // LDS bytes per wavefront (set per launch)
-  size_t lds_budget;
+  int pad[PNH_MAXD];           // accumulator extent per dim: tile + J - 1, or K
+                               // when one tile spans the axis (wraps inside)
+  int64_t pad_cells;           // prod(pad): cells of one partial slot
+  int64_t n_tiles;             // prod(tgrid)
+  double u_bound;              // bound of |coefficient| over all rows and offsets
+  int threads;                 // workgroup size of the gridding kernel
   int64_t n_items;
-  const int* items;            // [n_items][4]
-  const int* bin_rows;
-  int64_t n_merge;
-  const int* merge;            // [n_merge][3]
+  const int* items;            // [n_items][4] tile id, first row, rows, slot
+  const int* tile_slots;       // [n_tiles][2] first slot, slot count
 };
 
 // Lanes per sample used by k2y (and by the CSR table layout): one lane per
@@ -52,6 +50,13 @@ int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
 // out[m*D + d] (+)= sum_c Re(conj(g[m, c]) * sum_j d_d coef(m, j) * grid[cell(m, j), c])
 int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
                        const void* g, void* out, hipStream_t s);
+// grid[cell*nb + c] = sum over rows r of conj(coef(r, j)) * y[perm[r]*nb + c]
+// for every support cell of r; `part` holds n_slots * pad_cells * min(nb, 4)
+// elements of the operator's complex type.
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);
+// LDS bytes available to one workgroup on the current device.
+size_t pnh_lds_limit();
+// Coils gridded per pass for this tile and LDS limit (0 if none fits).
+int pnh_y2k_coils_per_pass(const BinArgs& b, int nb);

@@ -9,12 +9,11 @@
 // rows interleaved in slices of 64 so that lane r%64 reads entry j of its row
 // with unit stride).  The support cell along dim d is (base_d + j_d) mod K_d.
 //
-// y2k is a gather-free deterministic gridding: the K grid is cut into tiles,
-// each tile has the sorted list of rows whose support meets it, and one
-// wavefront accumulates one tile (or one chunk of a heavy tile) in its own
-// LDS region, one sample per step, so every cell receives its contributions
-// in the same order on every run.  Heavy tiles are split into chunks whose
-// partial tiles are summed in chunk order by a merge kernel.  No float
+// y2k is deterministic gridding by tile ownership: every row is gridded by
+// the tile that holds its first support cell, into an LDS accumulator padded
+// to hold whole support boxes, with exact 64-bit fixed-point integer atomics
+// (order-independent, hence bit-reproducible); a gather pass then sums the
+// padded partial tiles that cover each grid cell in a fixed order.  No float
 // atomics touch global memory.
 #include "common.h"
 #include "interp.h"
@@ -270,310 +269,323 @@ k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
 }
 
 // ---------------------------------------------------------------- y2k ----
-// Support offsets j in [0, J) whose cell (b + j) mod K lies in the tile
-// [T0, T0 + E).  Returns 1 and one interval [lo, lo + n) when the hit cells
-// are one contiguous run (the common case), 0 when they are two runs (a
-// wrapped support meeting both ends of a tile that spans or nearly spans the
-// axis).
-__device__ __forceinline__ int clip_axis(int b, int J, int K, int T0, int E,
-                                         int& lo, int& n) {
-  if (E == K) {
-    // the tile spans the axis: the hit cells are contiguous from b unless
-    // the support wraps past K - 1
-    lo = 0; n = J;
-    return b + J <= K ? 1 : 0;
-  }
-  const int T1 = T0 + E;
-  const int l1 = max(0, T0 - b), h1 = min(min(J, K - b), T1 - b);
-  const int l2 = max(K - b, T0 - b + K), h2 = min(J, T1 - b + K);
-  const bool a1 = h1 > l1, a2 = h2 > l2;
-  if (a1 && a2) { lo = 0; n = 0; return 0; }
-  if (a1) { lo = l1; n = h1 - l1; }
-  else if (a2) { lo = l2; n = h2 - l2; }
-  else { lo = 0; n = 0; }
-  return 1;
+// Gridding by tile ownership.  A row belongs to the tile that holds its
+// first support cell; rows are sorted by that tile, so each tile owns one
+// contiguous run of rows, cut into work items of at most kMaxItemRows.  One
+// workgroup grids one item into an LDS accumulator that covers the tile
+// plus J_d - 1 cells on the high side of every axis split into several
+// tiles (an axis spanned by one tile wraps inside the accumulator instead),
+// so every support box lies wholly inside and nothing is clipped.  Lanes
+// work on different samples at once: LPS lanes per sample, one per
+// last-axis offset, each looping over the other axes.
+//
+// Contributions are summed in 64-bit fixed point.  With 2^e bounding
+// |Re| and |Im| of every contribution of the item, each is rounded to a
+// multiple of 2^(e - 50) and added with an integer LDS atomic.  Integer
+// addition is exact, so the accumulator does not depend on the order in
+// which lanes and waves arrive and the result is bit-reproducible, while
+// the rounding (2^-50 of the item's largest contribution) is far below
+// complex64 resolution.  An item has at most 4096 rows, so a cell sums at
+// most 2^12 terms below 2^50 and cannot overflow.
+//
+// The finished accumulator is written, scaled back, to the item's partial
+// slot.  The gather pass then forms every grid cell as the sum of the slots
+// that cover it -- its own tile's and those of the up to 2^D - 1 lower
+// neighbours whose padding reaches it -- in a fixed order.  No float
+// atomics touch global memory.
+
+constexpr int kMaxItemRows = 4096;
+constexpr int kFixBits = 50;
+
+// nearest integer to v * scale for |v * scale| < 2^51: adding 1.5 * 2^52
+// leaves that integer in the low mantissa bits of the double
+__device__ __forceinline__ long long to_fixed(double v, double scale) {
+  return __double_as_longlong(__builtin_fma(v, scale, 6755399441055744.0)) -
+         0x4338000000000000LL;
 }
 
-// q -> (q / n, q % n) for q < 64, n <= 64 via a 16-bit reciprocal
-__device__ __forceinline__ void divmod_small(int q, int n, int magic, int& d, int& m) {
-  d = (q * magic) >> 16;
-  m = q - d * n;
+__device__ __forceinline__ void lds_add(long long* p, long long v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Gridding of one work item (tile id, entry start, entry count, slot) by one
-// wavefront with its own LDS accumulator.
-// Entries are taken in batches: lanes load one entry's row, sample value and
-// first cells each (and stage its per-axis factors in LDS); the entries of
-// the batch are then applied one after another, lanes spread over the
-// support cells that fall inside the tile.
-template <typename T, int D, bool CSRV, int NBC>
-__device__ __forceinline__ void y2k_body(const InterpArgs& a, const BinArgs& b,
-                                         const int4 it, unsigned char* wave_lds,
-                                         const void* __restrict__ y_,
-                                         void* __restrict__ grid_,
-                                         void* __restrict__ part_, int c0) {
-  typedef typename cplx<T>::t C;
-  const int lane = threadIdx.x & 63;
-  const int cells = b.tile_cells;      // compact tile (partials, output)
-  const int acells = b.acc_cells;      // LDS accumulator (padded rows)
-  const int J0 = a.J[0];
-  const int J1 = D > 1 ? a.J[1] : 1;
-  const int J2 = D > 2 ? a.J[2] : 1;
-  const int SJ = J0 + (D > 1 ? J1 : 0) + (D > 2 ? J2 : 0);
-  T* acc = reinterpret_cast<T*>(wave_lds);
-  C* stage = reinterpret_cast<C*>(acc + (size_t)acells * 2 * NBC);
-  int* srow = reinterpret_cast<int*>(stage + (size_t)b.batch * SJ);
-  const C* __restrict__ y = (const C*)y_;
-  const C* __restrict__ u = (const C*)a.u;
-  const int nb = a.nb;
+template <typename C>
+__device__ __forceinline__ void lds_add_c(long long* cell, C v, double scale) {
+  lds_add(cell, to_fixed((double)v.x, scale));
+  lds_add(cell + 1, to_fixed((double)v.y, scale));
+}
 
-  int tid = it.x;
-  int T0[PNH_MAXD], E[PNH_MAXD];
+// Complex product with contraction allowed (fused multiply-adds).
+template <typename C>
+__device__ __forceinline__ C cmulf(C a, C b) {
+  C z;
+  z.x = __builtin_fma(a.x, b.x, -(a.y * b.y));
+  z.y = __builtin_fma(a.x, b.y, a.y * b.x);
+  return z;
+}
+
+// Origin of a tile and, per axis, the index at which the accumulator wraps
+// (K_d for an axis spanned by one tile, else never).
+struct TileFrame {
+  int T0[PNH_MAXD];
+  int wrap[PNH_MAXD];
+};
+
+template <int D>
+__device__ __forceinline__ TileFrame tile_frame(const InterpArgs& a, const BinArgs& b, int tid) {
+  TileFrame f;
 #pragma unroll
   for (int d = D - 1; d >= 0; --d) {
     const int tg = b.tgrid[d];
     const int td = tid % tg;
     tid /= tg;
-    T0[d] = td * b.tile[d];
-    E[d] = min(b.tile[d], a.K[d] - T0[d]);
+    f.T0[d] = td * b.tile[d];
+    f.wrap[d] = tg == 1 ? a.K[d] : 0x40000000;
   }
-  const int E1 = D > 1 ? E[1] : 1;
-  const int E2 = D > 2 ? E[2] : 1;
-  // LDS strides: the last axis is contiguous, rows are padded by one cell so
-  // that a support box's rows fall on different banks
-  const int S2 = 1;
-  const int S1 = D == 3 ? E2 + 1 : 1;
-  const int S0 = D == 3 ? E1 * (E2 + 1) : (D == 2 ? E1 + 1 : 1);
+  return f;
+}
 
-  for (int i = lane; i < acells * 2 * NBC; i += 64) acc[i] = (T)0;
+__device__ __forceinline__ int wrap_local(int x, int wrap, int K) {
+  return x >= wrap ? x - K : x;
+}
 
-  const int JL = a.J[D - 1];
-  const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
-  const int LPS = pnh_lanes_per_sample(JL);
-  const int SPW = 64 / LPS;
-  const int* __restrict__ rows = b.bin_rows + it.y;
-  const int cnt = it.z;
-  const int BATCH = b.batch;
+// Max over the workgroup; `red` holds one slot per wave.
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) red[wave] = v;
+  __syncthreads();
+  T r = red[0];
+  for (int w = 1; w < nw; ++w) r = fmax(r, red[w]);
+  return r;
+}
 
-  for (int e0 = 0; e0 < cnt; e0 += BATCH) {
-    const int ne = min(BATCH, cnt - e0);
-    // ---- lane-parallel setup: lane l prepares entry e0 + l ----
-    int my_r = 0;
-    int my_b[PNH_MAXD] = {0, 0, 0};
-    C my_y[NBC];
-#pragma unroll
-    for (int c = 0; c < NBC; ++c) my_y[c] = mk<T>((T)0, (T)0);
-    int my_geo = 0, my_start = 0, my_m1 = 0, my_m2 = 0, my_tot = 0;
-    if (lane < ne) {
-      my_r = rows[e0 + lane];
-      const int64_t m = a.perm[my_r];
-#pragma unroll
-      for (int d = 0; d < D; ++d) my_b[d] = a.base[d * a.M + my_r];
-#pragma unroll
-      for (int c = 0; c < NBC; ++c) my_y[c] = y[m * nb + c0 + c];
-      // clip the support box to the tile: per axis first offset lo, count n
-      // and tile-local first cell s; flag 1 << 30 when some axis splits
-      int lo[PNH_MAXD] = {0, 0, 0}, n[PNH_MAXD] = {1, 1, 1}, st0[PNH_MAXD] = {0, 0, 0};
-      int single = 1;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        single &= clip_axis(my_b[d], a.J[d], a.K[d], T0[d], E[d], lo[d], n[d]);
-        st0[d] = wrap_add(my_b[d], lo[d], a.K[d]) - T0[d];
-      }
-      my_tot = single ? n[0] * n[1] * n[2] : -1;
-      my_geo = lo[0] | (lo[1] << 5) | (lo[2] << 10) | (n[1] << 20) | (n[2] << 25);
-      my_start = st0[0] * S0 + st0[1] * S1 + st0[2] * S2;
-      my_m2 = (65536 + n[2] - 1) / max(n[2], 1);
-      my_m1 = (65536 + n[1] - 1) / max(n[1], 1);
-    }
-    if (!CSRV) {
-      // rows of the batch through LDS (a lane may read any entry's row)
-      if (lane < ne) srow[lane] = my_r;
-      for (int idx = lane; idx < ne * SJ; idx += 64) {
-        const int ei = idx / SJ;
-        const int q = idx - ei * SJ;
-        const int64_t re = srow[ei];
-        int64_t src;
-        if (q < J0) src = a.u_off[0] + re * J0 + q;
-        else if (D > 1 && q < J0 + J1) src = a.u_off[1] + re * J1 + (q - J0);
-        else src = a.u_off[2] + re * J2 + (q - J0 - J1);
-        stage[idx] = u[src];
-      }
-    }
-    // ---- apply the entries in order ----
-    for (int i = 0; i < ne; ++i) {
-      const int tot = rl_i(my_tot, i);
-      C yv[NBC];
-#pragma unroll
-      for (int c = 0; c < NBC; ++c) yv[c] = rl_c(my_y[c], i);
-      const C* st = stage + i * SJ;
-      const int r = rl_i(my_r, i);
-      if (tot >= 0) {
-        const int geo = rl_i(my_geo, i);
-        const int start = rl_i(my_start, i);
-        const int m1 = rl_i(my_m1, i), m2 = rl_i(my_m2, i);
-        const int lo0 = geo & 31, lo1 = (geo >> 5) & 31, lo2 = (geo >> 10) & 31;
-        const int n1 = (geo >> 20) & 31, n2 = (geo >> 25) & 31;
-        for (int q = lane; q < tot; q += 64) {
-          int t0, t1, t2;
-          if (tot <= 256) {
-            // exact for q < 256 and n <= 16 (16-bit reciprocals)
-            const int t = (q * m2) >> 16;
-            t2 = q - t * n2;
-            t0 = (t * m1) >> 16;
-            t1 = t - t0 * n1;
-          } else {
-            t2 = q % n2;
-            const int t = q / n2;
-            t1 = t % n1;
-            t0 = t / n1;
-          }
-          const int cell = start + t0 * S0 + t1 * S1 + t2 * S2;
-          C w;
-          if (CSRV) {
-            const int j0 = lo0 + t0, j1 = lo1 + t1, j2 = lo2 + t2;
-            const int g = D == 1 ? 0 : (D == 2 ? j0 : j0 * J1 + j1);
-            const int jlast = D == 1 ? j0 : (D == 2 ? j1 : j2);
-            w = ((const C*)a.csr_val)[csr_index(r, g, jlast, G, JL, SPW)];
-          } else {
-            w = st[lo0 + t0];
-            if (D > 1) w = cmul(w, st[J0 + lo1 + t1]);
-            if (D > 2) w = cmul(w, st[J0 + J1 + lo2 + t2]);
-          }
-#pragma unroll
-          for (int c = 0; c < NBC; ++c) {
-            const C v = cmulc(w, yv[c]);
-            atomicAdd(&acc[(c * 2 + 0) * acells + cell], v.x);
-            atomicAdd(&acc[(c * 2 + 1) * acells + cell], v.y);
-          }
-        }
-      } else {
-        // a wrapped support meets both ends of this tile: masked walk over
-        // the whole support box
-        int bb[PNH_MAXD];
-#pragma unroll
-        for (int d = 0; d < D; ++d) bb[d] = rl_i(my_b[d], i);
-        const int P = J0 * J1 * J2;
-        const int J12 = J1 * J2;
-        for (int q = lane; q < P; q += 64) {
-          const int j0 = q / J12;
-          const int rem = q - j0 * J12;
-          const int j1 = D > 1 ? rem / J2 : 0;
-          const int j2 = D > 2 ? rem - j1 * J2 : 0;
-          const int l0 = wrap_add(bb[0], j0, a.K[0]) - T0[0];
-          if ((unsigned)l0 >= (unsigned)E[0]) continue;
-          int l1 = 0, l2 = 0;
-          if (D > 1) {
-            l1 = wrap_add(bb[1], j1, a.K[1]) - T0[1];
-            if ((unsigned)l1 >= (unsigned)E1) continue;
-          }
-          if (D > 2) {
-            l2 = wrap_add(bb[2], j2, a.K[2]) - T0[2];
-            if ((unsigned)l2 >= (unsigned)E2) continue;
-          }
-          C w;
-          if (CSRV) {
-            const int g = D == 1 ? 0 : (D == 2 ? j0 : j0 * J1 + j1);
-            const int jlast = D == 1 ? j0 : (D == 2 ? j1 : j2);
-            w = ((const C*)a.csr_val)[csr_index(r, g, jlast, G, JL, SPW)];
-          } else {
-            w = st[j0];
-            if (D > 1) w = cmul(w, st[J0 + j1]);
-            if (D > 2) w = cmul(w, st[J0 + J1 + j2]);
-          }
-          const int cell = l0 * S0 + l1 * S1 + l2 * S2;
-#pragma unroll
-          for (int c = 0; c < NBC; ++c) {
-            const C v = cmulc(w, yv[c]);
-            atomicAdd(&acc[(c * 2 + 0) * acells + cell], v.x);
-            atomicAdd(&acc[(c * 2 + 1) * acells + cell], v.y);
-          }
-        }
-      }
-    }
-  }
+// One work item (tile id, first row, rows, slot).  LDS: the accumulator
+// [pad_cells][NBC][re, im] as int64, then reduction scratch.
+template <typename T, int D, bool CSRV, int NBC, int JM>
+__device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
+                                         long long* __restrict__ acc,
+                                         const void* __restrict__ y_,
+                                         void* __restrict__ part_, int c0) {
+  typedef typename cplx<T>::t C;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int64_t pc = b.pad_cells;
+  double* red = reinterpret_cast<double*>(acc + pc * NBC * 2);
+  const C* __restrict__ y = (const C*)y_;
+  const int nb = a.nb;
+  const int r_begin = it.y, r_end = it.y + it.z;
 
-  // write the tile (direct) or the partial tile (split item)
-  const int ncell = E[0] * E1 * E2;
-  C* __restrict__ grid = (C*)grid_;
-  C* __restrict__ part = (C*)part_;
-  for (int cell = lane; cell < ncell; cell += 64) {
-    const int l2 = D > 2 ? cell % E2 : 0;
-    const int t = D > 2 ? cell / E2 : cell;
-    const int l1 = D > 1 ? t % E1 : 0;
-    const int l0 = D > 1 ? t / E1 : t;
-    int64_t g = (int64_t)(T0[0] + l0) * a.Kstride[0];
-    if (D > 1) g += (int64_t)(T0[1] + l1) * a.Kstride[1];
-    if (D > 2) g += (int64_t)(T0[2] + l2);
-    const int ac = l0 * S0 + l1 * S1 + l2 * S2;
+  for (int64_t i = tid; i < pc * NBC * 2; i += nthr) acc[i] = 0;
+  // scale: 2^e bounds (|Re y| + |Im y|) * |coef| for every row of the item
+  double ymax = 0.0;
+  int bad = 0;
+  for (int r = r_begin + tid; r < r_end; r += nthr) {
+    const int64_t m = a.perm[r];
 #pragma unroll
     for (int c = 0; c < NBC; ++c) {
-      const C v = mk<T>(acc[(c * 2 + 0) * acells + ac], acc[(c * 2 + 1) * acells + ac]);
-      if (it.w < 0) grid[g * nb + c0 + c] = v;
-      else part[((int64_t)it.w * cells + cell) * NBC + c] = v;
+      const C v = y[m * nb + c0 + c];
+      const double s = fabs((double)v.x) + fabs((double)v.y);
+      if (s <= 1.0e300) ymax = fmax(ymax, s);
+      else bad = 1;                       // NaN or infinity
     }
   }
-}
+  if (bad) ymax = __builtin_inf();
+  ymax = block_max(ymax, red);            // also orders the zeroing before the adds
+  // a non-finite sample poisons the item's whole slot
+  const bool poison = !(ymax <= 1.0e300);
+  const double B = poison ? 0.0 : ymax * b.u_bound;
+  int e = 0;
+  if (B > 0.0) (void)frexp(B, &e);       // B < 2^e
+  e = e < -960 ? -960 : e;
+  const double scale = ldexp(1.0, kFixBits - e);
+  const double unscale = ldexp(1.0, e - kFixBits);
 
-// One wavefront per work item (tile id, entry start, entry count, slot).
-template <typename T, int D, bool CSRV, int NBC>
-__global__ void __launch_bounds__(256)
-y2k_binned_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
-                  void* __restrict__ grid_, void* __restrict__ part_, int c0,
-                  const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  extern __shared__ unsigned char smem_raw[];
-  const int wave = threadIdx.x >> 6;
-  const int64_t item = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (item >= b.n_items) return;  // whole wave leaves together
-  const int4 it = reinterpret_cast<const int4*>(b.items)[item];
-  y2k_body<T, D, CSRV, NBC>(a, b, it, smem_raw + (size_t)wave * b.wave_lds_bytes,
-                            y_, grid_, part_, c0);
-}
+  const int JL = a.J[D - 1];
+  const int LPS = pnh_lanes_per_sample(JL);
+  const int SPW = 64 / LPS;
+  const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+  const int sl = lane / LPS;
+  const int jl = lane - sl * LPS;
+  const int P1 = D > 1 ? b.pad[1] : 1;
+  const int P2 = D > 2 ? b.pad[2] : 1;
+  const TileFrame f = tile_frame<D>(a, b, it.x);
+  const C* __restrict__ u = (const C*)a.u;
+  const int J0 = a.J[0];
+  const int J1 = D > 2 ? a.J[1] : 1;
 
-// Sum the partial tiles of every split tile in chunk order.
-template <typename T, int D, int NBC>
-__global__ void __launch_bounds__(256)
-y2k_merge_kernel(InterpArgs a, BinArgs b, const void* __restrict__ part_,
-                 void* __restrict__ grid_, int c0, const int* __restrict__ skip) {
-  typedef typename cplx<T>::t C;
-  if (skip && *skip) return;
-  const int cells = b.tile_cells;
-  const C* __restrict__ part = (const C*)part_;
-  C* __restrict__ grid = (C*)grid_;
-  for (int64_t mi = blockIdx.y; mi < b.n_merge; mi += gridDim.y) {
-    const int3 me = reinterpret_cast<const int3*>(b.merge)[mi];
-    int tid = me.x;
-    int T0[PNH_MAXD], E[PNH_MAXD];
+  for (int r0 = r_begin + wave * SPW; !poison && r0 < r_end; r0 += nw * SPW) {
+    const int r = r0 + sl;
+    if (sl >= SPW || jl >= JL || r >= r_end) continue;
+    const int64_t m = a.perm[r];
+    C yv[NBC];
 #pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      const int tg = b.tgrid[d];
-      const int td = tid % tg;
-      tid /= tg;
-      T0[d] = td * b.tile[d];
-      E[d] = min(b.tile[d], a.K[d] - T0[d]);
+    for (int c = 0; c < NBC; ++c) yv[c] = y[m * nb + c0 + c];
+    // this lane's last-axis accumulator index
+    const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
+                              f.wrap[D - 1], a.K[D - 1]);
+    if (CSRV) {
+      const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * a.J[1]);
+      const C* __restrict__ val = (const C*)a.csr_val;
+      const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
+      const int64_t gs = (int64_t)SPW * JL;
+      const int x0b = D > 1 ? a.base[r] - f.T0[0] : 0;
+      const int x1b = D > 2 ? a.base[a.M + r] - f.T0[1] : 0;
+      int j0 = 0, j1 = 0;
+      for (int g = 0; g < G; ++g) {
+        int idx = xl;
+        if (D == 2) idx += wrap_local(x0b + g, f.wrap[0], a.K[0]) * P1;
+        if (D == 3)
+          idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
+                  wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
+        const C w = cconj(val[i0 + g * gs]);
+        long long* cell = acc + (int64_t)idx * NBC * 2;
+#pragma unroll
+        for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+        if (D == 3 && ++j1 == J1) { j1 = 0; ++j0; }
+      }
+      continue;
     }
-    const int E1 = D > 1 ? E[1] : 1;
-    const int E2 = D > 2 ? E[2] : 1;
-    const int ncell = E[0] * E1 * E2;
-    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < ncell;
-         cell += gridDim.x * blockDim.x) {
-      const int l2 = D > 2 ? cell % E2 : 0;
-      const int t = D > 2 ? cell / E2 : cell;
-      const int l1 = D > 1 ? t % E1 : 0;
-      const int l0 = D > 1 ? t / E1 : t;
-      int64_t g = (int64_t)(T0[0] + l0) * a.Kstride[0];
-      if (D > 1) g += (int64_t)(T0[1] + l1) * a.Kstride[1];
-      if (D > 2) g += (int64_t)(T0[2] + l2);
+    const C ul = cconj(u[a.u_off[D - 1] + (int64_t)r * JL + jl]);
+    if (D == 1) {
+      long long* cell = acc + (int64_t)xl * NBC * 2;
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) {
-        C s = part[((int64_t)me.y * cells + cell) * NBC + c];
-        for (int k = 1; k < me.z; ++k)
-          s = cadd(s, part[((int64_t)(me.y + k) * cells + cell) * NBC + c]);
-        grid[g * a.nb + c0 + c] = s;
+      for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(ul, yv[c]), scale);
+      continue;
+    }
+    // NBC == 1: fold the sample value into the last-axis factor
+    const C wl = NBC == 1 ? cmulf(ul, yv[0]) : ul;
+    const int x0b = a.base[r] - f.T0[0];
+    const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
+    if (D == 2) {
+      for (int j0 = 0; j0 < J0; ++j0) {
+        const int idx = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 + xl;
+        const C w = cmulf(cconj(u0[j0]), wl);
+        long long* cell = acc + (int64_t)idx * NBC * 2;
+        if (NBC == 1) lds_add_c(cell, w, scale);
+        else {
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+        }
+      }
+      continue;
+    }
+    // D == 3: factors of axes 0 and 1 in registers, both loops unrolled
+    const int x1b = a.base[a.M + r] - f.T0[1];
+    const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
+    C u0c[JM], u1c[JM];
+    int off1[JM];
+#pragma unroll
+    for (int j = 0; j < JM; ++j) {
+      if (j < J0) u0c[j] = cconj(u0[j]);
+      if (j < J1) {
+        u1c[j] = cconj(u1[j]);
+        off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+      }
+    }
+#pragma unroll
+    for (int j0 = 0; j0 < JM; ++j0) {
+      if (j0 < J0) {
+        const int row = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 * P2;
+        const C a0 = cmulf(u0c[j0], wl);
+#pragma unroll
+        for (int j1 = 0; j1 < JM; ++j1) {
+          if (j1 < J1) {
+            const C w = cmulf(u1c[j1], a0);
+            long long* cell = acc + (int64_t)(row + off1[j1]) * NBC * 2;
+            if (NBC == 1) lds_add_c(cell, w, scale);
+            else {
+#pragma unroll
+              for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+            }
+          }
+        }
       }
     }
   }
+  __syncthreads();
+  // partial slot, scaled back
+  C* __restrict__ part = (C*)part_ + (int64_t)it.w * pc * NBC;
+  const T nan = (T)__builtin_nan("");
+  for (int64_t i = tid; i < pc * NBC; i += nthr) {
+    C v;
+    if (poison) v = mk<T>(nan, nan);
+    else v = mk<T>((T)((double)acc[2 * i] * unscale), (T)((double)acc[2 * i + 1] * unscale));
+    part[i] = v;
+  }
+}
+
+template <typename T, int D, bool CSRV, int NBC, int JM>
+__global__ void __launch_bounds__(1024)
+y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
+                void* __restrict__ part_, int c0, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ long long smem_acc[];
+  const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
+  y2k_item<T, D, CSRV, NBC, JM>(a, b, it, smem_acc, y_, part_, c0);
+}
+
+// grid[g] = sum over the partial slots covering cell g, in a fixed order:
+// neighbour masks 0 .. 2^D - 1 (bit d: the tile below along axis d), then
+// each tile's slots in chunk order.
+template <typename T, int D, int NBC>
+__device__ __forceinline__ void gather_cell(const InterpArgs& a, const BinArgs& b, int64_t g,
+                                            const typename cplx<T>::t* __restrict__ part,
+                                            typename cplx<T>::t* __restrict__ grid, int c0) {
+  typedef typename cplx<T>::t C;
+  int c[PNH_MAXD], td[PNH_MAXD], l[PNH_MAXD];
+  int64_t rest = g;
+#pragma unroll
+  for (int d = D - 1; d >= 0; --d) {
+    const int64_t q = rest / a.K[d];
+    c[d] = (int)(rest - q * a.K[d]);
+    rest = q;
+    td[d] = c[d] / b.tile[d];
+    l[d] = c[d] - td[d] * b.tile[d];
+  }
+  C s[NBC];
+#pragma unroll
+  for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
+  const int2* __restrict__ ts = reinterpret_cast<const int2*>(b.tile_slots);
+#pragma unroll
+  for (int mask = 0; mask < (1 << D); ++mask) {
+    bool ok = true;
+    int64_t tix = 0, pidx = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      int t = td[d], x = l[d];
+      if (b.tgrid[d] == 1) {
+        ok &= !((mask >> d) & 1);
+        x = c[d];
+      } else if ((mask >> d) & 1) {
+        ok &= x < a.J[d] - 1;
+        t = t == 0 ? b.tgrid[d] - 1 : t - 1;
+        x += min(b.tile[d], a.K[d] - t * b.tile[d]);
+      }
+      tix = tix * b.tgrid[d] + t;
+      pidx = pidx * b.pad[d] + x;
+    }
+    if (!ok) continue;
+    const int2 sl = ts[tix];
+    for (int k = 0; k < sl.y; ++k) {
+      const C* p = part + ((int64_t)(sl.x + k) * b.pad_cells + pidx) * NBC;
+#pragma unroll
+      for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
+    }
+  }
+#pragma unroll
+  for (int cc = 0; cc < NBC; ++cc) grid[g * a.nb + c0 + cc] = s[cc];
+}
+
+template <typename T, int D, int NBC>
+__global__ void __launch_bounds__(256)
+y2k_gather_kernel(InterpArgs a, BinArgs b, int64_t n_cells, const void* __restrict__ part_,
+                  void* __restrict__ grid_, int c0, const int* __restrict__ skip) {
+  typedef typename cplx<T>::t C;
+  if (skip && *skip) return;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_cells) return;
+  gather_cell<T, D, NBC>(a, b, g, (const C*)part_, (C*)grid_, c0);
 }
 
 template <typename T, int D, int LPS, bool CSRV, int NBC>
@@ -618,62 +630,74 @@ int launch_k2y_nb(const InterpArgs& a, const void* grid, void* y,
   return 0;
 }
 
+size_t y2k_lds_bytes(int64_t pad_cells, int nbc) {
+  return (size_t)pad_cells * nbc * 16 + 256;  // accumulator + reduction scratch
+}
+
+// Dynamic LDS above 64 KB must be allowed per kernel before the launch.
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes);
+}
+
+template <typename T, int D, bool CSRV, int NBC, int JM>
+int launch_tile(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
+                size_t lds, const int* skip, hipStream_t s) {
+  auto kern = y2k_tile_kernel<T, D, CSRV, NBC, JM>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y, part,
+                     c0, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, int D, bool CSRV, int NBC>
+int launch_tile_j(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
+                  size_t lds, const int* skip, hipStream_t s) {
+  if (D < 3) return launch_tile<T, D, CSRV, NBC, 1>(a, b, y, part, c0, lds, skip, s);
+  const int jm = a.J[0] > a.J[1] ? a.J[0] : a.J[1];
+  if (jm <= 8) return launch_tile<T, D, CSRV, NBC, 8>(a, b, y, part, c0, lds, skip, s);
+  if (jm <= 16) return launch_tile<T, D, CSRV, NBC, 16>(a, b, y, part, c0, lds, skip, s);
+  pnh_set_error("interpolator size %d above 16", jm);
+  return -2;
+}
+
 template <typename T, int D, bool CSRV>
-int launch_y2k_nb(const InterpArgs& a, const BinArgs& b0, const void* y,
+int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
                   void* grid, void* part, const int* skip, hipStream_t s) {
-  typedef typename cplx<T>::t C;
-  int SJ = 0;
-  for (int d = 0; d < D; ++d) SJ += a.J[d];
+  int64_t n_cells = 1;
+  for (int d = 0; d < D; ++d) n_cells *= a.K[d];
+  if (b.threads < 64 || b.threads > 1024 || b.threads % 64) {
+    pnh_set_error("gridding workgroup size %d is not a multiple of 64 in 64..1024", b.threads);
+    return -1;
+  }
   int c0 = 0;
   while (c0 < a.nb) {
-    const int left = a.nb - c0;
-    int nbc = left >= 4 ? 4 : (left >= 2 ? 2 : 1);
-    BinArgs b = b0;
-    // per-wave LDS: accumulators (2 * nbc * cells reals) + factor staging
-    // (batch * SJ complex) + batch row ids; shrink until one wave fits
-    int batch = 32;
-    size_t wl = 0;
-    for (;;) {
-      wl = (size_t)b.acc_cells * 2 * nbc * sizeof(T) + (size_t)batch * SJ * sizeof(C) +
-           (size_t)batch * sizeof(int);
-      wl = (wl + 15) & ~(size_t)15;
-      if (wl <= b.lds_budget) break;
-      if (nbc > 1) nbc >>= 1;
-      else if (batch > 4) batch >>= 1;
-      else {
-        pnh_set_error("y2k tile needs %zu bytes of LDS (budget %zu)", wl, b.lds_budget);
-        return -2;
-      }
+    const int nbc = pnh_y2k_coils_per_pass(b, a.nb - c0);
+    if (!nbc) {
+      pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)",
+                    y2k_lds_bytes(b.pad_cells, 1), pnh_lds_limit());
+      return -2;
     }
-    int waves = 4;
-    while (waves > 1 && (size_t)waves * wl > b.lds_budget) --waves;
-    b.batch = batch;
-    b.wave_lds_bytes = (int)wl;
-    b.waves_per_block = waves;
-    const size_t lds = (size_t)waves * wl;
-    const int64_t nblk = (b.n_items + waves - 1) / waves;
-    dim3 grid_dim((unsigned)nblk), blk(64 * waves);
-    switch (nbc) {
-      case 4: hipLaunchKernelGGL((y2k_binned_kernel<T, D, CSRV, 4>), grid_dim, blk, lds, s, a, b, y, grid, part, c0, skip); break;
-      case 2: hipLaunchKernelGGL((y2k_binned_kernel<T, D, CSRV, 2>), grid_dim, blk, lds, s, a, b, y, grid, part, c0, skip); break;
-      default: hipLaunchKernelGGL((y2k_binned_kernel<T, D, CSRV, 1>), grid_dim, blk, lds, s, a, b, y, grid, part, c0, skip); break;
+    const size_t lds = y2k_lds_bytes(b.pad_cells, nbc);
+    if (b.n_items > 0) {
+      int rc;
+      if (nbc == 4) rc = launch_tile_j<T, D, CSRV, 4>(a, b, y, part, c0, lds, skip, s);
+      else if (nbc == 2) rc = launch_tile_j<T, D, CSRV, 2>(a, b, y, part, c0, lds, skip, s);
+      else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, lds, skip, s);
+      if (rc) return rc;
     }
+    dim3 gg((unsigned)((n_cells + 255) / 256));
+    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
+    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
+    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
     PNH_LAUNCH_CHECK();
-    if (b.n_merge > 0) {
-      const unsigned gx = (unsigned)((b.tile_cells + 255) / 256);
-      dim3 mg(gx, (unsigned)(b.n_merge < 65535 ? b.n_merge : 65535));
-      switch (nbc) {
-        case 4: hipLaunchKernelGGL((y2k_merge_kernel<T, D, 4>), mg, dim3(256), 0, s, a, b, part, grid, c0, skip); break;
-        case 2: hipLaunchKernelGGL((y2k_merge_kernel<T, D, 2>), mg, dim3(256), 0, s, a, b, part, grid, c0, skip); break;
-        default: hipLaunchKernelGGL((y2k_merge_kernel<T, D, 1>), mg, dim3(256), 0, s, a, b, part, grid, c0, skip); break;
-      }
-      PNH_LAUNCH_CHECK();
-    }
     c0 += nbc;
   }
   return 0;
 }
-
 
 template <typename T, int D, int LPS>
 int launch_dom_t(const InterpArgs& a, const void* grid, const void* g, void* out,
@@ -754,6 +778,27 @@ int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
   return -2;
 }
 
+size_t pnh_lds_limit() {
+  static int cached[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess || v <= 0)
+      v = 65536;
+    cached[dev] = v;
+  }
+  return (size_t)cached[dev];
+}
+
+int pnh_y2k_coils_per_pass(const BinArgs& b, int nb) {
+  const size_t lim = pnh_lds_limit();
+  for (int nbc = nb >= 4 ? 4 : (nb >= 2 ? 2 : 1); nbc >= 1; nbc >>= 1)
+    if (y2k_lds_bytes(b.pad_cells, nbc) <= lim) return nbc;
+  return 0;
+}
+
 int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
                        const void* g, void* out, hipStream_t s) {
   if (!a.du) { pnh_set_error("plan has no factor derivatives"); return -1; }
@@ -799,75 +844,43 @@ k2y_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
                                 (C*)y_ + probs[p].y_off, 0);
 }
 
-template <typename T, int D>
-__global__ void __launch_bounds__(256)
-y2k_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
-                 int64_t n_items, int wave_lds, int batch_entries,
-                 const void* __restrict__ y_, void* __restrict__ k_, void* __restrict__ part_,
-                 const int* __restrict__ done, const int* __restrict__ all_done) {
+template <typename T, int D, int JM>
+__global__ void __launch_bounds__(1024)
+y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
+                      const void* __restrict__ y_, void* __restrict__ part_,
+                      const int* __restrict__ done, const int* __restrict__ all_done) {
   typedef typename cplx<T>::t C;
   if (all_done && *all_done) return;
-  extern __shared__ unsigned char smem_raw[];
-  const int wave = threadIdx.x >> 6;
-  const int64_t gi = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-  if (gi >= n_items) return;
-  const int2 pi = items[gi];
+  const int2 pi = items[blockIdx.x];
   const int p = pi.x;
-  if (done && done[p]) return;
+  if (done && done[p]) return;            // uniform over the workgroup
+  extern __shared__ long long smem_acc[];
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
-  BinArgs b = probs[p].bins;
-  b.batch = batch_entries;
-  b.wave_lds_bytes = wave_lds;
+  const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
-  y2k_body<T, D, false, 1>(a, b, it, smem_raw + (size_t)wave * wave_lds,
-                           (const C*)y_ + probs[p].y_off, (C*)k_ + probs[p].k_off,
-                           (C*)part_ + probs[p].part_off * b.tile_cells, 0);
+  y2k_item<T, D, false, 1, JM>(a, b, it, smem_acc, (const C*)y_ + probs[p].y_off,
+                               (C*)part_ + probs[p].part_off, 0);
 }
 
+// Gather of every problem's grid: chunk cells per block, blocks by table.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
-merge_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ merges,
-                   int64_t n_merge, const void* __restrict__ part_, void* __restrict__ k_,
-                   const int* __restrict__ done, const int* __restrict__ all_done) {
+gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t, int64_t chunk,
+                    const void* __restrict__ part_, void* __restrict__ k_,
+                    const int* __restrict__ done, const int* __restrict__ all_done) {
   typedef typename cplx<T>::t C;
   if (all_done && *all_done) return;
-  for (int64_t g = blockIdx.y; g < n_merge; g += gridDim.y) {
-    const int2 pm = merges[g];
-    const int p = pm.x;
-    if (done && done[p]) continue;
-    const InterpArgs& a = probs[p].ia;
-    const BinArgs& b = probs[p].bins;
-    const int3 me = reinterpret_cast<const int3*>(b.merge)[pm.y];
-    const int cells = b.tile_cells;
-    int tid = me.x;
-    int T0[PNH_MAXD], E[PNH_MAXD];
-#pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      const int tg = b.tgrid[d];
-      const int td = tid % tg;
-      tid /= tg;
-      T0[d] = td * b.tile[d];
-      E[d] = min(b.tile[d], a.K[d] - T0[d]);
-    }
-    const int E1 = D > 1 ? E[1] : 1;
-    const int E2 = D > 2 ? E[2] : 1;
-    const int ncell = E[0] * E1 * E2;
-    const C* part = (const C*)part_ + probs[p].part_off * cells;
-    C* grid = (C*)k_ + probs[p].k_off;
-    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < ncell;
-         cell += gridDim.x * blockDim.x) {
-      const int l2 = D > 2 ? cell % E2 : 0;
-      const int t = D > 2 ? cell / E2 : cell;
-      const int l1 = D > 1 ? t % E1 : 0;
-      const int l0 = D > 1 ? t / E1 : t;
-      int64_t gcell = (int64_t)(T0[0] + l0) * a.Kstride[0];
-      if (D > 1) gcell += (int64_t)(T0[1] + l1) * a.Kstride[1];
-      if (D > 2) gcell += (int64_t)(T0[2] + l2);
-      C s = part[(int64_t)me.y * cells + cell];
-      for (int k = 1; k < me.z; ++k) s = cadd(s, part[(int64_t)(me.y + k) * cells + cell]);
-      grid[gcell] = s;
-    }
-  }
+  const int p = t.tab[blockIdx.x];
+  if (done && done[p]) return;
+  const int64_t lb = (int64_t)blockIdx.x - t.first[p];
+  const InterpArgs a = probs[p].ia;
+  const BinArgs b = probs[p].bins;
+  const int64_t n = probs[p].n_grid;
+  const int64_t end = (lb + 1) * chunk < n ? (lb + 1) * chunk : n;
+  const C* part = (const C*)part_ + probs[p].part_off;
+  C* grid = (C*)k_ + probs[p].k_off;
+  for (int64_t g = lb * chunk + threadIdx.x; g < end; g += blockDim.x)
+    gather_cell<T, D, 1>(a, b, g, part, grid, 0);
 }
 
 template <typename T, int D>
@@ -912,29 +925,44 @@ int pnh_batch_k2y_launch(const BatchProblem* probs, const BatchTable& t, int D, 
 }
 
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const int2* merges, int64_t n_merge, int D, int fp64,
-                         int tile_cells, int wave_lds, int batch_entries, int waves,
-                         const void* y, void* k, void* part, const int* done,
-                         const int* all_done, hipStream_t s) {
-  const int64_t nblk = (n_items + waves - 1) / waves;
-  const size_t lds = (size_t)waves * wave_lds;
-  dim3 g((unsigned)nblk), b(64 * waves);
-#define YB(T, DD) hipLaunchKernelGGL((y2k_batch_kernel<T, DD>), g, b, lds, s, probs, items, \
-                                     n_items, wave_lds, batch_entries, y, k, part, done, all_done)
-#define MB(T, DD) hipLaunchKernelGGL((merge_batch_kernel<T, DD>), mg, dim3(256), 0, s, probs, \
-                                     merges, n_merge, part, k, done, all_done)
-  if (nblk > 0) {
-    if (fp64) { if (D == 1) YB(double, 1); else if (D == 2) YB(double, 2); else YB(double, 3); }
-    else { if (D == 1) YB(float, 1); else if (D == 2) YB(float, 2); else YB(float, 3); }
+                         const BatchTable& gather, int64_t chunk, int D, int jmax, int fp64,
+                         int64_t pad_cells, int threads, const void* y, void* k, void* part,
+                         const int* done, const int* all_done, hipStream_t s) {
+  const size_t lds = y2k_lds_bytes(pad_cells, 1);
+  if (lds > pnh_lds_limit()) {
+    pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
+                  pnh_lds_limit());
+    return -2;
+  }
+  if (jmax > 16) { pnh_set_error("interpolator size %d above 16", jmax); return -2; }
+#define YB(T, DD, JM)                                                                  \
+  do {                                                                                 \
+    auto kern = y2k_tile_batch_kernel<T, DD, JM>;                                      \
+    allow_lds(kern, lds);                                                              \
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs,    \
+                       items, y, part, done, all_done);                                \
+  } while (0)
+#define YBJ(T)                                                                         \
+  do {                                                                                 \
+    if (D == 1) YB(T, 1, 1);                                                           \
+    else if (D == 2) YB(T, 2, 1);                                                      \
+    else if (jmax <= 8) YB(T, 3, 8);                                                   \
+    else YB(T, 3, 16);                                                                 \
+  } while (0)
+#define GB(T, DD) hipLaunchKernelGGL((gather_batch_kernel<T, DD>), dim3((unsigned)gather.n_blocks), \
+                                     dim3(256), 0, s, probs, gather, chunk, part, k, done, all_done)
+  if (n_items > 0) {
+    if (fp64) YBJ(double);
+    else YBJ(float);
     PNH_LAUNCH_CHECK();
   }
-  if (n_merge > 0) {
-    dim3 mg((unsigned)((tile_cells + 255) / 256), (unsigned)(n_merge < 65535 ? n_merge : 65535));
-    if (fp64) { if (D == 1) MB(double, 1); else if (D == 2) MB(double, 2); else MB(double, 3); }
-    else { if (D == 1) MB(float, 1); else if (D == 2) MB(float, 2); else MB(float, 3); }
+  if (gather.n_blocks > 0) {
+    if (fp64) { if (D == 1) GB(double, 1); else if (D == 2) GB(double, 2); else GB(double, 3); }
+    else { if (D == 1) GB(float, 1); else if (D == 2) GB(float, 2); else GB(float, 3); }
     PNH_LAUNCH_CHECK();
   }
 #undef YB
-#undef MB
+#undef YBJ
+#undef GB
   return 0;
 }

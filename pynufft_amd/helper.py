@@ -209,93 +209,132 @@ def factor_gradients(st):
     return out
 
 
-def tile_shape(Kd):
-    """Gridding tile per axis: about 1024 cells, fastest axis longest."""
+GRID_LDS_MAX = 163840        # bytes: int64 (re, im) accumulator + 256 B scratch
+MAX_ITEM_ROWS = 4096         # rows per gridding work item (fixed-point headroom)
+
+
+def tile_valid(t, K, J):
+    """A tile of extent t on an axis of K cells is usable when it spans the
+    axis or when every tile, the last (partial) one included, is at least
+    J - 1 cells wide, so that a tile's padding reaches only the next tile."""
+    if t >= K:
+        return True
+    if t < max(1, J - 1):
+        return False
+    return K - (-(-K // t) - 1) * t >= J - 1
+
+
+def tile_pad(tile, Kd, Jd):
+    """Accumulator extent per axis: tile + J - 1, or K for a spanning tile."""
+    return tuple(K if t >= K else t + J - 1 for t, K, J in zip(tile, Kd, Jd))
+
+
+def _valid_near(pref, K, J):
+    """Largest valid tile <= pref, else the smallest valid one above it."""
+    for t in range(min(pref, K), 0, -1):
+        if tile_valid(t, K, J):
+            return t
+    for t in range(pref + 1, K + 1):
+        if tile_valid(t, K, J):
+            return t
+    return K
+
+
+def tile_shape(Kd, Jd=None):
+    """Gridding tile per axis: 1024 cells (1-D), 32 x 32 (2-D) or
+    16 x 16 x 16 (3-D) when valid for (Kd, Jd); then, while the padded
+    accumulator does not fit in LDS, the axis with the widest accumulator
+    steps down to its next valid tile."""
     d = len(Kd)
-    pref = {1: (1024,), 2: (32, 32), 3: (8, 8, 16)}[d]
-    return tuple(int(min(p, k)) for p, k in zip(pref, Kd))
+    Jd = tuple(Jd) if Jd is not None else (6,) * d
+    pref = {1: (1024,), 2: (32, 32), 3: (16, 16, 16)}[d]
+    tile = [_valid_near(p, K, J) for p, K, J in zip(pref, Kd, Jd)]
+
+    def fits(t):
+        return int(np.prod(tile_pad(t, Kd, Jd))) * 16 + 256 <= GRID_LDS_MAX
+
+    while not fits(tile):
+        pads = tile_pad(tile, Kd, Jd)
+        for a in np.argsort(pads, kind='stable')[::-1]:
+            t = tile[a] - 1
+            while t >= 1 and not tile_valid(t, Kd[a], Jd[a]):
+                t -= 1
+            if t >= 1:
+                tile[a] = t
+                break
+        else:
+            raise ValueError('Jd %s is too large for the LDS gridding accumulator' % (Jd,))
+    return tuple(int(t) for t in tile)
 
 
-def sort_and_bin(base, Kd, Jd, tile, chunk_entries=None):
-    """Trajectory sort and the deterministic gridding work list.
+def check_tile(tile, Kd, Jd):
+    """ValueError unless ``tile`` is valid on every axis and fits in LDS."""
+    tile = tuple(int(t) for t in tile)
+    if len(tile) != len(Kd):
+        raise ValueError('tile must have one extent per axis')
+    for t, K, J in zip(tile, Kd, Jd):
+        if t < 1 or not tile_valid(t, K, J):
+            raise ValueError('gridding tile %d on an axis of %d cells leaves a tile '
+                             'narrower than Jd - 1 = %d' % (t, K, J - 1))
+    if int(np.prod(tile_pad(tile, Kd, Jd))) * 16 + 256 > GRID_LDS_MAX:
+        raise ValueError('gridding tile %s needs more LDS than a workgroup has' % (tile,))
+    return tuple(min(t, K) for t, K in zip(tile, Kd))
 
-    Rows are samples sorted by (tile of first support cell, sample index).
-    Every tile met by a row's support gets that row in its bin; bins are
-    ordered by original sample index, so each grid cell sums its samples in
-    ascending sample order (the order of the reference's transposed-CSR
-    matvec, helper.py:285-311 + _nufft_class_methods_cpu.py:110-112).
-    Bins longer than ``chunk_entries`` are split into chunks that write
-    partial tiles, summed in chunk order afterwards.
 
-    Returns perm, items (n,4) int32, rows int32, merge (n,3) int32, n_slots.
+def sort_and_bin(base, Kd, Jd, tile, chunk_rows=None):
+    """Row order and the gridding work list (tile ownership).
+
+    A row belongs to the tile that holds its first support cell.  Rows are
+    the samples sorted by that tile (C order over the tile grid), ties in
+    sample order, so each tile owns one contiguous run of rows; runs longer
+    than ``chunk_rows`` (at most 4096) are cut into chunks.  Every chunk is
+    one work item with its own partial slot; a tile's slots are consecutive
+    and in row order, which fixes the order in which the gather pass adds
+    them.
+
+    Returns perm (M,) int64; items (n, 4) int32 rows [tile id, first row,
+    rows, slot], heaviest first; tile_slots (n_tiles, 2) int32 rows
+    [first slot, slot count].
     """
     d = len(Kd)
     M = base.shape[1]
     tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(d)]
     n_tiles = int(np.prod(tgrid))
-    # sort key: tile of the first support cell (C order over the tile grid)
     tkey = np.zeros(M, dtype=np.int64)
     for a in range(d):
         tkey = tkey * tgrid[a] + base[a] // tile[a]
     perm = np.argsort(tkey, kind='stable').astype(np.int64)
-    inv = np.empty(M, dtype=np.int64)
-    inv[perm] = np.arange(M)
-    if M == 0:
-        items = np.stack([np.arange(n_tiles), np.zeros(n_tiles), np.zeros(n_tiles),
-                          -np.ones(n_tiles)], axis=1).astype(np.int32)
-        return (perm, items, np.zeros(0, np.int32), np.zeros((0, 3), np.int32), 0)
-    # distinct tiles touched per axis, per sample: cells (b + j) mod K
-    per_axis = []
-    for a in range(d):
-        cells = np.mod(base[a][:, None] + np.arange(Jd[a])[None, :], Kd[a])
-        t = np.sort(cells // tile[a], axis=1)              # (M, J)
-        new = np.ones_like(t, dtype=bool)
-        new[:, 1:] = t[:, 1:] != t[:, :-1]
-        width = int(new.sum(axis=1).max()) if M else 1
-        out = np.full((M, width), -1, dtype=np.int64)
-        pos = np.cumsum(new, axis=1) - 1
-        rr, cc = np.nonzero(new)
-        out[rr, pos[rr, cc]] = t[rr, cc]
-        per_axis.append(out)
-    # Cartesian product of per-axis tile sets -> (sample, tile) pairs
-    tid = np.zeros((M, 1), dtype=np.int64)
-    valid = np.ones((M, 1), dtype=bool)
-    for a in range(d):
-        ta = per_axis[a]
-        tid = (tid[:, :, None] * tgrid[a] + ta[:, None, :]).reshape(M, -1)
-        valid = (valid[:, :, None] & (ta[:, None, :] >= 0)).reshape(M, -1)
-    samp = np.broadcast_to(np.arange(M, dtype=np.int64)[:, None], tid.shape)[valid]
-    key = tid[valid] * max(M, 1) + samp          # (tile, sample) lexicographic
-    key.sort()
-    ptile = key // max(M, 1)
-    rows = inv[key - ptile * max(M, 1)].astype(np.int32)
-    counts = np.bincount(ptile, minlength=n_tiles).astype(np.int64)
-    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
-    total = int(counts.sum())
-    if chunk_entries is None:
-        chunk_entries = int(max(64, min(1024, math.ceil(total / 8192))))
-    n_chunks = np.maximum(1, (counts + chunk_entries - 1) // chunk_entries)
-    split = n_chunks > 1
-    slot0 = np.zeros(n_tiles, dtype=np.int64)
-    slot0[split] = np.concatenate([[0], np.cumsum(n_chunks[split])[:-1]]) \
-        if split.any() else []
-    n_slots = int(n_chunks[split].sum())
+    counts = np.bincount(tkey, minlength=n_tiles).astype(np.int64)
+    starts = np.cumsum(counts) - counts
+    if chunk_rows is None:
+        chunk_rows = int(min(MAX_ITEM_ROWS, max(256, math.ceil(M / 2048))))
+    chunk_rows = int(chunk_rows)
+    if not 1 <= chunk_rows <= MAX_ITEM_ROWS:
+        raise ValueError('chunk rows must be in 1..%d' % MAX_ITEM_ROWS)
+    n_chunks = (counts + chunk_rows - 1) // chunk_rows
+    slot0 = np.cumsum(n_chunks) - n_chunks
     tiles_rep = np.repeat(np.arange(n_tiles), n_chunks)
-    k = np.arange(tiles_rep.size) - np.repeat(np.cumsum(n_chunks) - n_chunks, n_chunks)
-    st = starts[tiles_rep] + k * chunk_entries
-    cnt = np.minimum(chunk_entries, starts[tiles_rep] + counts[tiles_rep] - st)
-    cnt = np.maximum(cnt, 0)
-    slot = np.where(split[tiles_rep], slot0[tiles_rep] + k, -1)
-    items = np.stack([tiles_rep, st, cnt, slot], axis=1)
-    # heaviest items first (longest-processing-time order)
-    items = items[np.argsort(-items[:, 2], kind='stable')].astype(np.int32)
-    mt = np.nonzero(split)[0]
-    merge = np.stack([mt, slot0[mt], n_chunks[mt]], axis=1).astype(np.int32) \
-        if mt.size else np.zeros((0, 3), dtype=np.int32)
-    return perm, items, rows, merge, n_slots
+    k = np.arange(tiles_rep.size) - slot0[tiles_rep]
+    first = starts[tiles_rep] + k * chunk_rows
+    cnt = np.minimum(chunk_rows, starts[tiles_rep] + counts[tiles_rep] - first)
+    items = np.stack([tiles_rep, first, cnt, np.arange(tiles_rep.size)], axis=1)
+    items = items[np.argsort(-cnt, kind='stable')].astype(np.int32).reshape(-1, 4)
+    tile_slots = np.stack([slot0, n_chunks], axis=1).astype(np.int32)
+    return perm, items, tile_slots
 
 
-def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_entries=None):
+def coefficient_bound(ud):
+    """Upper bound of |coefficient| = prod_d |u_d[j_d]| over all rows and
+    offsets, with a margin for rounding the factors to complex64."""
+    if not len(ud) or ud[0].shape[0] == 0:
+        return 1.0
+    prod = np.ones(ud[0].shape[0])
+    for u in ud:
+        prod = prod * np.abs(u).max(axis=1)
+    return float(prod.max()) * (1.0 + 2.0 ** -20)
+
+
+def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None):
     """FP64 host plan: per-axis design, per-sample factors, sort and bins.
 
     Mirrors ``helper.plan(om, Nd, Kd, Jd, ft_axes)`` (reference
@@ -350,11 +389,10 @@ def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_entries=None):
     st['sn'] = sn.real
     st['snd'] = snd
     base = np.stack(based, axis=0) if M else np.zeros((dd, 0), np.int64)
-    tile = tuple(tile) if tile is not None else tile_shape(Kd)
-    perm, items, rows, merge, n_slots = sort_and_bin(base, Kd, Jd, tile,
-                                                     chunk_entries)
+    tile = check_tile(tile, Kd, Jd) if tile is not None else tile_shape(Kd, Jd)
+    perm, items, tile_slots = sort_and_bin(base, Kd, Jd, tile, chunk_rows)
     st.update(ud=ud, base=base, perm=perm, tile=tile, items=items,
-              bin_rows=rows, merge=merge, n_slots=n_slots)
+              tile_slots=tile_slots, u_bound=coefficient_bound(ud))
     return st
 
 

@@ -80,7 +80,7 @@ class NUFFT:
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
-             tile=None, chunk=None, host_plan=None):
+             tile=None, chunk=None, host_plan=None, threads=0):
         """Design the interpolator on the host in FP64 and upload it.
 
         Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
@@ -88,15 +88,16 @@ class NUFFT:
         follows the deprecated ``NUFFT_hsa.plan`` (``linalg/nufft_hsa.py:
         152-242``): coils share the trajectory, coil index fastest.
         ``radix == len(Nd)`` stores the full coefficient table like
-        ``legacy=True``.  ``tile`` (gridding tile per axis) and ``chunk``
-        (bin entries per gridding work item) are tuning knobs; the defaults
-        come from ``helper.tile_shape`` and ``helper.sort_and_bin``.
+        ``legacy=True``.  ``tile`` (gridding tile per axis), ``chunk`` (rows
+        per gridding work item, at most 4096) and ``threads`` (gridding
+        workgroup size) are tuning knobs; the defaults come from
+        ``helper.tile_shape``, ``helper.sort_and_bin`` and the library.
         ``host_plan`` may pass a ``helper.plan`` result computed elsewhere
         (e.g. in a thread pool).  Returns 0.
         """
         self.release()
         st = host_plan if host_plan is not None else \
-            helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_entries=chunk)
+            helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_rows=chunk)
         dd = len(Nd)
         self.ndims = dd
         self.ft_axes = tuple(range(dd)) if ft_axes is None else tuple(ft_axes)
@@ -115,6 +116,7 @@ class NUFFT:
         self.st = {k: st[k] for k in ('Nd', 'Kd', 'Jd', 'M', 'om', 'tol',
                                        'alpha', 'beta')}
         self._host = st
+        self._threads = int(threads)
         self._format = 'CSR' if (self.legacy or (radix is not None and radix >= dd)) \
             else 'factor'
         dev = self.device
@@ -143,11 +145,9 @@ class NUFFT:
             tw = helper.twiddles(Kd[a])
             t['tw64_%d' % a] = torch.from_numpy(tw.astype(np.complex64)).to(dev)
             t['tw128_%d' % a] = torch.from_numpy(tw).to(dev)
-        t['items'] = torch.from_numpy(np.ascontiguousarray(st['items'])).to(dev)
-        t['bin_rows'] = torch.from_numpy(
-            st['bin_rows'] if st['bin_rows'].size else np.zeros(1, np.int32)).to(dev)
-        t['merge'] = torch.from_numpy(np.ascontiguousarray(
-            st['merge'] if st['merge'].size else np.zeros((1, 3), np.int32))).to(dev)
+        t['items'] = torch.from_numpy(np.ascontiguousarray(
+            st['items'] if st['items'].size else np.zeros((1, 4), np.int32))).to(dev)
+        t['tile_slots'] = torch.from_numpy(np.ascontiguousarray(st['tile_slots'])).to(dev)
         if self._format == 'CSR':
             self._build_csr()
         self._create()
@@ -233,12 +233,13 @@ class NUFFT:
         d.csr_col = t['csr_col'].data_ptr() if 'csr_col' in t else None
         d.sn_f32 = t['sn_f32'].data_ptr()
         d.sn_f64 = t['sn_f64'].data_ptr()
+        d.u_bound = float(st['u_bound'])
         d.n_items = int(st['items'].shape[0])
         d.items = t['items'].data_ptr()
-        d.bin_rows = t['bin_rows'].data_ptr()
-        d.n_merge = int(st['merge'].shape[0])
-        d.merge = t['merge'].data_ptr()
-        d.n_slots = int(st['n_slots'])
+        d.n_tiles = int(st['tile_slots'].shape[0])
+        d.tile_slots = t['tile_slots'].data_ptr()
+        d.n_slots = int(st['items'].shape[0])
+        d.threads = self._threads
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self._libh.pnh_plan_create(ctypes.byref(d), ctypes.byref(h)))

@@ -99,9 +99,12 @@ def _fake_desc(ndim, Jd, tile, K=64, N=32, M=100):
         d.tile[a] = tile
         d.tw_c64[a] = 0x1000
         d.tw_c128[a] = 0x1000
-    for f in ('perm', 'base', 'u_c64', 'u_c128', 'sn_f32', 'items', 'bin_rows'):
+    for f in ('perm', 'base', 'u_c64', 'u_c128', 'sn_f32', 'items', 'tile_slots'):
         setattr(d, f, 0x1000)
     d.n_items = 4
+    d.n_slots = 4
+    d.n_tiles = (-(-K // tile)) ** ndim if tile < K else 1
+    d.u_bound = 1.0
     return d
 
 
@@ -116,3 +119,22 @@ def test_batch_validation_on_host(lib):
     jd = (_lib.PlanDesc * 2)(_fake_desc(2, 6, 32), _fake_desc(2, 4, 32))
     assert lib.pnh_batch_table_bytes(jd, 2, ctypes.byref(n)) == -1
     assert lib.pnh_batch_table_bytes(ok, 0, ctypes.byref(n)) == -1
+
+
+def test_gridding_tile_validation_on_host(lib):
+    """The library refuses tiles whose padding would skip a neighbour, tiles
+    whose accumulator exceeds LDS, and inconsistent work-list sizes."""
+    from pynufft_amd import _lib
+    n = ctypes.c_size_t()
+    bad = (_lib.PlanDesc * 1)(_fake_desc(2, 6, 12))      # 64 = 5 * 12 + 4, 4 < 5
+    assert lib.pnh_batch_table_bytes(bad, 1, ctypes.byref(n)) == -1
+    assert b'narrower than Jd - 1' in lib.pnh_last_error()
+    big = (_lib.PlanDesc * 1)(_fake_desc(3, 6, 32))       # 37^3 cells
+    assert lib.pnh_batch_table_bytes(big, 1, ctypes.byref(n)) == -2
+    assert b'LDS' in lib.pnh_last_error()
+    d = _fake_desc(2, 6, 32)
+    d.n_tiles = 3
+    assert lib.pnh_batch_table_bytes((_lib.PlanDesc * 1)(d), 1, ctypes.byref(n)) == -1
+    d = _fake_desc(2, 6, 32)
+    d.threads = 100
+    assert lib.pnh_batch_table_bytes((_lib.PlanDesc * 1)(d), 1, ctypes.byref(n)) == -1

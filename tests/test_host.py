@@ -1,6 +1,7 @@
-"""Host logic of the product (no GPU): the FP64 plan builder, the trajectory
-sort and the gridding work list (checked by a numpy emulation of the y2k
-kernel's algorithm against the oracle's transposed-CSR matvec)."""
+"""Host logic of the product (no GPU): the FP64 plan builder, the row sort
+and the gridding work list, checked by a numpy emulation of the y2k
+kernels' algorithm (tile ownership, padded accumulators, fixed point,
+gather order) against the oracle's transposed-CSR matvec."""
 import numpy as np
 import pytest
 
@@ -10,120 +11,117 @@ from pynufft_amd import workloads as W
 from conftest import TINY_CASES, golden, geometry, rel
 
 
-def _wrap(b, j, K):
-    c = b + j
-    return c - K if c >= K else c
-
-
-def clip_axis(b, J, K, T0, E):
-    """Mirror of the kernel's clip_axis (csrc/interp.hip)."""
-    if E == K:
-        return (1 if b + J <= K else 0), 0, J
-    T1 = T0 + E
-    l1, h1 = max(0, T0 - b), min(min(J, K - b), T1 - b)
-    l2, h2 = max(K - b, T0 - b + K), min(J, T1 - b + K)
-    a1, a2 = h1 > l1, h2 > l2
-    if a1 and a2:
-        return 0, 0, 0
-    if a1:
-        return 1, l1, h1 - l1
-    if a2:
-        return 1, l2, h2 - l2
-    return 1, 0, 0
-
-
-def emulate_y2k(st, Kd, Jd, y):
-    """The y2k kernel's algorithm in numpy, step for step: one accumulator per
-    work item, entries in bin order, support boxes clipped to the tile
-    (packed start cell, 16-bit reciprocal division) or, for a wrapped support
-    in a tile that (nearly) spans an axis, the masked walk; split tiles summed
-    in chunk order."""
+def tile_frame(tid, tile, tgrid, Kd):
+    """Tile origin per axis and the wrap index (K for a spanning tile)."""
     D = len(Kd)
-    tile = st['tile']
-    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
-    perm = st['perm']
-    base = st['base'][:, perm]
-    u = [st['ud'][a][perm] for a in range(D)]
-    cells = int(np.prod(tile))
-    J = list(Jd) + [1] * (3 - D)
-    K = list(Kd) + [1] * (3 - D)
-    out = np.full(Kd, np.nan, dtype=complex)
-    parts = {}
-    for tid, s, cnt, slot in st['items']:
-        t, T0, E = tid, [0, 0, 0], [1, 1, 1]
-        for d in range(D - 1, -1, -1):
-            T0[d] = (t % tgrid[d]) * tile[d]
-            t //= tgrid[d]
-            E[d] = min(tile[d], Kd[d] - T0[d])
-        E1, E2 = E[1], E[2]
-        acc = np.zeros(cells, complex)
-        for r in st['bin_rows'][s:s + cnt]:
-            bb = [int(base[d][r]) for d in range(D)] + [0] * (3 - D)
-            lo, n, st0, single = [0, 0, 0], [1, 1, 1], [0, 0, 0], 1
-            for d in range(D):
-                ok, lo[d], n[d] = clip_axis(bb[d], J[d], K[d], T0[d], E[d])
-                single &= ok
-                st0[d] = _wrap(bb[d], lo[d], K[d]) - T0[d]
-            yv = y[perm[r]]
-            if single:
-                tot = n[0] * n[1] * n[2]
-                start = (st0[0] * E1 + st0[1]) * E2 + st0[2]
-                m2 = (65536 + n[2] - 1) // max(n[2], 1)
-                m1 = (65536 + n[1] - 1) // max(n[1], 1)
-                for q in range(tot):
-                    if tot <= 256:
-                        tt = (q * m2) >> 16
-                        t2 = q - tt * n[2]
-                        t0 = (tt * m1) >> 16
-                        t1 = tt - t0 * n[1]
-                    else:
-                        t2, tt = q % n[2], q // n[2]
-                        t1, t0 = tt % n[1], tt // n[1]
-                    w = u[0][r][lo[0] + t0]
-                    if D > 1:
-                        w = w * u[1][r][lo[1] + t1]
-                    if D > 2:
-                        w = w * u[2][r][lo[2] + t2]
-                    acc[start + (t0 * E1 + t1) * E2 + t2] += np.conj(w) * yv
-            else:
-                for j in np.ndindex(*Jd):
-                    l = [_wrap(bb[d], j[d], K[d]) - T0[d] for d in range(D)]
-                    if all(0 <= l[d] < E[d] for d in range(D)):
-                        l = l + [0] * (3 - D)
-                        w = 1
-                        for d in range(D):
-                            w = w * u[d][r][j[d]]
-                        acc[(l[0] * E1 + l[1]) * E2 + l[2]] += np.conj(w) * yv
-        blk = acc[:E[0] * E1 * E2].reshape([E[d] for d in range(D)])
-        sl = tuple(slice(T0[d], T0[d] + E[d]) for d in range(D))
-        if slot < 0:
-            out[sl] = blk
-        else:
-            parts[slot] = (sl, blk)
-    for tid, s0, nn in st['merge']:
-        sl = parts[s0][0]
-        out[sl] = sum(parts[s0 + k][1] for k in range(nn))
+    T0, wrap = [0] * D, [0] * D
+    for d in range(D - 1, -1, -1):
+        T0[d] = (tid % tgrid[d]) * tile[d]
+        tid //= tgrid[d]
+        wrap[d] = Kd[d] if tgrid[d] == 1 else 1 << 30
+    return T0, wrap
+
+
+def gather_sources(c, tile, tgrid, Kd, Jd):
+    """(tile id, accumulator index) pairs the gather pass adds for grid cell
+    c, in its order: neighbour masks 0 .. 2^D - 1 (bit d = the tile below
+    along axis d)."""
+    D = len(Kd)
+    pad = helper.tile_pad(tile, Kd, Jd)
+    out = []
+    for mask in range(1 << D):
+        ok, tix, pidx = True, 0, 0
+        for d in range(D):
+            t, x = c[d] // tile[d], c[d] % tile[d]
+            if tgrid[d] == 1:
+                ok &= not (mask >> d) & 1
+                x = c[d]
+            elif (mask >> d) & 1:
+                ok &= x < Jd[d] - 1
+                t = tgrid[d] - 1 if t == 0 else t - 1
+                x += min(tile[d], Kd[d] - t * tile[d])
+            tix = tix * tgrid[d] + t
+            pidx = pidx * pad[d] + x
+        if ok:
+            out.append((tix, pidx))
     return out
 
 
-def test_clip_axis_exhaustive():
-    """clip_axis returns exactly the hit offsets, as one contiguous run of
-    cells, or flags the case (two runs) for the masked walk."""
-    for K in range(1, 26):
+def emulate_y2k(st, Kd, Jd, y):
+    """The y2k kernels' algorithm in numpy: per work item, every support box
+    of the tile's rows into the padded accumulator (asserting it lies
+    inside), 64-bit fixed point with the item's power-of-two scale, the
+    partial slot scaled back; then the gather pass in its fixed order."""
+    D = len(Kd)
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
+    pad = helper.tile_pad(tile, Kd, Jd)
+    perm = st['perm']
+    base = st['base'][:, perm]
+    u = [st['ud'][a][perm] for a in range(D)]
+    yrow = np.asarray(y)[perm]
+    slots = {}
+    for tid, first, cnt, slot in st['items']:
+        assert 1 <= cnt <= helper.MAX_ITEM_ROWS
+        T0, wrap = tile_frame(int(tid), tile, tgrid, Kd)
+        rows = range(first, first + cnt)
+        ymax = max(abs(yrow[r].real) + abs(yrow[r].imag) for r in rows)
+        B = ymax * st['u_bound']
+        e = max(np.frexp(B)[1], -960) if B > 0 else 0
+        scale = 2.0 ** (50 - e)
+        acc_re = np.zeros(int(np.prod(pad)), dtype=np.int64)
+        acc_im = np.zeros_like(acc_re)
+        for r in rows:
+            for d in range(D):           # the row belongs to this tile
+                assert T0[d] <= base[d][r] < T0[d] + tile[d]
+            for j in np.ndindex(*Jd):
+                x = [base[d][r] - T0[d] + j[d] for d in range(D)]
+                x = [x[d] - Kd[d] if x[d] >= wrap[d] else x[d] for d in range(D)]
+                assert all(0 <= x[d] < pad[d] for d in range(D))
+                w = 1.0
+                for d in range(D):
+                    w = w * u[d][r][j[d]]
+                v = np.conj(w) * yrow[r]
+                assert abs(v.real) * scale < 2 ** 50 and abs(v.imag) * scale < 2 ** 50
+                i = int(np.ravel_multi_index(x, pad))
+                acc_re[i] += int(np.rint(v.real * scale))
+                acc_im[i] += int(np.rint(v.imag * scale))
+        slots[int(slot)] = (acc_re + 1j * acc_im) / scale
+    out = np.zeros(Kd, dtype=complex)
+    for c in np.ndindex(*Kd):
+        for tix, pidx in gather_sources(c, tile, tgrid, Kd, Jd):
+            s0, ns = st['tile_slots'][tix]
+            for k in range(ns):
+                out[c] += slots[int(s0 + k)][pidx]
+    return out
+
+
+def test_tile_validity_and_gather_sources():
+    """For every valid tile on small axes, the gather's neighbour rule finds
+    exactly the (tile, accumulator index) pairs whose padded box holds the
+    cell, and each padded box reaches only its own and the next tile."""
+    for K in range(1, 30):
         for J in range(1, min(K, 8) + 1):
-            for tile in range(1, K + 1):
-                for T0 in range(0, K, tile):
-                    E = min(tile, K - T0)
-                    for b in range(K):
-                        hits = [j for j in range(J) if T0 <= (b + j) % K < T0 + E]
-                        single, lo, n = clip_axis(b, J, K, T0, E)
-                        if single:
-                            assert hits == list(range(lo, lo + n))
-                            cells = [(b + j) % K for j in hits]
-                            assert cells == list(range(cells[0], cells[0] + n)) if n else True
-                        else:
-                            runs = [(b + j) % K for j in hits]
-                            assert any(runs[i + 1] != runs[i] + 1 for i in range(len(runs) - 1))
+            for t in range(1, K + 2):
+                if not helper.tile_valid(t, K, J):
+                    continue
+                tt = min(t, K)
+                tg = -(-K // tt)
+                pad = helper.tile_pad((tt,), (K,), (J,))[0]
+                cover = {c: [] for c in range(K)}
+                for td in range(tg):
+                    T0 = td * tt
+                    E = min(tt, K - T0)
+                    span = E + J - 1 if tg > 1 else K
+                    for x in range(span):
+                        c = (T0 + x) % K
+                        assert x < pad
+                        if tg > 1:
+                            assert c // tt in (td, (td + 1) % tg)
+                        cover[c].append((td, x))
+                for c in range(K):
+                    got = sorted(gather_sources((c,), (tt,), (tg,), (K,), (J,)))
+                    assert got == sorted(cover[c]), (K, J, t, c)
 
 
 @pytest.mark.parametrize('name', TINY_CASES + ['plain2d'])
@@ -147,45 +145,82 @@ def test_plan_factors_bit_exact(name):
     assert np.array_equal(uu.ravel(), ref['p'].data)
 
 
+def _small_tiles(Kd, Jd):
+    """Valid tiles that split every axis that can be split (wrap paths)."""
+    out = []
+    for K, J in zip(Kd, Jd):
+        ts = [t for t in range(max(1, J - 1), K) if helper.tile_valid(t, K, J)]
+        out.append(ts[0] if ts else K)
+    return tuple(out)
+
+
 @pytest.mark.parametrize('name', TINY_CASES)
-@pytest.mark.parametrize('chunk', [None, 3])
-def test_binned_gridding_emulation(name, chunk):
+@pytest.mark.parametrize('mode', ['default', 'small_tiles', 'chunk3'])
+def test_gridding_emulation(name, mode):
     g = golden(name)
     Nd, Kd, Jd = geometry(g)
-    st = helper.plan(g['om'], Nd, Kd, Jd, chunk_entries=chunk)
+    tile = _small_tiles(Kd, Jd) if mode == 'small_tiles' else None
+    chunk = 3 if mode == 'chunk3' else None
+    st = helper.plan(g['om'], Nd, Kd, Jd, tile=tile, chunk_rows=chunk)
     k = emulate_y2k(st, Kd, Jd, g['adjoint_in'])
-    assert not np.isnan(k).any(), 'some grid cells were never written'
-    assert rel(k, g['y2k']) < 1e-14
+    assert rel(k, g['y2k']) < 1e-13
 
 
 def test_work_list_invariants():
     w = W.radial2d()
     sel = np.arange(0, w['om'].shape[0], 7)
     Nd, Kd, Jd = w['Nd'], w['Kd'], w['Jd']
-    st = helper.plan(w['om'][sel], Nd, Kd, Jd, chunk_entries=64)
+    st = helper.plan(w['om'][sel], Nd, Kd, Jd, chunk_rows=64)
     M = sel.size
     perm = st['perm']
     assert np.array_equal(np.sort(perm), np.arange(M))
     items = st['items'].astype(np.int64)
-    tgrid = [(Kd[a] + st['tile'][a] - 1) // st['tile'][a] for a in range(2)]
-    # every tile is written by at least one item; chunks tile each bin exactly
-    assert set(items[:, 0]) == set(range(int(np.prod(tgrid))))
-    total = 0
-    for t in np.unique(items[:, 0]):
-        it = items[items[:, 0] == t]
-        it = it[np.argsort(it[:, 1])]
-        assert np.all(it[1:, 1] == it[:-1, 1] + it[:-1, 2])
-        assert (len(it) > 1) == (it[0, 3] >= 0)
-        total += it[:, 2].sum()
-    assert total == st['bin_rows'].size
-    # (row, tile) pairs are unique and each bin is in ascending sample order
-    for t0 in range(0, items.shape[0], 97):
-        tid, s, c, _ = items[t0]
-        rows = st['bin_rows'][s:s + c]
-        assert len(np.unique(rows)) == len(rows)
-        assert np.all(np.diff(perm[rows]) > 0)
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(2)]
+    base = st['base'][:, perm]
+    owner = (base[0] // tile[0]) * tgrid[1] + base[1] // tile[1]
+    # rows are sorted by owning tile, ties in sample order
+    assert np.all(np.diff(owner) >= 0)
+    same = np.diff(owner) == 0
+    assert np.all(np.diff(perm)[same] > 0)
+    # items cover every row once, each within its tile, at most chunk rows
+    covered = np.zeros(M, dtype=int)
+    for tid, first, cnt, slot in items:
+        assert 1 <= cnt <= 64
+        assert np.all(owner[first:first + cnt] == tid)
+        covered[first:first + cnt] += 1
+    assert np.all(covered == 1)
+    # one slot per item; a tile's slots are consecutive and in row order
+    assert sorted(items[:, 3]) == list(range(len(items)))
+    ts = st['tile_slots'].astype(np.int64)
+    assert ts.shape == (int(np.prod(tgrid)), 2)
+    assert ts[:, 1].sum() == len(items)
+    by_slot = items[np.argsort(items[:, 3])]
+    for t in np.nonzero(ts[:, 1])[0]:
+        blk = by_slot[ts[t, 0]:ts[t, 0] + ts[t, 1]]
+        assert np.all(blk[:, 0] == t)
+        assert np.all(blk[1:, 1] == blk[:-1, 1] + blk[:-1, 2])
     # heaviest work first
     assert np.all(np.diff(items[:, 2]) <= 0)
+
+
+def test_tile_choice():
+    assert helper.tile_shape((256, 256, 256), (6, 6, 6)) == (16, 16, 16)
+    assert helper.tile_shape((512, 512), (6, 6)) == (32, 32)
+    # shrinks to fit LDS with a wide interpolator
+    t = helper.tile_shape((256, 256, 256), (8, 8, 8))
+    assert np.prod(helper.tile_pad(t, (256,) * 3, (8,) * 3)) * 16 + 256 <= helper.GRID_LDS_MAX
+    # every default tile is valid
+    for K in range(1, 200, 7):
+        for J in (1, 2, 4, 6, 8):
+            if J > K:
+                continue
+            for d in (1, 2, 3):
+                t = helper.tile_shape((K,) * d, (J,) * d)
+                assert all(helper.tile_valid(x, K, J) for x in t)
+    with pytest.raises(ValueError):
+        helper.check_tile((6, 6), (16, 16), (6, 6))       # last tile 4 < 5
+    assert helper.coefficient_bound([np.array([[3 + 4j, 1]]), np.array([[2, -1]])]) >= 10
 
 
 def test_plan_errors_follow_reference():
