@@ -1,0 +1,159 @@
+"""Measured choice of the gridding (y2k) work decomposition.
+
+How fast y2k runs depends on three knobs of the plan:
+
+* the tile shape, which sets the LDS accumulator size (tile + J - 1 per split
+  axis), hence how many workgroups fit on a CU, and how many padded partial
+  cells the gather pass reads back;
+* the workgroup size (threads), i.e. how many samples are gridded at once
+  into one accumulator;
+* the rows per work item (chunk), which splits heavy tiles so that dense
+  trajectory regions (the centre of radial and spiral k-space) spread over
+  many CUs.
+
+``tune`` times y2k on the device (HIP events on the launch stream) for a short
+list of valid candidates and returns the fastest.  The FP64 host design is
+computed once; each candidate only redoes the row sort.  Results are cached
+per device and geometry in a JSON file (``PYNUFFT_AMD_TUNE_CACHE``, default
+``~/.cache/pynufft_amd/tune.json``); ``cache=False`` skips it.
+"""
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import helper
+
+_TILES = {
+    1: [(1024,), (2048,), (4096,), (512,)],
+    2: [(32, 32), (64, 64), (64, 32), (32, 64), (16, 16), (128, 64)],
+    3: [(16, 16, 16), (16, 16, 8), (16, 8, 8), (8, 16, 16), (8, 8, 8), (24, 16, 8)],
+}
+_CHUNKS = (256, 512, 1024, 2048, 4096)
+
+
+def candidates(Kd, Jd):
+    """(tile, threads) pairs to try: the preferred tiles moved to the nearest
+    valid extent per axis, with workgroup sizes that suit the accumulator
+    (large accumulators leave room for one or two workgroups per CU, so they
+    get the larger workgroups)."""
+    Kd, Jd = tuple(Kd), tuple(Jd)
+    out = []
+    seen = set()
+    for pref in [helper.tile_shape(Kd, Jd)] + _TILES[len(Kd)]:
+        t = tuple(helper._valid_near(p, K, J) for p, K, J in zip(pref, Kd, Jd))
+        try:
+            t = helper.check_tile(t, Kd, Jd)
+        except ValueError:
+            continue
+        pad_bytes = int(np.prod(helper.tile_pad(t, Kd, Jd))) * 16
+        for thr in (256, 512, 1024):
+            if pad_bytes > 81920 and thr < 512:
+                continue
+            if pad_bytes <= 20480 and thr > 512:
+                continue
+            if (t, thr) not in seen:
+                seen.add((t, thr))
+                out.append((t, thr))
+    return out
+
+
+def _cache_path():
+    return os.environ.get('PYNUFFT_AMD_TUNE_CACHE',
+                          os.path.join(os.path.expanduser('~'), '.cache', 'pynufft_amd',
+                                       'tune.json'))
+
+
+def _key(device, Kd, Jd, M, nb):
+    name = torch.cuda.get_device_name(device)
+    mb = int(round(np.log2(max(M, 1)) * 2))      # half-octave bucket of M
+    return '%s|K%s|J%s|m%d|nb%d' % (name, 'x'.join(map(str, Kd)),
+                                   'x'.join(map(str, Jd)), mb, nb)
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, cfg):
+    path = _cache_path()
+    data = _load()
+    data[key] = cfg
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + '.tmp%d' % os.getpid()
+        with open(tmp, 'w') as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def _time(fn, reps):
+    fn()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        fn()
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / reps
+
+
+def tune(om, Nd, Kd, Jd, batch=None, host_plan=None, device=None, reps=5,
+         cache=True, chunks=_CHUNKS, verbose=False):
+    """Fastest y2k configuration for this geometry on this device.
+
+    Returns ``{'tile': tuple, 'threads': int, 'chunk': int or None, 'ms':
+    float}``.  Two stages: every (tile, threads) candidate with the default
+    rows per item, then the rows per item for the winner.
+    """
+    from .nufft import NUFFT
+    Nd, Kd, Jd = tuple(Nd), tuple(Kd), tuple(Jd)
+    nb = 1 if batch is None else int(batch)
+    st = host_plan if host_plan is not None else helper.plan(om, Nd, Kd, Jd)
+    M = int(st['M'])
+    dev = device if device is not None else torch.device('cuda', torch.cuda.current_device())
+    key = _key(dev, Kd, Jd, M, nb)
+    if cache:
+        hit = _load().get(key)
+        if hit is not None:
+            return {'tile': tuple(hit['tile']), 'threads': int(hit['threads']),
+                    'chunk': hit['chunk'], 'ms': float(hit['ms'])}
+    g = torch.Generator().manual_seed(12345)
+    shape = (M,) if batch is None else (M, nb)
+    y = torch.complex(torch.randn(shape, generator=g), torch.randn(shape, generator=g))
+    y = y.to(dev, torch.complex64)
+
+    def measure(tile, threads, chunk):
+        perm, items, ts = helper.sort_and_bin(st['base'], Kd, Jd, tile, chunk)
+        s2 = dict(st)
+        s2.update(perm=perm, items=items, tile_slots=ts, tile=tile)
+        A = NUFFT(dev)
+        A.plan(om, Nd, Kd, Jd, batch=batch, host_plan=s2, threads=threads)
+        ms = _time(lambda: A.y2k(y), reps)
+        A.release()
+        if verbose:
+            print('tune: tile %s threads %d chunk %s -> %.3f ms' % (tile, threads, chunk, ms),
+                  flush=True)
+        return ms
+
+    best = None
+    for tile, threads in candidates(Kd, Jd):
+        ms = measure(tile, threads, None)
+        if best is None or ms < best['ms']:
+            best = {'tile': tile, 'threads': threads, 'chunk': None, 'ms': ms}
+    for chunk in chunks:
+        ms = measure(best['tile'], best['threads'], chunk)
+        if ms < best['ms']:
+            best = dict(best, chunk=chunk, ms=ms)
+    if cache:
+        _store(key, {'tile': list(best['tile']), 'threads': best['threads'],
+                     'chunk': best['chunk'], 'ms': best['ms']})
+    return best

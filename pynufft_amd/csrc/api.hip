@@ -220,7 +220,7 @@ int do_y2k(const pnh_plan* p, int fp64, const void* y, void* k, void* part,
 int do_k2y(const pnh_plan* p, int fp64, const void* k, void* y, const int* skip,
            hipStream_t s) {
   InterpArgs ia = interp_args(p, fp64);
-  return pnh_k2y_launch(ia, fp64, k, y, skip, s);
+  return pnh_k2y_launch(ia, p->bins, fp64, k, y, skip, s);
 }
 
 
@@ -269,6 +269,7 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
     if (pnh_fft_factor((int)d.Kd[a], radix, ns, &np)) return -2;
     REQUIRE(d.tw_c64[a], "tw_c64[%d] missing", a);
   }
+  REQUIRE(p->Ktot < (1LL << 31), "grids of 2^31 or more cells are not supported");
   REQUIRE(pnh_lanes_per_sample(d.Jd[d.ndim - 1]) > 0, "Jd above 16 on the last axis");
   for (int a = 0; a + 1 < p->D; ++a)
     REQUIRE(d.Jd[a] <= 16, "Jd above 16 on axis %d", a);

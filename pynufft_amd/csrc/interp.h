@@ -45,7 +45,9 @@ __host__ __device__ inline int pnh_lanes_per_sample(int J_last) {
   return 0;
 }
 
-int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
+// y[perm[r]*nb + c] = sum_j coef(r, j) * grid[cell(r, j)*nb + c], by the
+// gridding work list (tile ownership): each tile's padded box staged in LDS.
+int pnh_k2y_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s);
 // out[m*D + d] (+)= sum_c Re(conj(g[m, c]) * sum_j d_d coef(m, j) * grid[cell(m, j), c])
 int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,

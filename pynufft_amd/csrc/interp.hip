@@ -323,6 +323,15 @@ __device__ __forceinline__ C cmulf(C a, C b) {
   return z;
 }
 
+// acc + a * b with contraction allowed
+template <typename C>
+__device__ __forceinline__ C cfma(C a, C b, C acc) {
+  C z;
+  z.x = __builtin_fma(a.x, b.x, __builtin_fma(-a.y, b.y, acc.x));
+  z.y = __builtin_fma(a.x, b.y, __builtin_fma(a.y, b.x, acc.y));
+  return z;
+}
+
 // Origin of a tile and, per axis, the index at which the accumulator wraps
 // (K_d for an axis spanned by one tile, else never).
 struct TileFrame {
@@ -525,24 +534,213 @@ y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
   y2k_item<T, D, CSRV, NBC, JM>(a, b, it, smem_acc, y_, part_, c0);
 }
 
-// grid[g] = sum over the partial slots covering cell g, in a fixed order:
-// neighbour masks 0 .. 2^D - 1 (bit d: the tile below along axis d), then
-// each tile's slots in chunk order.
-template <typename T, int D, int NBC>
-__device__ __forceinline__ void gather_cell(const InterpArgs& a, const BinArgs& b, int64_t g,
-                                            const typename cplx<T>::t* __restrict__ part,
-                                            typename cplx<T>::t* __restrict__ grid, int c0) {
+// ------------------------------------------------------------ k2y (tiles) --
+// Interpolation by the same tile ownership as y2k: a workgroup copies the
+// padded box of its tile (tile + J - 1 cells on split axes, wrapping across
+// the grid edge; the whole axis on spanning axes) from the grid into LDS
+// once, then every row of the work item gathers its support from LDS.  LPS
+// lanes per sample, one per last-axis offset, loop over the other axes with
+// the factors in registers; a segmented shuffle sums the lanes of a sample.
+// Nothing is accumulated across rows, so the result does not depend on
+// scheduling.  LDS: [pad_cells][NBC] of the operator's complex type.
+template <typename T, int D, bool CSRV, int NBC, int JM>
+__device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, const int4 it,
+                                         typename cplx<T>::t* __restrict__ box,
+                                         const void* __restrict__ grid_,
+                                         void* __restrict__ y_, int c0) {
   typedef typename cplx<T>::t C;
-  int c[PNH_MAXD], td[PNH_MAXD], l[PNH_MAXD];
-  int64_t rest = g;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nb = a.nb;
+  const TileFrame f = tile_frame<D>(a, b, it.x);
+  const int P1 = D > 1 ? b.pad[1] : 1;
+  const int P2 = D > 2 ? b.pad[2] : 1;
+  const C* __restrict__ grid = (const C*)grid_;
+  // padded box -> LDS (last axis fastest for coalescing)
+  for (int i = tid; i < (int)b.pad_cells; i += nthr) {
+    int rest = i;
+    int64_t g = 0;
 #pragma unroll
-  for (int d = D - 1; d >= 0; --d) {
-    const int64_t q = rest / a.K[d];
-    c[d] = (int)(rest - q * a.K[d]);
-    rest = q;
-    td[d] = c[d] / b.tile[d];
-    l[d] = c[d] - td[d] * b.tile[d];
+    for (int d = D - 1; d >= 0; --d) {
+      const int P = b.pad[d];
+      const int q = rest / P;
+      int c = f.T0[d] + (rest - q * P);
+      rest = q;
+      if (c >= a.K[d]) c -= a.K[d];
+      g += (int64_t)c * a.Kstride[d];
+    }
+#pragma unroll
+    for (int cc = 0; cc < NBC; ++cc) box[i * NBC + cc] = grid[g * nb + c0 + cc];
   }
+  __syncthreads();
+
+  const int JL = a.J[D - 1];
+  const int LPS = pnh_lanes_per_sample(JL);
+  const int SPW = 64 / LPS;
+  const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+  const int sl = lane / LPS;
+  const int jl = lane - sl * LPS;
+  const C* __restrict__ u = (const C*)a.u;
+  const int J0 = a.J[0];
+  const int J1 = D > 2 ? a.J[1] : 1;
+  const int r_end = it.y + it.z;
+  C* __restrict__ y = (C*)y_;
+
+  for (int r0 = it.y + wave * SPW; r0 < r_end; r0 += nw * SPW) {
+    const int r = r0 + sl;
+    const bool valid = sl < SPW && jl < JL && r < r_end;
+    C acc[NBC];
+#pragma unroll
+    for (int c = 0; c < NBC; ++c) acc[c] = mk<T>((T)0, (T)0);
+    if (valid) {
+      const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
+                                f.wrap[D - 1], a.K[D - 1]);
+      if (CSRV) {
+        const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * a.J[1]);
+        const C* __restrict__ val = (const C*)a.csr_val;
+        const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
+        const int64_t gs = (int64_t)SPW * JL;
+        const int x0b = D > 1 ? a.base[r] - f.T0[0] : 0;
+        const int x1b = D > 2 ? a.base[a.M + r] - f.T0[1] : 0;
+        int j0 = 0, j1 = 0;
+        for (int g = 0; g < G; ++g) {
+          int idx = xl;
+          if (D == 2) idx += wrap_local(x0b + g, f.wrap[0], a.K[0]) * P1;
+          if (D == 3)
+            idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
+                    wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
+          const C w = val[i0 + g * gs];
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+          if (D == 3 && ++j1 == J1) { j1 = 0; ++j0; }
+        }
+      } else {
+        const C ul = u[a.u_off[D - 1] + (int64_t)r * JL + jl];
+        if (D == 1) {
+#pragma unroll
+          for (int c = 0; c < NBC; ++c) acc[c] = cmulf(ul, box[xl * NBC + c]);
+        } else {
+          const int x0b = a.base[r] - f.T0[0];
+          const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
+          if (D == 2) {
+            for (int j0 = 0; j0 < J0; ++j0) {
+              const int idx = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 + xl;
+              const C w = cmulf(u0[j0], ul);
+#pragma unroll
+              for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+            }
+          } else {
+            const int x1b = a.base[a.M + r] - f.T0[1];
+            const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
+            C u0c[JM], u1c[JM];
+            int off1[JM];
+#pragma unroll
+            for (int j = 0; j < JM; ++j) {
+              if (j < J0) u0c[j] = u0[j];
+              if (j < J1) {
+                u1c[j] = u1[j];
+                off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+              }
+            }
+#pragma unroll
+            for (int j0 = 0; j0 < JM; ++j0) {
+              if (j0 < J0) {
+                const int row = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 * P2;
+                const C a0 = cmulf(u0c[j0], ul);
+#pragma unroll
+                for (int j1 = 0; j1 < JM; ++j1) {
+                  if (j1 < J1) {
+                    const C w = cmulf(a0, u1c[j1]);
+                    const int idx = row + off1[j1];
+#pragma unroll
+                    for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+                  }
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    // segmented sum over the LPS lanes of each sample (all lanes take part)
+    for (int off = 1; off < LPS; off <<= 1) {
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) {
+        const C o = shfl_down_c(acc[c], off);
+        if (jl + off < LPS) acc[c] = cadd(acc[c], o);
+      }
+    }
+    if (valid && jl == 0) {
+      const int64_t m = a.perm[r];
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
+    }
+  }
+}
+
+template <typename T, int D, bool CSRV, int NBC, int JM>
+__global__ void __launch_bounds__(1024)
+k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
+                void* __restrict__ y_, int c0, const int* __restrict__ skip) {
+  typedef typename cplx<T>::t C;
+  if (skip && *skip) return;
+  extern __shared__ long long smem_box[];
+  const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
+  k2y_item<T, D, CSRV, NBC, JM>(a, b, it, reinterpret_cast<C*>(smem_box), grid_, y_, c0);
+}
+
+// Grid cells from the partial slots that cover them.  Cell c sums, in a
+// fixed order, the slots of the tiles whose padded box holds it: neighbour
+// masks 0 .. 2^D - 1 (bit d: the tile below along axis d, whose padding
+// reaches the J_d - 1 lowest cells of the next tile), then each tile's slots
+// in row order.  One workgroup handles a run of cells of one grid row (all
+// axes but the last fixed), so the leading axes are resolved once per
+// workgroup and the last axis per thread.
+struct AxisPos {
+  int t, x;          // tile index and accumulator index along the axis
+};
+
+__device__ __forceinline__ bool axis_source(const InterpArgs& a, const BinArgs& b, int d,
+                                            int c, bool below, AxisPos& p) {
+  const int tile = b.tile[d], tg = b.tgrid[d];
+  if (tg == 1) {
+    p.t = 0;
+    p.x = c;
+    return !below;
+  }
+  const int t = c / tile;
+  const int l = c - t * tile;
+  if (!below) {
+    p.t = t;
+    p.x = l;
+    return true;
+  }
+  const int tp = t == 0 ? tg - 1 : t - 1;
+  p.t = tp;
+  p.x = l + min(tile, a.K[d] - tp * tile);
+  return l < a.J[d] - 1;
+}
+
+template <typename T, int D, int NBC>
+__device__ __forceinline__ void gather_row(const InterpArgs& a, const BinArgs& b, int64_t row,
+                                           int c_last, const typename cplx<T>::t* __restrict__ part,
+                                           typename cplx<T>::t* __restrict__ grid, int c0) {
+  typedef typename cplx<T>::t C;
+  // leading axes: both choices (own / below) per axis
+  AxisPos lead[2][PNH_MAXD];
+  bool lead_ok[2][PNH_MAXD];
+  int rest = (int)row;               // grids hold fewer than 2^31 cells
+#pragma unroll
+  for (int d = D - 2; d >= 0; --d) {
+    const int q = rest / a.K[d];
+    const int c = rest - q * a.K[d];
+    rest = q;
+    lead_ok[0][d] = axis_source(a, b, d, c, false, lead[0][d]);
+    lead_ok[1][d] = axis_source(a, b, d, c, true, lead[1][d]);
+  }
+  AxisPos lp[2];
+  bool lok[2];
+  lok[0] = axis_source(a, b, D - 1, c_last, false, lp[0]);
+  lok[1] = axis_source(a, b, D - 1, c_last, true, lp[1]);
   C s[NBC];
 #pragma unroll
   for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
@@ -553,17 +751,12 @@ __device__ __forceinline__ void gather_cell(const InterpArgs& a, const BinArgs& 
     int64_t tix = 0, pidx = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      int t = td[d], x = l[d];
-      if (b.tgrid[d] == 1) {
-        ok &= !((mask >> d) & 1);
-        x = c[d];
-      } else if ((mask >> d) & 1) {
-        ok &= x < a.J[d] - 1;
-        t = t == 0 ? b.tgrid[d] - 1 : t - 1;
-        x += min(b.tile[d], a.K[d] - t * b.tile[d]);
-      }
-      tix = tix * b.tgrid[d] + t;
-      pidx = pidx * b.pad[d] + x;
+      const int bit = (mask >> d) & 1;
+      AxisPos p;
+      if (d == D - 1) { ok &= lok[bit]; p = lp[bit]; }
+      else { ok &= lead_ok[bit][d]; p = lead[bit][d]; }
+      tix = tix * b.tgrid[d] + p.t;
+      pidx = pidx * b.pad[d] + p.x;
     }
     if (!ok) continue;
     const int2 sl = ts[tix];
@@ -573,19 +766,22 @@ __device__ __forceinline__ void gather_cell(const InterpArgs& a, const BinArgs& 
       for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
     }
   }
+  const int64_t g = row * a.K[D - 1] + c_last;
 #pragma unroll
   for (int cc = 0; cc < NBC; ++cc) grid[g * a.nb + c0 + cc] = s[cc];
 }
 
+// blockIdx.x: grid row (all axes but the last), blockIdx.y * blockDim.x +
+// threadIdx.x: cell along the last axis
 template <typename T, int D, int NBC>
 __global__ void __launch_bounds__(256)
-y2k_gather_kernel(InterpArgs a, BinArgs b, int64_t n_cells, const void* __restrict__ part_,
+y2k_gather_kernel(InterpArgs a, BinArgs b, const void* __restrict__ part_,
                   void* __restrict__ grid_, int c0, const int* __restrict__ skip) {
   typedef typename cplx<T>::t C;
   if (skip && *skip) return;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_cells) return;
-  gather_cell<T, D, NBC>(a, b, g, (const C*)part_, (C*)grid_, c0);
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (c >= a.K[D - 1]) return;
+  gather_row<T, D, NBC>(a, b, blockIdx.x, c, (const C*)part_, (C*)grid_, c0);
 }
 
 template <typename T, int D, int LPS, bool CSRV, int NBC>
@@ -643,6 +839,55 @@ void allow_lds(K kernel, size_t bytes) {
 }
 
 template <typename T, int D, bool CSRV, int NBC, int JM>
+int launch_k2y_tile(const InterpArgs& a, const BinArgs& b, const void* grid, void* y, int c0,
+                    size_t lds, const int* skip, hipStream_t s) {
+  auto kern = k2y_tile_kernel<T, D, CSRV, NBC, JM>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, grid, y,
+                     c0, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+template <typename T, int D, bool CSRV, int NBC>
+int launch_k2y_tile_j(const InterpArgs& a, const BinArgs& b, const void* grid, void* y, int c0,
+                      size_t lds, const int* skip, hipStream_t s) {
+  if (D < 3) return launch_k2y_tile<T, D, CSRV, NBC, 1>(a, b, grid, y, c0, lds, skip, s);
+  const int jm = a.J[0] > a.J[1] ? a.J[0] : a.J[1];
+  if (jm <= 8) return launch_k2y_tile<T, D, CSRV, NBC, 8>(a, b, grid, y, c0, lds, skip, s);
+  if (jm <= 16) return launch_k2y_tile<T, D, CSRV, NBC, 16>(a, b, grid, y, c0, lds, skip, s);
+  pnh_set_error("interpolator size %d above 16", jm);
+  return -2;
+}
+
+// k2y over the tile work list: coils in passes of up to 4 that fit in LDS.
+template <typename T, int D, bool CSRV>
+int launch_k2y_tiles(const InterpArgs& a, const BinArgs& b, const void* grid, void* y,
+                     const int* skip, hipStream_t s) {
+  typedef typename cplx<T>::t C;
+  if (b.n_items == 0) return 0;
+  const size_t lim = pnh_lds_limit();
+  int c0 = 0;
+  while (c0 < a.nb) {
+    const int left = a.nb - c0;
+    int nbc = left >= 4 ? 4 : (left >= 2 ? 2 : 1);
+    while (nbc > 1 && (size_t)b.pad_cells * nbc * sizeof(C) > lim) nbc >>= 1;
+    const size_t lds = (size_t)b.pad_cells * nbc * sizeof(C);
+    if (lds > lim) {
+      pnh_set_error("interpolation tile needs %zu bytes of LDS (device limit %zu)", lds, lim);
+      return -2;
+    }
+    int rc;
+    if (nbc == 4) rc = launch_k2y_tile_j<T, D, CSRV, 4>(a, b, grid, y, c0, lds, skip, s);
+    else if (nbc == 2) rc = launch_k2y_tile_j<T, D, CSRV, 2>(a, b, grid, y, c0, lds, skip, s);
+    else rc = launch_k2y_tile_j<T, D, CSRV, 1>(a, b, grid, y, c0, lds, skip, s);
+    if (rc) return rc;
+    c0 += nbc;
+  }
+  return 0;
+}
+
+template <typename T, int D, bool CSRV, int NBC, int JM>
 int launch_tile(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
                 size_t lds, const int* skip, hipStream_t s) {
   auto kern = y2k_tile_kernel<T, D, CSRV, NBC, JM>;
@@ -689,10 +934,12 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
       else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, lds, skip, s);
       if (rc) return rc;
     }
-    dim3 gg((unsigned)((n_cells + 255) / 256));
-    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
-    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
-    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(256), 0, s, a, b, n_cells, part, grid, c0, skip);
+    const int lastK = a.K[D - 1];
+    const int gt = lastK >= 256 ? 256 : (lastK >= 128 ? 128 : 64);
+    dim3 gg((unsigned)(n_cells / lastK), (unsigned)((lastK + gt - 1) / gt));
+    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
+    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
+    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
     PNH_LAUNCH_CHECK();
     c0 += nbc;
   }
@@ -739,13 +986,17 @@ int launch_dom_j(const InterpArgs& a, const void* grid, const void* g, void* out
 
 }  // namespace
 
-int pnh_k2y_launch(const InterpArgs& a, int fp64, const void* grid, void* y,
+int pnh_k2y_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s) {
   const bool csrv = a.csr_val != nullptr;
+  if (b.threads < 64 || b.threads > 1024 || b.threads % 64) {
+    pnh_set_error("workgroup size %d is not a multiple of 64 in 64..1024", b.threads);
+    return -1;
+  }
 #define K2Y_CASE(T, D)                                                       \
-  return csrv ? launch_k2y_nb<T, D, true>(a, grid, y, skip, s)               \
-              : launch_k2y_nb<T, D, false>(a, grid, y, skip, s);
-#define K2Y_CASE64(D) return launch_k2y_nb<double, D, false>(a, grid, y, skip, s);
+  return csrv ? launch_k2y_tiles<T, D, true>(a, b, grid, y, skip, s)         \
+              : launch_k2y_tiles<T, D, false>(a, b, grid, y, skip, s);
+#define K2Y_CASE64(D) return launch_k2y_tiles<double, D, false>(a, b, grid, y, skip, s);
   if (fp64) {
     if (csrv) { pnh_set_error("FP64 operators use the per-axis factors"); return -2; }
     switch (a.D) { case 1: K2Y_CASE64(1) case 2: K2Y_CASE64(2) case 3: K2Y_CASE64(3) }
@@ -879,8 +1130,11 @@ gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t, int64_
   const int64_t end = (lb + 1) * chunk < n ? (lb + 1) * chunk : n;
   const C* part = (const C*)part_ + probs[p].part_off;
   C* grid = (C*)k_ + probs[p].k_off;
-  for (int64_t g = lb * chunk + threadIdx.x; g < end; g += blockDim.x)
-    gather_cell<T, D, 1>(a, b, g, part, grid, 0);
+  const int lastK = a.K[D - 1];
+  for (int g = (int)(lb * chunk) + threadIdx.x; g < (int)end; g += blockDim.x) {
+    const int row = g / lastK;
+    gather_row<T, D, 1>(a, b, row, g - row * lastK, part, grid, 0);
+  }
 }
 
 template <typename T, int D>

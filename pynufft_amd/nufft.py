@@ -80,7 +80,7 @@ class NUFFT:
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
-             tile=None, chunk=None, host_plan=None, threads=0):
+             tile=None, chunk=None, host_plan=None, threads=0, tune=False):
         """Design the interpolator on the host in FP64 and upload it.
 
         Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
@@ -91,13 +91,23 @@ class NUFFT:
         ``legacy=True``.  ``tile`` (gridding tile per axis), ``chunk`` (rows
         per gridding work item, at most 4096) and ``threads`` (gridding
         workgroup size) are tuning knobs; the defaults come from
-        ``helper.tile_shape``, ``helper.sort_and_bin`` and the library.
+        ``helper.tile_shape``, ``helper.sort_and_bin`` and the library;
+        ``tune=True`` measures candidates on the device and keeps the fastest
+        (``pynufft_amd.autotune``).
         ``host_plan`` may pass a ``helper.plan`` result computed elsewhere
         (e.g. in a thread pool).  Returns 0.
         """
         self.release()
         st = host_plan if host_plan is not None else \
             helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_rows=chunk)
+        if tune:
+            from . import autotune
+            cfg = autotune.tune(om, Nd, Kd, Jd, batch=batch, host_plan=st, device=self.device)
+            perm, items, ts = helper.sort_and_bin(st['base'], Kd, Jd, cfg['tile'], cfg['chunk'])
+            st = dict(st)
+            st.update(perm=perm, items=items, tile_slots=ts, tile=cfg['tile'])
+            threads = cfg['threads']
+            self.tuned = cfg
         dd = len(Nd)
         self.ndims = dd
         self.ft_axes = tuple(range(dd)) if ft_axes is None else tuple(ft_axes)

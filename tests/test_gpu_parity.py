@@ -248,3 +248,72 @@ def test_random_geometries(hip, case):
     assert rel(A.k2y(k), ref.k2y(k)) < TOL64
     assert rel(A.forward(x), ref.forward(x)) < TOL
     assert rel(A.adjoint(y.astype(np.complex64)), ref.adjoint(y.astype(np.complex64))) < TOL
+
+
+def test_autotuned_plan_matches_oracle(hip, tmp_path, monkeypatch):
+    """plan(..., tune=True) measures the gridding candidates, caches the
+    winner and keeps parity with the reference outputs."""
+    from pynufft_amd import autotune
+    from pynufft_amd.nufft import NUFFT
+    monkeypatch.setenv('PYNUFFT_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    g = golden('plain2d')
+    Nd, Kd, Jd = geometry(g)
+    A = NUFFT()
+    A.plan(g['om'], Nd, Kd, Jd, tune=True)
+    assert (tmp_path / 'tune.json').exists()
+    assert A.tuned['tile'] in [c[0] for c in autotune.candidates(Kd, Jd)]
+    assert rel(A.forward(g['x']), g['forward']) < 1e-5
+    assert rel(A.adjoint(g['adjoint_in']), g['adjoint']) < 1e-5
+    # a second plan reads the cache
+    B = NUFFT()
+    B.plan(g['om'], Nd, Kd, Jd, tune=True)
+    assert B.tuned == A.tuned
+
+
+def _split_tiles(Kd, Jd):
+    """Smallest valid tile per axis: splits every axis that can be split, so
+    padded accumulators wrap across the grid edge."""
+    from pynufft_amd import helper
+    out = []
+    for K, J in zip(Kd, Jd):
+        ts = [t for t in range(max(1, J - 1), K) if helper.tile_valid(t, K, J)]
+        out.append(ts[0] if ts else K)
+    return tuple(out)
+
+
+@pytest.mark.parametrize('name', TINY_CASES)
+@pytest.mark.parametrize('threads', [64, 256])
+def test_split_tiles_and_chunks(hip, name, threads):
+    """Small tiles (padding wrapping across the grid edge), 3-row work items
+    and a single-wave workgroup: complex64 and FP64 y2k against the
+    reference; y2k is bit-identical across tile choices only up to rounding,
+    so each is compared with the reference."""
+    from pynufft_amd.nufft import NUFFT
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    A = NUFFT()
+    A.plan(g['om'], Nd, Kd, Jd, tile=_split_tiles(Kd, Jd), chunk=3, threads=threads)
+    assert rel(A.y2k(g['adjoint_in']), g['y2k']) < 1e-6
+    k64 = A.y2k(g['adjoint_in'].astype(np.complex128))
+    assert rel(k64, g['y2k']) < 1e-13
+    assert rel(A.adjoint(g['adjoint_in']), g['adjoint']) < 1e-5
+
+
+def test_non_finite_samples_reach_their_cells(hip):
+    """A NaN sample makes (at least) its own support cells NaN and leaves
+    tiles it does not touch finite."""
+    from pynufft_amd.nufft import NUFFT
+    g = golden('plain2d')
+    Nd, Kd, Jd = geometry(g)
+    A = NUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    y = g['adjoint_in'].astype(np.complex64).copy()
+    y[7] = np.nan
+    k = A.y2k(y)
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], Nd, Kd, Jd)
+    yz = np.zeros_like(y)
+    yz[7] = 1.0
+    support = np.abs(B.y2k(yz)) > 0
+    assert np.all(np.isnan(k[support]))
+    assert np.isfinite(k).sum() > 0.5 * k.size

@@ -96,6 +96,50 @@ def emulate_y2k(st, Kd, Jd, y):
     return out
 
 
+def emulate_k2y(st, Kd, Jd, k):
+    """The tile k2y kernel's algorithm in numpy: per work item, the padded
+    box of the tile copied from the grid ((T0 + x) wrapped once), every row
+    gathering its support box from it."""
+    D = len(Kd)
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
+    pad = helper.tile_pad(tile, Kd, Jd)
+    perm = st['perm']
+    base = st['base'][:, perm]
+    u = [st['ud'][a][perm] for a in range(D)]
+    y = np.full(perm.size, np.nan, dtype=complex)
+    for tid, first, cnt, slot in st['items']:
+        T0, wrap = tile_frame(int(tid), tile, tgrid, Kd)
+        box = np.zeros(pad, dtype=complex)
+        for x in np.ndindex(*pad):
+            c = [T0[d] + x[d] for d in range(D)]
+            c = tuple(c[d] - Kd[d] if c[d] >= Kd[d] else c[d] for d in range(D))
+            assert all(0 <= c[d] < Kd[d] for d in range(D))
+            box[x] = k[c]
+        for r in range(first, first + cnt):
+            acc = 0j
+            for j in np.ndindex(*Jd):
+                x = [base[d][r] - T0[d] + j[d] for d in range(D)]
+                x = tuple(x[d] - Kd[d] if x[d] >= wrap[d] else x[d] for d in range(D))
+                w = 1.0
+                for d in range(D):
+                    w = w * u[d][r][j[d]]
+                acc += w * box[x]
+            y[perm[r]] = acc
+    return y
+
+
+@pytest.mark.parametrize('name', TINY_CASES)
+@pytest.mark.parametrize('mode', ['default', 'small_tiles'])
+def test_tile_k2y_emulation(name, mode):
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    tile = _small_tiles(Kd, Jd) if mode == 'small_tiles' else None
+    st = helper.plan(g['om'], Nd, Kd, Jd, tile=tile, chunk_rows=3)
+    y = emulate_k2y(st, Kd, Jd, g['xx2k'])
+    assert rel(y, g['forward']) < 1e-6      # xx2k is stored in complex64
+
+
 def test_tile_validity_and_gather_sources():
     """For every valid tile on small axes, the gather's neighbour rule finds
     exactly the (tile, accumulator index) pairs whose padded box holds the
@@ -220,6 +264,17 @@ def test_tile_choice():
                 assert all(helper.tile_valid(x, K, J) for x in t)
     with pytest.raises(ValueError):
         helper.check_tile((6, 6), (16, 16), (6, 6))       # last tile 4 < 5
+
+
+def test_autotune_candidates_are_valid():
+    from pynufft_amd import autotune
+    for Kd, Jd in [((256, 256, 256), (6, 6, 6)), ((512, 512), (6, 6)), ((384, 384), (6, 6)),
+                   ((64,), (6,)), ((100, 90), (8, 5)), ((24, 16, 20), (4, 6, 8))]:
+        cands = autotune.candidates(Kd, Jd)
+        assert cands and len(set(cands)) == len(cands)
+        for tile, threads in cands:
+            assert helper.check_tile(tile, Kd, Jd) == tile
+            assert threads in (256, 512, 1024)
     assert helper.coefficient_bound([np.array([[3 + 4j, 1]]), np.array([[2, -1]])]) >= 10
 
 
