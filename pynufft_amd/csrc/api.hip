@@ -519,12 +519,13 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 
 struct pnh_batch {
   int n, D, J_last, J_max, grid_threads;
+  int J[PNH_MAXD];
   int64_t N_total, M_total, K_total, part_elems, pad_cells_max;
   int fft_threads;
   BatchProblem* probs;                 // device (inside the caller's table)
-  BatchTable k2y_t;
   const int2* items;
   int64_t n_items;
+  BatchTable gather_t;                 // one block per tile
   BatchTable fft_t[3][PNH_MAXD];       // [kind][axis]
   size_t fft_lds[3][PNH_MAXD];
   int fft_pow2[3][PNH_MAXD];
@@ -593,6 +594,7 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   b->D = D;
   b->J_last = plans[0].J[D - 1];
   b->J_max = 1;
+  for (int a = 0; a < PNH_MAXD; ++a) b->J[a] = a < D ? plans[0].J[a] : 1;
   for (int a = 0; a < D; ++a) b->J_max = std::max(b->J_max, plans[0].J[a]);
   b->pad_cells_max = 1;
   b->grid_threads = 64;
@@ -630,12 +632,11 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
 
   Blob blob;
   const size_t off_probs = blob.add(probs.data(), probs.size() * sizeof(BatchProblem));
-  // k2y: 4 wavefronts of SPW samples per block
-  const int SPW = 64 / pnh_lanes_per_sample(b->J_last);
-  std::vector<int64_t> nb_k2y(n);
-  for (int i = 0; i < n; ++i) nb_k2y[i] = ((plans[i].M + SPW - 1) / SPW + 3) / 4;
-  size_t ok_tab, ok_first;
-  add_table(blob, nb_k2y, &ok_tab, &ok_first, &b->k2y_t.n_blocks);
+  // gridding gather: one block per tile
+  std::vector<int64_t> nb_tiles(n);
+  for (int i = 0; i < n; ++i) nb_tiles[i] = plans[i].bins.n_tiles;
+  size_t og_tab, og_first;
+  add_table(blob, nb_tiles, &og_tab, &og_first, &b->gather_t.n_blocks);
   // FFT passes per kind and axis
   size_t of_tab[3][PNH_MAXD], of_first[3][PNH_MAXD];
   for (int kind = 0; kind < 3; ++kind)
@@ -695,8 +696,8 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
                                hipMemcpyHostToDevice, stream));
   PNH_CHECK_HIP(hipStreamSynchronize(stream));
   b->probs = (BatchProblem*)(base + off_probs);
-  b->k2y_t.tab = (const int*)(base + ok_tab);
-  b->k2y_t.first = (const int*)(base + ok_first);
+  b->gather_t.tab = (const int*)(base + og_tab);
+  b->gather_t.first = (const int*)(base + og_first);
   for (int kind = 0; kind < 3; ++kind)
     for (int a = 0; a < D; ++a) {
       b->fft_t[kind][a].tab = (const int*)(base + of_tab[kind][a]);
@@ -731,8 +732,8 @@ int batch_passes(const pnh_batch* b, int kind, int fp64, const void* image_in,
 
 int batch_y2k(const pnh_batch* b, int fp64, const void* y, void* k, void* part,
               const int* done, const int* all_done, hipStream_t s) {
-  return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->vec_t[0], b->vec_chunk,
-                              b->D, b->J_max, fp64, b->pad_cells_max, b->grid_threads, y, k,
+  return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->gather_t,
+                              b->D, b->J, fp64, b->pad_cells_max, b->grid_threads, y, k,
                               part, done, all_done, s);
 }
 
@@ -794,7 +795,8 @@ int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double 
   for (int it = 0; it < maxiter; ++it) {
     if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, b->vec_t[0], ch, w.p, w.r, nullptr,
                                    nullptr, cg, it == 0, s))) return rc;
-    if ((rc = pnh_batch_k2y_launch(b->probs, b->k2y_t, b->D, b->J_last, 1, w.p, w.yt, cg.done,
+    if ((rc = pnh_batch_k2y_launch(b->probs, b->items, b->n_items, b->D, b->J, 1,
+                                   b->pad_cells_max, b->grid_threads, w.p, w.yt, cg.done,
                                    cg.all_done, s))) return rc;
     if ((rc = batch_y2k(b, 1, w.yt, w.q, w.part, cg.done, cg.all_done, s))) return rc;
     if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, b->vec_t[0], ch, w.p, w.q, nullptr,
@@ -875,7 +877,8 @@ int pnh_batch_forward(const pnh_batch* b, const void* x, void* y, void* ws, void
   hipStream_t s = (hipStream_t)stream;
   int rc = batch_passes(b, BFFT_FWD, 0, x, nullptr, ws, nullptr, nullptr, s);
   if (rc) return rc;
-  return pnh_batch_k2y_launch(b->probs, b->k2y_t, b->D, b->J_last, 0, ws, y, nullptr, nullptr, s);
+  return pnh_batch_k2y_launch(b->probs, b->items, b->n_items, b->D, b->J, 0,
+                              b->pad_cells_max, b->grid_threads, ws, y, nullptr, nullptr, s);
 }
 
 int pnh_batch_adjoint(const pnh_batch* b, const void* y, void* x, void* ws, void* stream) {

@@ -41,13 +41,14 @@ struct BatchCg {
   double2* part;             // per-block partial sums
 };
 
-int pnh_batch_k2y_launch(const BatchProblem* probs, const BatchTable& t, int D, int J_last,
-                         int fp64, const void* k, void* y, const int* done,
-                         const int* all_done, hipStream_t s);
+// Interpolates every problem's items (tile ownership, padded boxes in LDS).
+int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n_items, int D,
+                         const int* Jd, int fp64, int64_t pad_cells, int threads, const void* k,
+                         void* y, const int* done, const int* all_done, hipStream_t s);
 // Grids every problem's items into partial slots, then gathers the grids
-// (blocks of `chunk` cells by table `gather`).
+// (one block per tile, by table `gather`).
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const BatchTable& gather, int64_t chunk, int D, int jmax, int fp64,
+                         const BatchTable& gather, int D, const int* Jd, int fp64,
                          int64_t pad_cells, int threads, const void* y, void* k, void* part,
                          const int* done, const int* all_done, hipStream_t s);
 int pnh_batch_fft_launch(const BatchProblem* probs, const BatchTable& t, int axis,

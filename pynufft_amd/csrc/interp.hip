@@ -1,20 +1,25 @@
 // Interpolation (k2y) and deterministic gridding (y2k) for the min-max NUFFT.
 //
-// Samples are stored in a trajectory-sorted order (row r -> sample perm[r]),
-// so neighbouring lanes gather neighbouring grid cells.  The coefficient of
-// row r at support offset (j0, j1, j2) is the Kronecker product of the FP64
-// per-dimension min-max factors, ((u0[j0] * u1[j1]) * u2[j2]) -- the product
-// order of the reference's Khatri-Rao construction -- either computed on the
-// fly from the factors (FACT) or read from a stored complex64 table (CSR,
-// rows interleaved in slices of 64 so that lane r%64 reads entry j of its row
-// with unit stride).  The support cell along dim d is (base_d + j_d) mod K_d.
+// Rows are the samples sorted by the gridding tile that holds their first
+// support cell (row r -> sample perm[r]).  The coefficient of row r at
+// support offset (j0, j1, j2) is the product of the per-dimension min-max
+// factors u0[j0] u1[j1] u2[j2] (the reference's Khatri-Rao construction),
+// either formed on the fly from the factors or read from a stored complex64
+// table (CSR format: rows interleaved in slices of S = 64 / lanes-per-sample
+// so that a wavefront reads consecutive entries).  The support cell along
+// dim d is (base_d + j_d) mod K_d.
 //
-// y2k is deterministic gridding by tile ownership: every row is gridded by
-// the tile that holds its first support cell, into an LDS accumulator padded
-// to hold whole support boxes, with exact 64-bit fixed-point integer atomics
+// Both directions work tile by tile over the same work list.  k2y copies a
+// tile's padded box of the grid into LDS once and gathers every row of the
+// tile from it.  y2k grids every row into an LDS accumulator padded to hold
+// whole support boxes, with exact 64-bit fixed-point integer atomics
 // (order-independent, hence bit-reproducible); a gather pass then sums the
 // padded partial tiles that cover each grid cell in a fixed order.  No float
 // atomics touch global memory.
+#include <string.h>
+
+#include <type_traits>
+
 #include "common.h"
 #include "interp.h"
 
@@ -56,113 +61,6 @@ __device__ __forceinline__ int64_t csr_index(int64_t r, int g, int jl, int G,
   const int64_t slice = r / SPW;
   const int rl = (int)(r - slice * SPW);
   return ((slice * G + g) * SPW + rl) * JL + jl;
-}
-
-// ---------------------------------------------------------------- k2y ----
-// JLC lanes per sample (one per last-axis support offset), SPW = 64/JLC
-// samples per wavefront; per-lane partial sums are combined with a segmented
-// shuffle reduction.  JLC == 0: generic, 16 lanes per sample (J_last <= 16).
-// y[perm[r]*nb + c0 + c] = sum_j coef(r, j) * grid[cell(r, j)*nb + c0 + c]
-template <typename T, int D, int JLC, bool CSRV, int NBC>
-__device__ __forceinline__ void k2y_body(const InterpArgs& a, int64_t wave,
-                                         const void* __restrict__ grid_,
-                                         void* __restrict__ y_, int c0) {
-  typedef typename cplx<T>::t C;
-  constexpr int LPS = JLC > 0 ? JLC : 16;
-  constexpr int SPW = 64 / LPS;
-  const int lane = threadIdx.x & 63;
-  const int sl = lane / LPS;
-  const int jl = lane - sl * LPS;
-  const int JL = a.J[D - 1];
-  const int64_t r = wave * SPW + sl;
-  const bool valid = sl < SPW && jl < JL && r < a.M;
-  const C* __restrict__ grid = (const C*)grid_;
-  const int nb = a.nb;
-
-  C acc[NBC];
-#pragma unroll
-  for (int c = 0; c < NBC; ++c) acc[c] = mk<T>((T)0, (T)0);
-
-  if (valid) {
-    const int J0 = a.J[0];
-    const int J1 = D > 1 ? a.J[1] : 1;
-    if (CSRV) {
-      const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
-      const C* __restrict__ val = (const C*)a.csr_val;
-      const int* __restrict__ col = a.csr_col;
-      const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
-      const int64_t gs = (int64_t)SPW * JL;
-      for (int g = 0; g < G; ++g) {
-        const C w = val[i0 + g * gs];
-        const int64_t cell = col[i0 + g * gs];
-#pragma unroll
-        for (int c = 0; c < NBC; ++c)
-          acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-      }
-    } else {
-      const C* __restrict__ u = (const C*)a.u;
-      const C ul = u[a.u_off[D - 1] + r * JL + jl];
-      const int bl = a.base[(D - 1) * a.M + r];
-      const int64_t cl = wrap_add(bl, jl, a.K[D - 1]);
-      if (D == 1) {
-#pragma unroll
-        for (int c = 0; c < NBC; ++c)
-          acc[c] = cmul(ul, grid[cl * nb + c0 + c]);
-      } else {
-        const int b0 = a.base[r];
-        const C* u0 = u + a.u_off[0] + r * J0;
-        if (D == 2) {
-          for (int j0 = 0; j0 < J0; ++j0) {
-            const int64_t cell = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
-            const C w = cmul(u0[j0], ul);
-#pragma unroll
-            for (int c = 0; c < NBC; ++c)
-              acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-          }
-        } else {
-          const int b1 = a.base[a.M + r];
-          const C* u1 = u + a.u_off[1] + r * J1;
-          for (int j0 = 0; j0 < J0; ++j0) {
-            const int64_t i0 = (int64_t)wrap_add(b0, j0, a.K[0]) * a.Kstride[0] + cl;
-            const C w0 = u0[j0];
-            for (int j1 = 0; j1 < J1; ++j1) {
-              const int64_t cell = i0 + (int64_t)wrap_add(b1, j1, a.K[1]) * a.Kstride[1];
-              const C w = cmul(cmul(w0, u1[j1]), ul);
-#pragma unroll
-              for (int c = 0; c < NBC; ++c)
-                acc[c] = cadd(acc[c], cmul(w, grid[cell * nb + c0 + c]));
-            }
-          }
-        }
-      }
-    }
-  }
-  // segmented reduction over the LPS lanes of each sample (lane jl == 0 ends
-  // with the sum of lanes jl < JL of its group)
-#pragma unroll
-  for (int off = 1; off < LPS; off <<= 1) {
-#pragma unroll
-    for (int c = 0; c < NBC; ++c) {
-      const C o = shfl_down_c(acc[c], off);
-      if (jl + off < LPS) acc[c] = cadd(acc[c], o);
-    }
-  }
-  if (valid && jl == 0) {
-    const int64_t m = a.perm[r];
-    C* __restrict__ y = (C*)y_;
-#pragma unroll
-    for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
-  }
-}
-
-
-template <typename T, int D, int JLC, bool CSRV, int NBC>
-__global__ void __launch_bounds__(256)
-k2y_kernel(InterpArgs a, const void* __restrict__ grid_, void* __restrict__ y_,
-           int c0, const int* __restrict__ skip) {
-  if (skip && *skip) return;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  k2y_body<T, D, JLC, CSRV, NBC>(a, wave, grid_, y_, c0);
 }
 
 // ------------------------------------------------------------- dy/dom ----
@@ -318,8 +216,8 @@ __device__ __forceinline__ void lds_add_c(long long* cell, C v, double scale) {
 template <typename C>
 __device__ __forceinline__ C cmulf(C a, C b) {
   C z;
-  z.x = __builtin_fma(a.x, b.x, -(a.y * b.y));
-  z.y = __builtin_fma(a.x, b.y, a.y * b.x);
+  z.x = fma(a.x, b.x, -(a.y * b.y));
+  z.y = fma(a.x, b.y, a.y * b.x);
   return z;
 }
 
@@ -327,8 +225,8 @@ __device__ __forceinline__ C cmulf(C a, C b) {
 template <typename C>
 __device__ __forceinline__ C cfma(C a, C b, C acc) {
   C z;
-  z.x = __builtin_fma(a.x, b.x, __builtin_fma(-a.y, b.y, acc.x));
-  z.y = __builtin_fma(a.x, b.y, __builtin_fma(a.y, b.x, acc.y));
+  z.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+  z.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
   return z;
 }
 
@@ -357,6 +255,16 @@ __device__ __forceinline__ int wrap_local(int x, int wrap, int K) {
   return x >= wrap ? x - K : x;
 }
 
+// q = i / n and r = i % n for 0 <= i < 2^22 and 1 <= n, with rcp = 1.0f / n
+// (the float estimate is off by at most one; one correction step fixes it)
+__device__ __forceinline__ int fdivmod(int i, int n, float rcp, int& r) {
+  int q = (int)((float)i * rcp);
+  r = i - q * n;
+  if (r < 0) { --q; r += n; }
+  else if (r >= n) { ++q; r -= n; }
+  return q;
+}
+
 // Max over the workgroup; `red` holds one slot per wave.
 template <typename T>
 __device__ __forceinline__ T block_max(T v, T* red) {
@@ -371,21 +279,26 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 }
 
 // One work item (tile id, first row, rows, slot).  LDS: the accumulator
-// [pad_cells][NBC][re, im] as int64, then reduction scratch.
-template <typename T, int D, bool CSRV, int NBC, int JM>
+// [pad_cells][NBC][re, im] as int64, then reduction scratch.  JN bounds the
+// unrolled support loops of the leading axes; JE > 0 states that they are
+// exactly JE long (no per-offset guards).
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
 __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          long long* __restrict__ acc,
                                          const void* __restrict__ y_,
                                          void* __restrict__ part_, int c0) {
   typedef typename cplx<T>::t C;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int64_t pc = b.pad_cells;
-  double* red = reinterpret_cast<double*>(acc + pc * NBC * 2);
+  const int pc = (int)b.pad_cells;
+  double* red = reinterpret_cast<double*>(acc + (int64_t)pc * NBC * 2);
   const C* __restrict__ y = (const C*)y_;
   const int nb = a.nb;
   const int r_begin = it.y, r_end = it.y + it.z;
 
-  for (int64_t i = tid; i < pc * NBC * 2; i += nthr) acc[i] = 0;
+  {
+    longlong2* acc2 = reinterpret_cast<longlong2*>(acc);
+    for (int i = tid; i < pc * NBC; i += nthr) acc2[i] = make_longlong2(0, 0);
+  }
   // scale: 2^e bounds (|Re y| + |Im y|) * |coef| for every row of the item
   double ymax = 0.0;
   int bad = 0;
@@ -420,8 +333,8 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   const int P2 = D > 2 ? b.pad[2] : 1;
   const TileFrame f = tile_frame<D>(a, b, it.x);
   const C* __restrict__ u = (const C*)a.u;
-  const int J0 = a.J[0];
-  const int J1 = D > 2 ? a.J[1] : 1;
+  const int J0 = JE > 0 ? JE : a.J[0];
+  const int J1 = D > 2 ? (JE > 0 ? JE : a.J[1]) : 1;
 
   for (int r0 = r_begin + wave * SPW; !poison && r0 < r_end; r0 += nw * SPW) {
     const int r = r0 + sl;
@@ -434,7 +347,7 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
                               f.wrap[D - 1], a.K[D - 1]);
     if (CSRV) {
-      const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * a.J[1]);
+      const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
       const C* __restrict__ val = (const C*)a.csr_val;
       const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
       const int64_t gs = (int64_t)SPW * JL;
@@ -466,15 +379,25 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     const C wl = NBC == 1 ? cmulf(ul, yv[0]) : ul;
     const int x0b = a.base[r] - f.T0[0];
     const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
-    if (D == 2) {
-      for (int j0 = 0; j0 < J0; ++j0) {
-        const int idx = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 + xl;
-        const C w = cmulf(cconj(u0[j0]), wl);
-        long long* cell = acc + (int64_t)idx * NBC * 2;
-        if (NBC == 1) lds_add_c(cell, w, scale);
-        else {
+    C u0c[JN];
+    int row0[JN];
 #pragma unroll
-          for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+    for (int j = 0; j < JN; ++j)
+      if (JE > 0 || j < J0) {
+        u0c[j] = cconj(u0[j]);
+        row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+      }
+    if (D == 2) {
+#pragma unroll
+      for (int j0 = 0; j0 < JN; ++j0) {
+        if (JE > 0 || j0 < J0) {
+          const C w = cmulf(u0c[j0], wl);
+          long long* cell = acc + (int64_t)(row0[j0] + xl) * NBC * 2;
+          if (NBC == 1) lds_add_c(cell, w, scale);
+          else {
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+          }
         }
       }
       continue;
@@ -482,26 +405,23 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     // D == 3: factors of axes 0 and 1 in registers, both loops unrolled
     const int x1b = a.base[a.M + r] - f.T0[1];
     const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
-    C u0c[JM], u1c[JM];
-    int off1[JM];
+    C u1c[JN];
+    int off1[JN];
 #pragma unroll
-    for (int j = 0; j < JM; ++j) {
-      if (j < J0) u0c[j] = cconj(u0[j]);
-      if (j < J1) {
+    for (int j = 0; j < JN; ++j)
+      if (JE > 0 || j < J1) {
         u1c[j] = cconj(u1[j]);
         off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
       }
-    }
 #pragma unroll
-    for (int j0 = 0; j0 < JM; ++j0) {
-      if (j0 < J0) {
-        const int row = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 * P2;
+    for (int j0 = 0; j0 < JN; ++j0) {
+      if (JE > 0 || j0 < J0) {
         const C a0 = cmulf(u0c[j0], wl);
 #pragma unroll
-        for (int j1 = 0; j1 < JM; ++j1) {
-          if (j1 < J1) {
+        for (int j1 = 0; j1 < JN; ++j1) {
+          if (JE > 0 || j1 < J1) {
             const C w = cmulf(u1c[j1], a0);
-            long long* cell = acc + (int64_t)(row + off1[j1]) * NBC * 2;
+            long long* cell = acc + (int64_t)(row0[j0] + off1[j1]) * NBC * 2;
             if (NBC == 1) lds_add_c(cell, w, scale);
             else {
 #pragma unroll
@@ -516,22 +436,27 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   // partial slot, scaled back
   C* __restrict__ part = (C*)part_ + (int64_t)it.w * pc * NBC;
   const T nan = (T)__builtin_nan("");
-  for (int64_t i = tid; i < pc * NBC; i += nthr) {
+  const longlong2* acc2 = reinterpret_cast<const longlong2*>(acc);
+  for (int i = tid; i < pc * NBC; i += nthr) {
     C v;
-    if (poison) v = mk<T>(nan, nan);
-    else v = mk<T>((T)((double)acc[2 * i] * unscale), (T)((double)acc[2 * i + 1] * unscale));
+    if (poison) {
+      v = mk<T>(nan, nan);
+    } else {
+      const longlong2 q = acc2[i];
+      v = mk<T>((T)((double)q.x * unscale), (T)((double)q.y * unscale));
+    }
     part[i] = v;
   }
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JM>
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
 __global__ void __launch_bounds__(1024)
 y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
                 void* __restrict__ part_, int c0, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ long long smem_acc[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  y2k_item<T, D, CSRV, NBC, JM>(a, b, it, smem_acc, y_, part_, c0);
+  y2k_item<T, D, CSRV, NBC, JN, JE>(a, b, it, smem_acc, y_, part_, c0);
 }
 
 // ------------------------------------------------------------ k2y (tiles) --
@@ -543,7 +468,7 @@ y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
 // the factors in registers; a segmented shuffle sums the lanes of a sample.
 // Nothing is accumulated across rows, so the result does not depend on
 // scheduling.  LDS: [pad_cells][NBC] of the operator's complex type.
-template <typename T, int D, bool CSRV, int NBC, int JM>
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
 __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          typename cplx<T>::t* __restrict__ box,
                                          const void* __restrict__ grid_,
@@ -556,20 +481,26 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
   const int P2 = D > 2 ? b.pad[2] : 1;
   const C* __restrict__ grid = (const C*)grid_;
   // padded box -> LDS (last axis fastest for coalescing)
-  for (int i = tid; i < (int)b.pad_cells; i += nthr) {
-    int rest = i;
-    int64_t g = 0;
+  {
+    const float r1 = 1.0f / (float)P1, r2 = 1.0f / (float)P2;
+    const int pc = (int)b.pad_cells;
+    for (int i = tid; i < pc; i += nthr) {
+      int x2 = 0, x1 = 0, x0 = i;
+      if (D > 2) x0 = fdivmod(x0, P2, r2, x2);
+      if (D > 1) x0 = fdivmod(x0, P1, r1, x1);
+      int c = f.T0[0] + x0;
+      int g = (c >= a.K[0] ? c - a.K[0] : c) * (int)a.Kstride[0];
+      if (D > 1) {
+        c = f.T0[1] + x1;
+        g += (c >= a.K[1] ? c - a.K[1] : c) * (int)a.Kstride[1];
+      }
+      if (D > 2) {
+        c = f.T0[2] + x2;
+        g += c >= a.K[2] ? c - a.K[2] : c;
+      }
 #pragma unroll
-    for (int d = D - 1; d >= 0; --d) {
-      const int P = b.pad[d];
-      const int q = rest / P;
-      int c = f.T0[d] + (rest - q * P);
-      rest = q;
-      if (c >= a.K[d]) c -= a.K[d];
-      g += (int64_t)c * a.Kstride[d];
+      for (int cc = 0; cc < NBC; ++cc) box[i * NBC + cc] = grid[(int64_t)g * nb + c0 + cc];
     }
-#pragma unroll
-    for (int cc = 0; cc < NBC; ++cc) box[i * NBC + cc] = grid[g * nb + c0 + cc];
   }
   __syncthreads();
 
@@ -580,8 +511,8 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
   const int sl = lane / LPS;
   const int jl = lane - sl * LPS;
   const C* __restrict__ u = (const C*)a.u;
-  const int J0 = a.J[0];
-  const int J1 = D > 2 ? a.J[1] : 1;
+  const int J0 = JE > 0 ? JE : a.J[0];
+  const int J1 = D > 2 ? (JE > 0 ? JE : a.J[1]) : 1;
   const int r_end = it.y + it.z;
   C* __restrict__ y = (C*)y_;
 
@@ -595,7 +526,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
       const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
                                 f.wrap[D - 1], a.K[D - 1]);
       if (CSRV) {
-        const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * a.J[1]);
+        const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
         const C* __restrict__ val = (const C*)a.csr_val;
         const int64_t i0 = csr_index(r, 0, jl, G, JL, SPW);
         const int64_t gs = (int64_t)SPW * JL;
@@ -621,36 +552,44 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
         } else {
           const int x0b = a.base[r] - f.T0[0];
           const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
-          if (D == 2) {
-            for (int j0 = 0; j0 < J0; ++j0) {
-              const int idx = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 + xl;
-              const C w = cmulf(u0[j0], ul);
+          C u0c[JN];
+          int row0[JN];
 #pragma unroll
-              for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+          for (int j = 0; j < JN; ++j)
+            if (JE > 0 || j < J0) {
+              u0c[j] = u0[j];
+              row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+            }
+          if (D == 2) {
+#pragma unroll
+            for (int j0 = 0; j0 < JN; ++j0) {
+              if (JE > 0 || j0 < J0) {
+                const C w = cmulf(u0c[j0], ul);
+                const int idx = row0[j0] + xl;
+#pragma unroll
+                for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+              }
             }
           } else {
             const int x1b = a.base[a.M + r] - f.T0[1];
             const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
-            C u0c[JM], u1c[JM];
-            int off1[JM];
+            C u1c[JN];
+            int off1[JN];
 #pragma unroll
-            for (int j = 0; j < JM; ++j) {
-              if (j < J0) u0c[j] = u0[j];
-              if (j < J1) {
+            for (int j = 0; j < JN; ++j)
+              if (JE > 0 || j < J1) {
                 u1c[j] = u1[j];
                 off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
               }
-            }
 #pragma unroll
-            for (int j0 = 0; j0 < JM; ++j0) {
-              if (j0 < J0) {
-                const int row = wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 * P2;
+            for (int j0 = 0; j0 < JN; ++j0) {
+              if (JE > 0 || j0 < J0) {
                 const C a0 = cmulf(u0c[j0], ul);
 #pragma unroll
-                for (int j1 = 0; j1 < JM; ++j1) {
-                  if (j1 < J1) {
+                for (int j1 = 0; j1 < JN; ++j1) {
+                  if (JE > 0 || j1 < J1) {
                     const C w = cmulf(a0, u1c[j1]);
-                    const int idx = row + off1[j1];
+                    const int idx = row0[j0] + off1[j1];
 #pragma unroll
                     for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
                   }
@@ -677,7 +616,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
   }
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JM>
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
 __global__ void __launch_bounds__(1024)
 k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
                 void* __restrict__ y_, int c0, const int* __restrict__ skip) {
@@ -685,145 +624,99 @@ k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
   if (skip && *skip) return;
   extern __shared__ long long smem_box[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  k2y_item<T, D, CSRV, NBC, JM>(a, b, it, reinterpret_cast<C*>(smem_box), grid_, y_, c0);
+  k2y_item<T, D, CSRV, NBC, JN, JE>(a, b, it, reinterpret_cast<C*>(smem_box), grid_, y_, c0);
 }
 
-// Grid cells from the partial slots that cover them.  Cell c sums, in a
-// fixed order, the slots of the tiles whose padded box holds it: neighbour
-// masks 0 .. 2^D - 1 (bit d: the tile below along axis d, whose padding
-// reaches the J_d - 1 lowest cells of the next tile), then each tile's slots
-// in row order.  One workgroup handles a run of cells of one grid row (all
-// axes but the last fixed), so the leading axes are resolved once per
-// workgroup and the last axis per thread.
-struct AxisPos {
-  int t, x;          // tile index and accumulator index along the axis
-};
-
-__device__ __forceinline__ bool axis_source(const InterpArgs& a, const BinArgs& b, int d,
-                                            int c, bool below, AxisPos& p) {
-  const int tile = b.tile[d], tg = b.tgrid[d];
-  if (tg == 1) {
-    p.t = 0;
-    p.x = c;
-    return !below;
-  }
-  const int t = c / tile;
-  const int l = c - t * tile;
-  if (!below) {
-    p.t = t;
-    p.x = l;
-    return true;
-  }
-  const int tp = t == 0 ? tg - 1 : t - 1;
-  p.t = tp;
-  p.x = l + min(tile, a.K[d] - tp * tile);
-  return l < a.J[d] - 1;
-}
-
+// Grid cells from the partial slots that cover them, one workgroup per
+// tile.  A cell of tile t sums, in a fixed order, the slots of the tiles
+// whose padded box holds it: neighbour masks 0 .. 2^D - 1 (bit d: the tile
+// below along axis d, whose padding reaches the J_d - 1 lowest cells of t),
+// then each tile's slots in row order.  The tile's own slot is read
+// contiguously and every slot cell is read exactly once over the launch.
 template <typename T, int D, int NBC>
-__device__ __forceinline__ void gather_row(const InterpArgs& a, const BinArgs& b, int64_t row,
-                                           int c_last, const typename cplx<T>::t* __restrict__ part,
-                                           typename cplx<T>::t* __restrict__ grid, int c0) {
+__device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& b, int tid,
+                                            const typename cplx<T>::t* __restrict__ part,
+                                            typename cplx<T>::t* __restrict__ grid, int c0) {
   typedef typename cplx<T>::t C;
-  // leading axes: both choices (own / below) per axis
-  AxisPos lead[2][PNH_MAXD];
-  bool lead_ok[2][PNH_MAXD];
-  int rest = (int)row;               // grids hold fewer than 2^31 cells
+  int td[PNH_MAXD], T0[PNH_MAXD], E[PNH_MAXD], below[PNH_MAXD], tdb[PNH_MAXD];
+  {
+    int rest = tid;
 #pragma unroll
-  for (int d = D - 2; d >= 0; --d) {
-    const int q = rest / a.K[d];
-    const int c = rest - q * a.K[d];
-    rest = q;
-    lead_ok[0][d] = axis_source(a, b, d, c, false, lead[0][d]);
-    lead_ok[1][d] = axis_source(a, b, d, c, true, lead[1][d]);
+    for (int d = D - 1; d >= 0; --d) {
+      const int tg = b.tgrid[d];
+      td[d] = rest % tg;
+      rest /= tg;
+      T0[d] = td[d] * b.tile[d];
+      E[d] = min(b.tile[d], a.K[d] - T0[d]);
+      tdb[d] = td[d] == 0 ? tg - 1 : td[d] - 1;
+      // cells of the tile below reached by its padding (none on a spanning axis)
+      below[d] = tg == 1 ? 0 : a.J[d] - 1;
+    }
   }
-  AxisPos lp[2];
-  bool lok[2];
-  lok[0] = axis_source(a, b, D - 1, c_last, false, lp[0]);
-  lok[1] = axis_source(a, b, D - 1, c_last, true, lp[1]);
-  C s[NBC];
-#pragma unroll
-  for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
+  // slots and accumulator offsets of the 2^D source tiles (uniform)
   const int2* __restrict__ ts = reinterpret_cast<const int2*>(b.tile_slots);
+  int2 src[1 << D];
+  int off[1 << D][PNH_MAXD];
 #pragma unroll
   for (int mask = 0; mask < (1 << D); ++mask) {
+    int tix = 0;
     bool ok = true;
-    int64_t tix = 0, pidx = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const int bit = (mask >> d) & 1;
-      AxisPos p;
-      if (d == D - 1) { ok &= lok[bit]; p = lp[bit]; }
-      else { ok &= lead_ok[bit][d]; p = lead[bit][d]; }
-      tix = tix * b.tgrid[d] + p.t;
-      pidx = pidx * b.pad[d] + p.x;
+      const bool bit = (mask >> d) & 1;
+      ok &= !bit || below[d] > 0;
+      const int t = bit ? tdb[d] : td[d];
+      tix = tix * b.tgrid[d] + t;
+      off[mask][d] = bit ? min(b.tile[d], a.K[d] - t * b.tile[d]) : 0;
     }
-    if (!ok) continue;
-    const int2 sl = ts[tix];
-    for (int k = 0; k < sl.y; ++k) {
-      const C* p = part + ((int64_t)(sl.x + k) * b.pad_cells + pidx) * NBC;
-#pragma unroll
-      for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
-    }
+    src[mask] = ok ? ts[tix] : make_int2(0, 0);
   }
-  const int64_t g = row * a.K[D - 1] + c_last;
+  const int E1 = D > 1 ? E[1] : 1, E2 = D > 2 ? E[2] : 1;
+  const int P1 = D > 1 ? b.pad[1] : 1, P2 = D > 2 ? b.pad[2] : 1;
+  const int n = E[0] * E1 * E2;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int l[PNH_MAXD];
+    {
+      int rest = i;
+      if (D > 2) { l[2] = rest % E2; rest /= E2; }
+      if (D > 1) { l[1] = rest % E1; rest /= E1; }
+      l[0] = rest;
+    }
+    C s[NBC];
 #pragma unroll
-  for (int cc = 0; cc < NBC; ++cc) grid[g * a.nb + c0 + cc] = s[cc];
+    for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
+#pragma unroll
+    for (int mask = 0; mask < (1 << D); ++mask) {
+      bool ok = src[mask].y > 0;
+      int pidx = 0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const bool bit = (mask >> d) & 1;
+        ok &= !bit || l[d] < below[d];
+        pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
+      }
+      if (!ok) continue;
+      for (int k = 0; k < src[mask].y; ++k) {
+        const C* p = part + ((int64_t)(src[mask].x + k) * b.pad_cells + pidx) * NBC;
+#pragma unroll
+        for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
+      }
+    }
+    int64_t g = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) g += (int64_t)(T0[d] + l[d]) * a.Kstride[d];
+#pragma unroll
+    for (int cc = 0; cc < NBC; ++cc) grid[g * a.nb + c0 + cc] = s[cc];
+  }
 }
 
-// blockIdx.x: grid row (all axes but the last), blockIdx.y * blockDim.x +
-// threadIdx.x: cell along the last axis
 template <typename T, int D, int NBC>
 __global__ void __launch_bounds__(256)
 y2k_gather_kernel(InterpArgs a, BinArgs b, const void* __restrict__ part_,
                   void* __restrict__ grid_, int c0, const int* __restrict__ skip) {
   typedef typename cplx<T>::t C;
   if (skip && *skip) return;
-  const int c = blockIdx.y * blockDim.x + threadIdx.x;
-  if (c >= a.K[D - 1]) return;
-  gather_row<T, D, NBC>(a, b, blockIdx.x, c, (const C*)part_, (C*)grid_, c0);
-}
-
-template <typename T, int D, int LPS, bool CSRV, int NBC>
-int launch_k2y_t(const InterpArgs& a, const void* grid, void* y, int c0,
-                 const int* skip, hipStream_t s) {
-  constexpr int SPW = 64 / LPS;
-  const int64_t waves = (a.M + SPW - 1) / SPW;
-  const int64_t nblk = (waves + 3) / 4;
-  hipLaunchKernelGGL((k2y_kernel<T, D, LPS, CSRV, NBC>), dim3((unsigned)nblk),
-                     dim3(256), 0, s, a, grid, y, c0, skip);
-  PNH_LAUNCH_CHECK();
-  return 0;
-}
-
-template <typename T, int D, bool CSRV, int NBC>
-int launch_k2y_j(const InterpArgs& a, const void* grid, void* y, int c0,
-                 const int* skip, hipStream_t s) {
-  switch (pnh_lanes_per_sample(a.J[D - 1])) {
-    case 1: return launch_k2y_t<T, D, 1, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 2: return launch_k2y_t<T, D, 2, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 3: return launch_k2y_t<T, D, 3, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 4: return launch_k2y_t<T, D, 4, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 6: return launch_k2y_t<T, D, 6, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 8: return launch_k2y_t<T, D, 8, CSRV, NBC>(a, grid, y, c0, skip, s);
-    case 16: return launch_k2y_t<T, D, 16, CSRV, NBC>(a, grid, y, c0, skip, s);
-    default:
-      pnh_set_error("interpolator size %d on the last axis is above 16", a.J[D - 1]);
-      return -2;
-  }
-}
-
-template <typename T, int D, bool CSRV>
-int launch_k2y_nb(const InterpArgs& a, const void* grid, void* y,
-                  const int* skip, hipStream_t s) {
-  int c0 = 0;
-  while (c0 < a.nb) {
-    int rc;
-    if (a.nb - c0 >= 4) { rc = launch_k2y_j<T, D, CSRV, 4>(a, grid, y, c0, skip, s); c0 += 4; }
-    else { rc = launch_k2y_j<T, D, CSRV, 1>(a, grid, y, c0, skip, s); c0 += 1; }
-    if (rc) return rc;
-  }
-  return 0;
+  gather_tile<T, D, NBC>(a, b, blockIdx.x, (const C*)part_, (C*)grid_, c0);
 }
 
 size_t y2k_lds_bytes(int64_t pad_cells, int nbc) {
@@ -838,26 +731,37 @@ void allow_lds(K kernel, size_t bytes) {
                               (int)bytes);
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JM>
-int launch_k2y_tile(const InterpArgs& a, const BinArgs& b, const void* grid, void* y, int c0,
-                    size_t lds, const int* skip, hipStream_t s) {
-  auto kern = k2y_tile_kernel<T, D, CSRV, NBC, JM>;
-  allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, grid, y,
-                     c0, skip);
-  PNH_LAUNCH_CHECK();
-  return 0;
+// Support-loop specialization of the leading axes: exact sizes 4, 6 and 8
+// (factor format) unrolled without guards, other sizes up to 8 or 16 with
+// guards.  Calls f(JN, JE) with std::integral_constant arguments.
+template <int D, bool CSRV, typename F>
+int dispatch_j(const InterpArgs& a, F&& f) {
+  typedef std::integral_constant<int, 0> E0;
+  if (D == 1) return f(std::integral_constant<int, 1>(), E0());
+  const int j0 = a.J[0], j1 = D > 2 ? a.J[1] : a.J[0];
+  if (!CSRV && j0 == j1) {
+    if (j0 == 4) return f(std::integral_constant<int, 4>(), std::integral_constant<int, 4>());
+    if (j0 == 6) return f(std::integral_constant<int, 6>(), std::integral_constant<int, 6>());
+    if (j0 == 8) return f(std::integral_constant<int, 8>(), std::integral_constant<int, 8>());
+  }
+  const int jm = j0 > j1 ? j0 : j1;
+  if (jm <= 8) return f(std::integral_constant<int, 8>(), E0());
+  if (jm <= 16) return f(std::integral_constant<int, 16>(), E0());
+  pnh_set_error("interpolator size %d above 16", jm);
+  return -2;
 }
 
 template <typename T, int D, bool CSRV, int NBC>
 int launch_k2y_tile_j(const InterpArgs& a, const BinArgs& b, const void* grid, void* y, int c0,
                       size_t lds, const int* skip, hipStream_t s) {
-  if (D < 3) return launch_k2y_tile<T, D, CSRV, NBC, 1>(a, b, grid, y, c0, lds, skip, s);
-  const int jm = a.J[0] > a.J[1] ? a.J[0] : a.J[1];
-  if (jm <= 8) return launch_k2y_tile<T, D, CSRV, NBC, 8>(a, b, grid, y, c0, lds, skip, s);
-  if (jm <= 16) return launch_k2y_tile<T, D, CSRV, NBC, 16>(a, b, grid, y, c0, lds, skip, s);
-  pnh_set_error("interpolator size %d above 16", jm);
-  return -2;
+  return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
+    auto kern = k2y_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, grid, y,
+                       c0, skip);
+    PNH_LAUNCH_CHECK();
+    return 0;
+  });
 }
 
 // k2y over the tile work list: coils in passes of up to 4 that fit in LDS.
@@ -887,33 +791,22 @@ int launch_k2y_tiles(const InterpArgs& a, const BinArgs& b, const void* grid, vo
   return 0;
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JM>
-int launch_tile(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
-                size_t lds, const int* skip, hipStream_t s) {
-  auto kern = y2k_tile_kernel<T, D, CSRV, NBC, JM>;
-  allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y, part,
-                     c0, skip);
-  PNH_LAUNCH_CHECK();
-  return 0;
-}
-
 template <typename T, int D, bool CSRV, int NBC>
 int launch_tile_j(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
                   size_t lds, const int* skip, hipStream_t s) {
-  if (D < 3) return launch_tile<T, D, CSRV, NBC, 1>(a, b, y, part, c0, lds, skip, s);
-  const int jm = a.J[0] > a.J[1] ? a.J[0] : a.J[1];
-  if (jm <= 8) return launch_tile<T, D, CSRV, NBC, 8>(a, b, y, part, c0, lds, skip, s);
-  if (jm <= 16) return launch_tile<T, D, CSRV, NBC, 16>(a, b, y, part, c0, lds, skip, s);
-  pnh_set_error("interpolator size %d above 16", jm);
-  return -2;
+  return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
+    auto kern = y2k_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y, part,
+                       c0, skip);
+    PNH_LAUNCH_CHECK();
+    return 0;
+  });
 }
 
 template <typename T, int D, bool CSRV>
 int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
                   void* grid, void* part, const int* skip, hipStream_t s) {
-  int64_t n_cells = 1;
-  for (int d = 0; d < D; ++d) n_cells *= a.K[d];
   if (b.threads < 64 || b.threads > 1024 || b.threads % 64) {
     pnh_set_error("gridding workgroup size %d is not a multiple of 64 in 64..1024", b.threads);
     return -1;
@@ -934,12 +827,10 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
       else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, lds, skip, s);
       if (rc) return rc;
     }
-    const int lastK = a.K[D - 1];
-    const int gt = lastK >= 256 ? 256 : (lastK >= 128 ? 128 : 64);
-    dim3 gg((unsigned)(n_cells / lastK), (unsigned)((lastK + gt - 1) / gt));
-    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
-    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
-    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(gt), 0, s, a, b, part, grid, c0, skip);
+    dim3 gg((unsigned)b.n_tiles);
+    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
+    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
+    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
     PNH_LAUNCH_CHECK();
     c0 += nbc;
   }
@@ -1079,23 +970,25 @@ int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
 
 namespace {
 
-template <typename T, int D, int LPS>
-__global__ void __launch_bounds__(256)
-k2y_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
-                 const void* __restrict__ k_, void* __restrict__ y_,
-                 const int* __restrict__ done, const int* __restrict__ all_done) {
+template <typename T, int D, int JN, int JE>
+__global__ void __launch_bounds__(1024)
+k2y_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
+                      const void* __restrict__ k_, void* __restrict__ y_,
+                      const int* __restrict__ done, const int* __restrict__ all_done) {
   typedef typename cplx<T>::t C;
   if (all_done && *all_done) return;
-  const int p = t.tab[blockIdx.x];
-  if (done && done[p]) return;
-  const int64_t lb = (int64_t)blockIdx.x - t.first[p];
+  const int2 pi = items[blockIdx.x];
+  const int p = pi.x;
+  if (done && done[p]) return;            // uniform over the workgroup
+  extern __shared__ long long smem_box[];
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
-  const int64_t wave = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  k2y_body<T, D, LPS, false, 1>(a, wave, (const C*)k_ + probs[p].k_off,
-                                (C*)y_ + probs[p].y_off, 0);
+  const BinArgs b = probs[p].bins;
+  const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
+  k2y_item<T, D, false, 1, JN, JE>(a, b, it, reinterpret_cast<C*>(smem_box),
+                               (const C*)k_ + probs[p].k_off, (C*)y_ + probs[p].y_off, 0);
 }
 
-template <typename T, int D, int JM>
+template <typename T, int D, int JN, int JE>
 __global__ void __launch_bounds__(1024)
 y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
                       const void* __restrict__ y_, void* __restrict__ part_,
@@ -1109,77 +1002,91 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
-  y2k_item<T, D, false, 1, JM>(a, b, it, smem_acc, (const C*)y_ + probs[p].y_off,
+  y2k_item<T, D, false, 1, JN, JE>(a, b, it, smem_acc, (const C*)y_ + probs[p].y_off,
                                (C*)part_ + probs[p].part_off, 0);
 }
 
-// Gather of every problem's grid: chunk cells per block, blocks by table.
+// Gather of every problem's grid: one block per tile, blocks by table.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
-gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t, int64_t chunk,
+gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
                     const void* __restrict__ part_, void* __restrict__ k_,
                     const int* __restrict__ done, const int* __restrict__ all_done) {
   typedef typename cplx<T>::t C;
   if (all_done && *all_done) return;
   const int p = t.tab[blockIdx.x];
   if (done && done[p]) return;
-  const int64_t lb = (int64_t)blockIdx.x - t.first[p];
+  const int lb = (int)(blockIdx.x - t.first[p]);
   const InterpArgs a = probs[p].ia;
   const BinArgs b = probs[p].bins;
-  const int64_t n = probs[p].n_grid;
-  const int64_t end = (lb + 1) * chunk < n ? (lb + 1) * chunk : n;
-  const C* part = (const C*)part_ + probs[p].part_off;
-  C* grid = (C*)k_ + probs[p].k_off;
-  const int lastK = a.K[D - 1];
-  for (int g = (int)(lb * chunk) + threadIdx.x; g < (int)end; g += blockDim.x) {
-    const int row = g / lastK;
-    gather_row<T, D, 1>(a, b, row, g - row * lastK, part, grid, 0);
-  }
-}
-
-template <typename T, int D>
-int batch_k2y_d(const BatchProblem* probs, const BatchTable& t, int J_last, const void* k,
-                void* y, const int* done, const int* all_done, hipStream_t s) {
-  dim3 g((unsigned)t.n_blocks), b(256);
-  switch (pnh_lanes_per_sample(J_last)) {
-    case 1: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 1>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 2: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 2>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 3: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 3>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 4: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 4>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 6: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 6>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 8: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 8>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    case 16: hipLaunchKernelGGL((k2y_batch_kernel<T, D, 16>), g, b, 0, s, probs, t, k, y, done, all_done); break;
-    default: pnh_set_error("interpolator size above 16"); return -2;
-  }
-  PNH_LAUNCH_CHECK();
-  return 0;
+  gather_tile<T, D, 1>(a, b, lb, (const C*)part_ + probs[p].part_off,
+                       (C*)k_ + probs[p].k_off, 0);
 }
 
 }  // namespace
 
-int pnh_batch_k2y_launch(const BatchProblem* probs, const BatchTable& t, int D, int J_last,
-                         int fp64, const void* k, void* y, const int* done,
-                         const int* all_done, hipStream_t s) {
-  if (t.n_blocks == 0) return 0;
-  if (fp64) {
-    switch (D) {
-      case 1: return batch_k2y_d<double, 1>(probs, t, J_last, k, y, done, all_done, s);
-      case 2: return batch_k2y_d<double, 2>(probs, t, J_last, k, y, done, all_done, s);
-      case 3: return batch_k2y_d<double, 3>(probs, t, J_last, k, y, done, all_done, s);
-    }
-  } else {
-    switch (D) {
-      case 1: return batch_k2y_d<float, 1>(probs, t, J_last, k, y, done, all_done, s);
-      case 2: return batch_k2y_d<float, 2>(probs, t, J_last, k, y, done, all_done, s);
-      case 3: return batch_k2y_d<float, 3>(probs, t, J_last, k, y, done, all_done, s);
-    }
+namespace {
+
+// Host-side argument block holding only the common support sizes (for the
+// support-loop dispatch of batch launches).
+InterpArgs j_only(const int* Jd, int D) {
+  InterpArgs a;
+  memset(&a, 0, sizeof(a));
+  a.D = D;
+  for (int d = 0; d < D; ++d) a.J[d] = Jd[d];
+  return a;
+}
+
+template <typename T, int D>
+int batch_k2y_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
+                size_t lds, int threads, const void* k, void* y, const int* done,
+                const int* all_done, hipStream_t s) {
+  return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
+    auto kern = k2y_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, k, y,
+                       done, all_done);
+    PNH_LAUNCH_CHECK();
+    return 0;
+  });
+}
+
+template <typename T, int D>
+int batch_y2k_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
+                size_t lds, int threads, const void* y, void* part, const int* done,
+                const int* all_done, hipStream_t s) {
+  return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
+    auto kern = y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, y,
+                       part, done, all_done);
+    PNH_LAUNCH_CHECK();
+    return 0;
+  });
+}
+
+}  // namespace
+
+int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n_items, int D,
+                         const int* Jd, int fp64, int64_t pad_cells, int threads, const void* k,
+                         void* y, const int* done, const int* all_done, hipStream_t s) {
+  if (n_items == 0) return 0;
+  const size_t lds = (size_t)pad_cells * (fp64 ? 16 : 8);
+  if (lds > pnh_lds_limit()) {
+    pnh_set_error("interpolation tile needs %zu bytes of LDS (device limit %zu)", lds,
+                  pnh_lds_limit());
+    return -2;
   }
-  pnh_set_error("unsupported ndim %d", D);
-  return -2;
+#define KB(T, DD) return batch_k2y_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, done, all_done, s)
+  if (fp64) { if (D == 1) KB(double, 1); if (D == 2) KB(double, 2); KB(double, 3); }
+  if (D == 1) KB(float, 1);
+  if (D == 2) KB(float, 2);
+  KB(float, 3);
+#undef KB
 }
 
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const BatchTable& gather, int64_t chunk, int D, int jmax, int fp64,
+                         const BatchTable& gather, int D, const int* Jd, int fp64,
                          int64_t pad_cells, int threads, const void* y, void* k, void* part,
                          const int* done, const int* all_done, hipStream_t s) {
   const size_t lds = y2k_lds_bytes(pad_cells, 1);
@@ -1188,35 +1095,21 @@ int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n
                   pnh_lds_limit());
     return -2;
   }
-  if (jmax > 16) { pnh_set_error("interpolator size %d above 16", jmax); return -2; }
-#define YB(T, DD, JM)                                                                  \
-  do {                                                                                 \
-    auto kern = y2k_tile_batch_kernel<T, DD, JM>;                                      \
-    allow_lds(kern, lds);                                                              \
-    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs,    \
-                       items, y, part, done, all_done);                                \
-  } while (0)
-#define YBJ(T)                                                                         \
-  do {                                                                                 \
-    if (D == 1) YB(T, 1, 1);                                                           \
-    else if (D == 2) YB(T, 2, 1);                                                      \
-    else if (jmax <= 8) YB(T, 3, 8);                                                   \
-    else YB(T, 3, 16);                                                                 \
-  } while (0)
-#define GB(T, DD) hipLaunchKernelGGL((gather_batch_kernel<T, DD>), dim3((unsigned)gather.n_blocks), \
-                                     dim3(256), 0, s, probs, gather, chunk, part, k, done, all_done)
+  int rc = 0;
+#define YB(T, DD) rc = batch_y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, y, part, done, all_done, s)
   if (n_items > 0) {
-    if (fp64) YBJ(double);
-    else YBJ(float);
-    PNH_LAUNCH_CHECK();
+    if (fp64) { if (D == 1) YB(double, 1); else if (D == 2) YB(double, 2); else YB(double, 3); }
+    else { if (D == 1) YB(float, 1); else if (D == 2) YB(float, 2); else YB(float, 3); }
+    if (rc) return rc;
   }
+#undef YB
+#define GB(T, DD) hipLaunchKernelGGL((gather_batch_kernel<T, DD>), dim3((unsigned)gather.n_blocks), \
+                                     dim3(256), 0, s, probs, gather, part, k, done, all_done)
   if (gather.n_blocks > 0) {
     if (fp64) { if (D == 1) GB(double, 1); else if (D == 2) GB(double, 2); else GB(double, 3); }
     else { if (D == 1) GB(float, 1); else if (D == 2) GB(float, 2); else GB(float, 3); }
     PNH_LAUNCH_CHECK();
   }
-#undef YB
-#undef YBJ
 #undef GB
   return 0;
 }
