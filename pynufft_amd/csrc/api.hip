@@ -129,7 +129,8 @@ int build_pass(const pnh_plan* p, int a, bool inverse, bool src_image,
   const size_t esz = fp64 ? 16 : 8;
   const int n = f->n;
   f->lds_pad = 1;
-  const int64_t cap = (int64_t)(kLdsBudget / (2 * esz));  // complex elements per buffer
+  // complex elements per line buffer (two buffers plus 24 B of offsets per line)
+  const int64_t cap = (int64_t)((kLdsBudget - 16 * 24) / (2 * esz));
   if ((int64_t)(n + f->lds_pad) > cap) f->lds_pad = 0;
   if ((int64_t)(n + f->lds_pad) > cap) {
     pnh_set_error("FFT length %d exceeds the LDS-resident limit (%lld)", n, (long long)cap);
@@ -649,8 +650,7 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
                                       : (kind == BFFT_INV ? probs[i].inv[a] : probs[i].inv64[a]);
         const int64_t lines = f.n_outer * f.n_inner;
         nbk[i] = (lines + f.lines_per_block - 1) / f.lines_per_block;
-        const size_t esz = kind == BFFT_INV64 ? 16 : 8;
-        lds = std::max(lds, 2 * (size_t)f.lines_per_block * (f.n + f.lds_pad) * esz);
+        lds = std::max(lds, pnh_fft_lds_bytes(f, kind == BFFT_INV64));
         for (int q = 0; q < f.npass; ++q)
           pow2 &= (f.radix[q] == 2 || f.radix[q] == 4 || f.radix[q] == 8);
       }

@@ -41,3 +41,5 @@ struct FftDesc {
 int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
                    const void* tw, const int* skip, hipStream_t stream);
 int pnh_fft_factor(int n, int* radix, int* ns, int* npass);
+// LDS bytes of one workgroup: two line buffers plus per-line offsets.
+size_t pnh_fft_lds_bytes(const FftDesc& d, int fp64);

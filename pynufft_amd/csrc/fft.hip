@@ -133,8 +133,25 @@ __device__ __forceinline__ void dft_r(typename cplx<T>::t* v, int n,
   else dftg<T, R>(v, n, tw, sign);
 }
 
+// q = i / n and r = i % n for 0 <= i < 2^22 and 1 <= n, with rcp = 1.0f / n
+// (the float estimate is off by at most one; one correction step fixes it)
+__device__ __forceinline__ int fdivmod(int i, int n, float rcp, int& r) {
+  int q = (int)((float)i * rcp);
+  r = i - q * n;
+  if (r < 0) { --q; r += n; }
+  else if (r >= n) { ++q; r -= n; }
+  return q;
+}
+
+// i / n, i % n: shifts for powers of two, else the float reciprocal
+template <bool POW2>
+__device__ __forceinline__ int divmod_n(int i, int n, int log2n, float rcp, int& r) {
+  if (POW2) { r = i & (n - 1); return i >> log2n; }
+  return fdivmod(i, n, rcp, r);
+}
+
 // One Stockham radix-R pass over LB lines held in LDS (src -> dst).
-template <typename T, int R>
+template <typename T, int R, bool POW2>
 __device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
                                               typename cplx<T>::t* dst, int n,
                                               int ld, int LB, int ns,
@@ -144,10 +161,13 @@ __device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
   const int nb = n / R;
   const int twstep = n / (ns * R);
   const int total = LB * nb;
+  const int lg_nb = POW2 ? __builtin_ctz(nb) : 0;
+  const int lg_ns = POW2 ? __builtin_ctz(ns) : 0;
+  const float r_nb = 1.0f / (float)nb, r_ns = 1.0f / (float)ns;
   for (int idx = tid; idx < total; idx += nthr) {
-    const int l = idx / nb;
-    const int j = idx - l * nb;
-    const int k = j % ns;
+    int j, k;
+    const int l = divmod_n<POW2>(idx, nb, lg_nb, r_nb, j);
+    const int jq = divmod_n<POW2>(j, ns, lg_ns, r_ns, k);
     C v[R];
     const C* s = src + l * ld;
 #pragma unroll
@@ -161,7 +181,7 @@ __device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
       }
     }
     dft_r<T, R>(v, n, tw, sign);
-    C* dd = dst + l * ld + (j / ns) * ns * R + k;
+    C* dd = dst + l * ld + jq * ns * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) dd[r * ns] = v[r];
   }
@@ -178,9 +198,10 @@ __device__ __forceinline__ int64_t line_base(const FftSide& s, int64_t o0,
   return o0 * s.o_stride[0] + o1 * s.o_stride[1] + b * s.b_stride;
 }
 
-// POW2: every radix is 2, 4 or 8 (smaller register footprint).
-// in_ and out_ may alias (in-place passes): every input of the block is read
-// into LDS before the first store.
+// POW2: every radix is 2, 4 or 8 (smaller register footprint, shift/mask
+// indexing).  in_ and out_ may alias (in-place passes): every input of the
+// block is read into LDS before the first store.  LDS: two line buffers,
+// then per-line source/destination/apodization offsets.
 template <typename T, bool POW2>
 __device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void* out_,
                                          const void* __restrict__ tw_, int64_t blk,
@@ -189,32 +210,55 @@ __device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void
   C* buf0 = reinterpret_cast<C*>(smem_raw);
   const int n = d.n;
   const int ld = n + d.lds_pad;           // padded LDS line length
-  C* buf1 = buf0 + (size_t)d.lines_per_block * ld;
+  const int LB = d.lines_per_block;
+  C* buf1 = buf0 + (size_t)LB * ld;
+  int64_t* loff = reinterpret_cast<int64_t*>(buf1 + (size_t)LB * ld);  // [3][LB]
   const C* in = (const C*)in_;
   C* out = (C*)out_;
   const C* __restrict__ tw = (const C*)tw_;
 
-  const int LB = d.lines_per_block;
   const int64_t line0 = blk * LB;
   const int64_t n_lines = d.n_outer * d.n_inner;
   const int nthr = blockDim.x;
   const int tid = threadIdx.x;
+  const int nl = (int)(n_lines - line0 < LB ? n_lines - line0 : LB);  // lines present
+
+  if (tid < LB) {
+    const int64_t L = line0 + (tid < nl ? tid : 0);
+    const int64_t o = L / d.n_inner, b = L - o * d.n_inner;
+    const int64_t o0 = o / d.o_split, o1 = o - o0 * d.o_split;
+    loff[tid] = line_base(d.src, o0, o1, b);
+    loff[LB + tid] = line_base(d.dst, o0, o1, b);
+    loff[2 * LB + tid] = o0 * d.sn_o_stride[0] + o1 * d.sn_o_stride[1];
+  }
+  __syncthreads();
+
+  // element index -> (line l, position t); lines fastest when inner_major
+  const int lg_n = POW2 ? __builtin_ctz(n) : 0;
+  const int lg_lb = (LB & (LB - 1)) == 0 ? __builtin_ctz(LB) : -1;
+  const float r_n = 1.0f / (float)n, r_lb = 1.0f / (float)LB;
+  const float r_nout = 1.0f / (float)d.n_out;
+  auto split = [&](int idx, int len, float r_len, bool len_pow2, int lg_len, int& l, int& t) {
+    if (d.inner_major) {
+      if (lg_lb >= 0) { t = idx >> lg_lb; l = idx & (LB - 1); }
+      else t = fdivmod(idx, LB, r_lb, l);
+    } else {
+      if (len_pow2) { l = idx >> lg_len; t = idx & (len - 1); }
+      else l = fdivmod(idx, len, r_len, t);
+    }
+  };
 
   // ---- load (with zero padding and optional apodization) ----
   const int tot_in = LB * n;
+  const int64_t st = d.src.t_stride;
   for (int idx = tid; idx < tot_in; idx += nthr) {
     int l, t;
-    if (d.inner_major) { t = idx / LB; l = idx - t * LB; }
-    else { l = idx / n; t = idx - l * n; }
-    int64_t L = line0 + l;
+    split(idx, n, r_n, POW2, lg_n, l, t);
     C v = mk<T>((T)0, (T)0);
-    if (L < n_lines && t < d.n_in) {
-      int64_t o = L / d.n_inner, b = L - o * d.n_inner;
-      int64_t o0 = o / d.o_split, o1 = o - o0 * d.o_split;
-      v = in[line_base(d.src, o0, o1, b) + (int64_t)t * d.src.t_stride];
+    if (l < nl && t < d.n_in) {
+      v = in[loff[l] + (int64_t)t * st];
       if (d.sn_side == 1) {
-        T s = ld_sn<T>(d.sn, o0 * d.sn_o_stride[0] + o1 * d.sn_o_stride[1] +
-                                 (int64_t)t * d.sn_t_stride);
+        const T s = ld_sn<T>(d.sn, loff[2 * LB + l] + (int64_t)t * d.sn_t_stride);
         v.x *= s; v.y *= s;
       }
     }
@@ -230,41 +274,39 @@ __device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void
     const int R = d.radix[p];
     const int ns = d.ns[p];
     if (POW2) {
-      if (R == 8) stockham_pass<T, 8>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
-      else if (R == 4) stockham_pass<T, 4>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
-      else stockham_pass<T, 2>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+      if (R == 8) stockham_pass<T, 8, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+      else if (R == 4) stockham_pass<T, 4, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
+      else stockham_pass<T, 2, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
     } else switch (R) {
-      case 2: stockham_pass<T, 2>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 3: stockham_pass<T, 3>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 4: stockham_pass<T, 4>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 5: stockham_pass<T, 5>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 7: stockham_pass<T, 7>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 8: stockham_pass<T, 8>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 11: stockham_pass<T, 11>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      default: stockham_pass<T, 13>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 2: stockham_pass<T, 2, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 3: stockham_pass<T, 3, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 4: stockham_pass<T, 4, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 5: stockham_pass<T, 5, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 7: stockham_pass<T, 7, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 8: stockham_pass<T, 8, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      case 11: stockham_pass<T, 11, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
+      default: stockham_pass<T, 13, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
     }
     __syncthreads();
     C* tmp = src; src = dst; dst = tmp;
   }
 
   // ---- store (crop, scale, optional apodization) ----
-  const int tot_out = LB * d.n_out;
+  const int n_out = d.n_out;
+  const bool nout_pow2 = (n_out & (n_out - 1)) == 0;
+  const int lg_nout = nout_pow2 ? __builtin_ctz(n_out) : 0;
+  const int tot_out = LB * n_out;
   const T scale = (T)d.scale;
+  const int64_t dt = d.dst.t_stride;
   for (int idx = tid; idx < tot_out; idx += nthr) {
     int l, t;
-    if (d.inner_major) { t = idx / LB; l = idx - t * LB; }
-    else { l = idx / d.n_out; t = idx - l * d.n_out; }
-    int64_t L = line0 + l;
-    if (L >= n_lines) continue;
-    int64_t o = L / d.n_inner, b = L - o * d.n_inner;
-    int64_t o0 = o / d.o_split, o1 = o - o0 * d.o_split;
+    split(idx, n_out, r_nout, nout_pow2, lg_nout, l, t);
+    if (l >= nl) continue;
     C v = src[l * ld + t];
     T s = scale;
-    if (d.sn_side == 2)
-      s *= ld_sn<T>(d.sn, o0 * d.sn_o_stride[0] + o1 * d.sn_o_stride[1] +
-                              (int64_t)t * d.sn_t_stride);
+    if (d.sn_side == 2) s *= ld_sn<T>(d.sn, loff[2 * LB + l] + (int64_t)t * d.sn_t_stride);
     v.x *= s; v.y *= s;
-    out[line_base(d.dst, o0, o1, b) + (int64_t)t * d.dst.t_stride] = v;
+    out[loff[LB + l] + (int64_t)t * dt] = v;
   }
 }
 
@@ -285,7 +327,7 @@ int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
   if (n_lines <= 0) return 0;
   const int64_t nblk = (n_lines + d.lines_per_block - 1) / d.lines_per_block;
   const size_t esz = fp64 ? 16 : 8;
-  const size_t lds = 2 * (size_t)d.lines_per_block * (d.n + d.lds_pad) * esz;
+  const size_t lds = pnh_fft_lds_bytes(d, fp64);
   dim3 grid((unsigned)nblk), block(d.threads);
   bool pow2 = true;
   for (int p = 0; p < d.npass; ++p)
@@ -307,6 +349,12 @@ int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
   }
   PNH_LAUNCH_CHECK();
   return 0;
+}
+
+size_t pnh_fft_lds_bytes(const FftDesc& d, int fp64) {
+  const size_t esz = fp64 ? 16 : 8;
+  return 2 * (size_t)d.lines_per_block * (d.n + d.lds_pad) * esz +
+         3 * (size_t)d.lines_per_block * sizeof(int64_t);
 }
 
 // Factor n into radices (8,4,2 first, then 3,5, then generic odd primes).

@@ -206,10 +206,11 @@ __device__ __forceinline__ void lds_add(long long* p, long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// real part at cell[0], imaginary part one plane further
 template <typename C>
-__device__ __forceinline__ void lds_add_c(long long* cell, C v, double scale) {
+__device__ __forceinline__ void lds_add_c(long long* cell, int plane, C v, double scale) {
   lds_add(cell, to_fixed((double)v.x, scale));
-  lds_add(cell + 1, to_fixed((double)v.y, scale));
+  lds_add(cell + plane, to_fixed((double)v.y, scale));
 }
 
 // Complex product with contraction allowed (fused multiply-adds).
@@ -290,7 +291,8 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   typedef typename cplx<T>::t C;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int pc = (int)b.pad_cells;
-  double* red = reinterpret_cast<double*>(acc + (int64_t)pc * NBC * 2);
+  const int plane = pc * NBC;            // real plane, then imaginary plane
+  double* red = reinterpret_cast<double*>(acc + (int64_t)plane * 2);
   const C* __restrict__ y = (const C*)y_;
   const int nb = a.nb;
   const int r_begin = it.y, r_end = it.y + it.z;
@@ -361,18 +363,18 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
           idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
                   wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
         const C w = cconj(val[i0 + g * gs]);
-        long long* cell = acc + (int64_t)idx * NBC * 2;
+        long long* cell = acc + idx * NBC;
 #pragma unroll
-        for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+        for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
         if (D == 3 && ++j1 == J1) { j1 = 0; ++j0; }
       }
       continue;
     }
     const C ul = cconj(u[a.u_off[D - 1] + (int64_t)r * JL + jl]);
     if (D == 1) {
-      long long* cell = acc + (int64_t)xl * NBC * 2;
+      long long* cell = acc + xl * NBC;
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(ul, yv[c]), scale);
+      for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(ul, yv[c]), scale);
       continue;
     }
     // NBC == 1: fold the sample value into the last-axis factor
@@ -392,11 +394,11 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
       for (int j0 = 0; j0 < JN; ++j0) {
         if (JE > 0 || j0 < J0) {
           const C w = cmulf(u0c[j0], wl);
-          long long* cell = acc + (int64_t)(row0[j0] + xl) * NBC * 2;
-          if (NBC == 1) lds_add_c(cell, w, scale);
+          long long* cell = acc + (row0[j0] + xl) * NBC;
+          if (NBC == 1) lds_add_c(cell, plane, w, scale);
           else {
 #pragma unroll
-            for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+            for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
           }
         }
       }
@@ -421,11 +423,11 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
         for (int j1 = 0; j1 < JN; ++j1) {
           if (JE > 0 || j1 < J1) {
             const C w = cmulf(u1c[j1], a0);
-            long long* cell = acc + (int64_t)(row0[j0] + off1[j1]) * NBC * 2;
-            if (NBC == 1) lds_add_c(cell, w, scale);
+            long long* cell = acc + (row0[j0] + off1[j1]) * NBC;
+            if (NBC == 1) lds_add_c(cell, plane, w, scale);
             else {
 #pragma unroll
-              for (int c = 0; c < NBC; ++c) lds_add_c(cell + 2 * c, cmulf(w, yv[c]), scale);
+              for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
             }
           }
         }
@@ -436,15 +438,10 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   // partial slot, scaled back
   C* __restrict__ part = (C*)part_ + (int64_t)it.w * pc * NBC;
   const T nan = (T)__builtin_nan("");
-  const longlong2* acc2 = reinterpret_cast<const longlong2*>(acc);
-  for (int i = tid; i < pc * NBC; i += nthr) {
+  for (int i = tid; i < plane; i += nthr) {
     C v;
-    if (poison) {
-      v = mk<T>(nan, nan);
-    } else {
-      const longlong2 q = acc2[i];
-      v = mk<T>((T)((double)q.x * unscale), (T)((double)q.y * unscale));
-    }
+    if (poison) v = mk<T>(nan, nan);
+    else v = mk<T>((T)((double)acc[i] * unscale), (T)((double)acc[plane + i] * unscale));
     part[i] = v;
   }
 }
