@@ -201,8 +201,8 @@ def main():
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the batched uneven-problem CG comparison')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
-                    help='after timing, copy this many MiB device-to-device '
-                         '(a known-size kernel to calibrate HBM byte counters)')
+                    help='after timing, copy this many MiB with an elementwise '
+                         'kernel (known bytes, to calibrate HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU: the operator does not shard across '
@@ -262,12 +262,13 @@ def main():
             traffic = json.load(f).get('%s_%s_%s' % (args.workload, fmt, kname))
 
     if args.copy_probe_mib:
+        # a plain elementwise copy (reads and writes copy_probe_mib MiB each)
         src = torch.empty(args.copy_probe_mib << 18, dtype=torch.float32, device=dev)
         src.fill_(1.0)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 
