@@ -128,12 +128,13 @@ int build_pass(const pnh_plan* p, int a, bool inverse, bool src_image,
   if (pnh_fft_factor(f->n, f->radix, f->ns, &f->npass)) return -2;
   const size_t esz = fp64 ? 16 : 8;
   const int n = f->n;
-  f->lds_pad = 1;
-  // complex elements per line buffer (two buffers plus 24 B of offsets per line)
-  const int64_t cap = (int64_t)((kLdsBudget - 16 * 24) / (2 * esz));
-  if ((int64_t)(n + f->lds_pad) > cap) f->lds_pad = 0;
-  if ((int64_t)(n + f->lds_pad) > cap) {
-    pnh_set_error("FFT length %d exceeds the LDS-resident limit (%lld)", n, (long long)cap);
+  f->lds_pad = 0;
+  // LDS per line: line buffers (none for one pass, one for two, two for
+  // more) plus 24 B of per-line offsets
+  const int nbuf = f->npass >= 3 ? 2 : (f->npass == 2 ? 1 : 0);
+  const size_t line_bytes = (size_t)nbuf * pnh_fft_lds_line(n) * esz + 24;
+  if (line_bytes > kLdsBudget) {
+    pnh_set_error("FFT length %d exceeds the LDS-resident limit", n);
     return -2;
   }
   const int64_t n_lines = n_outer * n_inner;
@@ -141,18 +142,19 @@ int build_pass(const pnh_plan* p, int a, bool inverse, bool src_image,
   if (n_inner >= 8) {
     f->inner_major = 1;
     LB = 16;
-    while (LB > 8 && (int64_t)LB * (n + f->lds_pad) > cap) LB >>= 1;
-    while (LB > 1 && (int64_t)LB * (n + f->lds_pad) > cap) LB >>= 1;
+    while (LB > 1 && (size_t)LB * line_bytes > kLdsBudget) LB >>= 1;
   } else {
     f->inner_major = 0;
-    LB = (int)(cap / (n + f->lds_pad));
+    LB = (int)(kLdsBudget / line_bytes);
     if (LB > 16) LB = 16;
     if (LB < 1) LB = 1;
     // keep at least ~1024 workgroups when the lines allow it
     while (LB > 1 && (n_lines + LB - 1) / LB < 1024) LB >>= 1;
   }
   f->lines_per_block = LB;
-  int64_t work = (int64_t)LB * n / 2;
+  int rmin = f->radix[0];
+  for (int q = 1; q < f->npass; ++q) rmin = f->radix[q] < rmin ? f->radix[q] : rmin;
+  const int64_t work = (int64_t)LB * n / rmin;     // butterflies of the widest pass
   f->threads = work >= 256 ? 256 : (work >= 128 ? 128 : 64);
   return 0;
 }

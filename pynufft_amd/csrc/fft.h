@@ -33,7 +33,7 @@ struct FftDesc {
   int radix[PNH_FFT_MAX_PASS];
   int ns[PNH_FFT_MAX_PASS];
   int lines_per_block;
-  int lds_pad;
+  int lds_pad;           // unused (line padding is fixed by pnh_fft_lds_line)
   int threads;
   int inner_major;      // 1: threads walk lines fastest (strided axis)
 };
@@ -41,5 +41,8 @@ struct FftDesc {
 int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
                    const void* tw, const int* skip, hipStream_t stream);
 int pnh_fft_factor(int n, int* radix, int* ns, int* npass);
-// LDS bytes of one workgroup: two line buffers plus per-line offsets.
+// LDS line length in elements: one pad element per 16 (bank spreading).
+__host__ __device__ inline int pnh_fft_lds_line(int n) { return n + (n >> 4) + 1; }
+// LDS bytes of one workgroup: line buffers (none for a single pass, one for
+// two passes, two for more) plus per-line offsets.
 size_t pnh_fft_lds_bytes(const FftDesc& d, int fp64);

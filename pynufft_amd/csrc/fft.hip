@@ -66,6 +66,37 @@ __device__ __forceinline__ void dft8(typename cplx<T>::t* v, int sign) {
   v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
 }
 
+// 16-point DFT as 4 x 4: DFTs over n1 of x[4 n1 + n2], twiddles
+// W16^(n2 k1), DFTs over n2; W = exp(sign 2 pi i / 16).
+template <typename T>
+__device__ __forceinline__ void dft16(typename cplx<T>::t* v, int sign) {
+  typedef typename cplx<T>::t C;
+  const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173;
+  const T h = (T)0.70710678118654752440;
+  const T sg = (T)sign;
+  C a[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    C t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+    dft4<T>(t, sign);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+  }
+  // W16^e for e = n2 * k1 in {1, 2, 3, 4, 6, 9}
+  const C w1 = mk<T>(c1, sg * s1), w2 = mk<T>(h, sg * h), w3 = mk<T>(s1, sg * c1);
+  const C w4 = mk<T>((T)0, sg), w6 = mk<T>(-h, sg * h), w9 = mk<T>(-c1, -sg * s1);
+  a[1][1] = cmul(a[1][1], w1); a[1][2] = cmul(a[1][2], w2); a[1][3] = cmul(a[1][3], w3);
+  a[2][1] = cmul(a[2][1], w2); a[2][2] = cmul(a[2][2], w4); a[2][3] = cmul(a[2][3], w6);
+  a[3][1] = cmul(a[3][1], w3); a[3][2] = cmul(a[3][2], w6); a[3][3] = cmul(a[3][3], w9);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    C t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+    dft4<T>(t, sign);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void dft3(typename cplx<T>::t* v, int sign) {
   typedef typename cplx<T>::t C;
@@ -130,6 +161,7 @@ __device__ __forceinline__ void dft_r(typename cplx<T>::t* v, int n,
   else if (R == 4) dft4<T>(v, sign);
   else if (R == 5) dft5<T>(v, sign);
   else if (R == 8) dft8<T>(v, sign);
+  else if (R == 16) dft16<T>(v, sign);
   else dftg<T, R>(v, n, tw, sign);
 }
 
@@ -150,28 +182,36 @@ __device__ __forceinline__ int divmod_n(int i, int n, int log2n, float rcp, int&
   return fdivmod(i, n, rcp, r);
 }
 
-// One Stockham radix-R pass over LB lines held in LDS (src -> dst).
-template <typename T, int R, bool POW2>
-__device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
-                                              typename cplx<T>::t* dst, int n,
-                                              int ld, int LB, int ns,
-                                              const typename cplx<T>::t* tw,
-                                              int sign, int tid, int nthr) {
+// One Stockham radix-R pass over LB lines.  Butterfly (l, j), j < n/R,
+// reads in(l, j + r n/R) for r < R, applies the twiddles of a stage after
+// ns points, and writes out(l, (j / ns) ns R + j % ns + r ns).  With
+// lines_fast, consecutive threads take consecutive lines (coalesced global
+// access along a strided axis); else consecutive j.
+template <typename T, int R, bool POW2, typename LD, typename ST>
+__device__ __forceinline__ void stockham_pass(int n, int LB, int ns, bool lines_fast,
+                                              const typename cplx<T>::t* tw, int sign,
+                                              int tid, int nthr, LD load, ST store) {
   typedef typename cplx<T>::t C;
   const int nb = n / R;
   const int twstep = n / (ns * R);
   const int total = LB * nb;
   const int lg_nb = POW2 ? __builtin_ctz(nb) : 0;
   const int lg_ns = POW2 ? __builtin_ctz(ns) : 0;
-  const float r_nb = 1.0f / (float)nb, r_ns = 1.0f / (float)ns;
+  const bool lb_pow2 = (LB & (LB - 1)) == 0;
+  const int lg_lb = lb_pow2 ? __builtin_ctz(LB) : 0;
+  const float r_nb = 1.0f / (float)nb, r_ns = 1.0f / (float)ns, r_lb = 1.0f / (float)LB;
   for (int idx = tid; idx < total; idx += nthr) {
-    int j, k;
-    const int l = divmod_n<POW2>(idx, nb, lg_nb, r_nb, j);
+    int l, j, k;
+    if (lines_fast) {
+      if (lb_pow2) { l = idx & (LB - 1); j = idx >> lg_lb; }
+      else j = fdivmod(idx, LB, r_lb, l);
+    } else {
+      l = divmod_n<POW2>(idx, nb, lg_nb, r_nb, j);
+    }
     const int jq = divmod_n<POW2>(j, ns, lg_ns, r_ns, k);
     C v[R];
-    const C* s = src + l * ld;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = s[j + r * nb];
+    for (int r = 0; r < R; ++r) v[r] = load(l, j + r * nb);
     if (ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
@@ -181,9 +221,33 @@ __device__ __forceinline__ void stockham_pass(const typename cplx<T>::t* src,
       }
     }
     dft_r<T, R>(v, n, tw, sign);
-    C* dd = dst + l * ld + jq * ns * R + k;
+    const int o = jq * ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) dd[r * ns] = v[r];
+    for (int r = 0; r < R; ++r) store(l, o + r * ns, v[r]);
+  }
+}
+
+template <typename T, bool POW2, typename LD, typename ST>
+__device__ __forceinline__ void run_pass(int R, int n, int LB, int ns, bool lines_fast,
+                                         const typename cplx<T>::t* tw, int sign, int tid,
+                                         int nthr, LD load, ST store) {
+  if (POW2) {
+    if (R == 16) stockham_pass<T, 16, true>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store);
+    else if (R == 8) stockham_pass<T, 8, true>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store);
+    else if (R == 4) stockham_pass<T, 4, true>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store);
+    else stockham_pass<T, 2, true>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store);
+    return;
+  }
+  switch (R) {
+    case 2: stockham_pass<T, 2, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 3: stockham_pass<T, 3, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 4: stockham_pass<T, 4, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 5: stockham_pass<T, 5, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 7: stockham_pass<T, 7, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 8: stockham_pass<T, 8, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 11: stockham_pass<T, 11, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    case 13: stockham_pass<T, 13, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
+    default: stockham_pass<T, 16, false>(n, LB, ns, lines_fast, tw, sign, tid, nthr, load, store); break;
   }
 }
 
@@ -198,21 +262,27 @@ __device__ __forceinline__ int64_t line_base(const FftSide& s, int64_t o0,
   return o0 * s.o_stride[0] + o1 * s.o_stride[1] + b * s.b_stride;
 }
 
-// POW2: every radix is 2, 4 or 8 (smaller register footprint, shift/mask
-// indexing).  in_ and out_ may alias (in-place passes): every input of the
-// block is read into LDS before the first store.  LDS: two line buffers,
+// POW2: every radix is 2, 4, 8 or 16 (shift/mask indexing).  The first
+// pass reads its inputs from global memory (zero padding and load-side
+// apodization fused) and the last pass writes its outputs there (crop,
+// scale and store-side apodization fused); the passes in between go through
+// LDS.  in_ and out_ may alias (in-place passes): a workgroup reads all its
+// lines before its first store, and other workgroups own other lines.
+// LDS: one line buffer (two with three or more passes), lines of
+// n + n/16 + 1 elements (one pad element per 16 against bank conflicts),
 // then per-line source/destination/apodization offsets.
 template <typename T, bool POW2>
 __device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void* out_,
                                          const void* __restrict__ tw_, int64_t blk,
                                          unsigned char* smem_raw) {
   typedef typename cplx<T>::t C;
-  C* buf0 = reinterpret_cast<C*>(smem_raw);
   const int n = d.n;
-  const int ld = n + d.lds_pad;           // padded LDS line length
   const int LB = d.lines_per_block;
+  const int ld = pnh_fft_lds_line(n);
+  const int nbuf = d.npass >= 3 ? 2 : (d.npass == 2 ? 1 : 0);
+  C* buf0 = reinterpret_cast<C*>(smem_raw);
   C* buf1 = buf0 + (size_t)LB * ld;
-  int64_t* loff = reinterpret_cast<int64_t*>(buf1 + (size_t)LB * ld);  // [3][LB]
+  int64_t* loff = reinterpret_cast<int64_t*>(buf0 + (size_t)nbuf * LB * ld);  // [3][LB]
   const C* in = (const C*)in_;
   C* out = (C*)out_;
   const C* __restrict__ tw = (const C*)tw_;
@@ -233,81 +303,52 @@ __device__ __forceinline__ void fft_body(const FftDesc& d, const void* in_, void
   }
   __syncthreads();
 
-  // element index -> (line l, position t); lines fastest when inner_major
-  const int lg_n = POW2 ? __builtin_ctz(n) : 0;
-  const int lg_lb = (LB & (LB - 1)) == 0 ? __builtin_ctz(LB) : -1;
-  const float r_n = 1.0f / (float)n, r_lb = 1.0f / (float)LB;
-  const float r_nout = 1.0f / (float)d.n_out;
-  auto split = [&](int idx, int len, float r_len, bool len_pow2, int lg_len, int& l, int& t) {
-    if (d.inner_major) {
-      if (lg_lb >= 0) { t = idx >> lg_lb; l = idx & (LB - 1); }
-      else t = fdivmod(idx, LB, r_lb, l);
-    } else {
-      if (len_pow2) { l = idx >> lg_len; t = idx & (len - 1); }
-      else l = fdivmod(idx, len, r_len, t);
-    }
-  };
-
-  // ---- load (with zero padding and optional apodization) ----
-  const int tot_in = LB * n;
-  const int64_t st = d.src.t_stride;
-  for (int idx = tid; idx < tot_in; idx += nthr) {
-    int l, t;
-    split(idx, n, r_n, POW2, lg_n, l, t);
+  const int64_t st = d.src.t_stride, dt = d.dst.t_stride;
+  const int n_in = d.n_in, n_out = d.n_out;
+  const int sn_side = d.sn_side;
+  const T scale = (T)d.scale;
+  auto gload = [&](int l, int t) -> C {
     C v = mk<T>((T)0, (T)0);
-    if (l < nl && t < d.n_in) {
+    if (l < nl && t < n_in) {
       v = in[loff[l] + (int64_t)t * st];
-      if (d.sn_side == 1) {
+      if (sn_side == 1) {
         const T s = ld_sn<T>(d.sn, loff[2 * LB + l] + (int64_t)t * d.sn_t_stride);
         v.x *= s; v.y *= s;
       }
     }
-    buf0[l * ld + t] = v;
-  }
-  __syncthreads();
+    return v;
+  };
+  auto gstore = [&](int l, int t, C v) {
+    if (l < nl && t < n_out) {
+      T s = scale;
+      if (sn_side == 2) s *= ld_sn<T>(d.sn, loff[2 * LB + l] + (int64_t)t * d.sn_t_stride);
+      v.x *= s; v.y *= s;
+      out[loff[LB + l] + (int64_t)t * dt] = v;
+    }
+  };
+  auto at = [&](C* b, int l, int t) -> C& { return b[l * ld + t + (t >> 4)]; };
+  const int sign = d.sign;
+  const bool fast = d.inner_major != 0;
 
-  // ---- Stockham passes in LDS ----
+  if (d.npass == 1) {
+    run_pass<T, POW2>(d.radix[0], n, LB, 1, fast, tw, sign, tid, nthr, gload, gstore);
+    return;
+  }
+  run_pass<T, POW2>(d.radix[0], n, LB, d.ns[0], fast, tw, sign, tid, nthr, gload,
+                    [&](int l, int t, C v) { at(buf0, l, t) = v; });
+  __syncthreads();
   C* src = buf0;
   C* dst = buf1;
-  const int sign = d.sign;
-  for (int p = 0; p < d.npass; ++p) {
-    const int R = d.radix[p];
-    const int ns = d.ns[p];
-    if (POW2) {
-      if (R == 8) stockham_pass<T, 8, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
-      else if (R == 4) stockham_pass<T, 4, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
-      else stockham_pass<T, 2, true>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr);
-    } else switch (R) {
-      case 2: stockham_pass<T, 2, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 3: stockham_pass<T, 3, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 4: stockham_pass<T, 4, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 5: stockham_pass<T, 5, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 7: stockham_pass<T, 7, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 8: stockham_pass<T, 8, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      case 11: stockham_pass<T, 11, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-      default: stockham_pass<T, 13, false>(src, dst, n, ld, LB, ns, tw, sign, tid, nthr); break;
-    }
+  for (int p = 1; p + 1 < d.npass; ++p) {
+    run_pass<T, POW2>(d.radix[p], n, LB, d.ns[p], false, tw, sign, tid, nthr,
+                      [&](int l, int t) -> C { return at(src, l, t); },
+                      [&](int l, int t, C v) { at(dst, l, t) = v; });
     __syncthreads();
     C* tmp = src; src = dst; dst = tmp;
   }
-
-  // ---- store (crop, scale, optional apodization) ----
-  const int n_out = d.n_out;
-  const bool nout_pow2 = (n_out & (n_out - 1)) == 0;
-  const int lg_nout = nout_pow2 ? __builtin_ctz(n_out) : 0;
-  const int tot_out = LB * n_out;
-  const T scale = (T)d.scale;
-  const int64_t dt = d.dst.t_stride;
-  for (int idx = tid; idx < tot_out; idx += nthr) {
-    int l, t;
-    split(idx, n_out, r_nout, nout_pow2, lg_nout, l, t);
-    if (l >= nl) continue;
-    C v = src[l * ld + t];
-    T s = scale;
-    if (d.sn_side == 2) s *= ld_sn<T>(d.sn, loff[2 * LB + l] + (int64_t)t * d.sn_t_stride);
-    v.x *= s; v.y *= s;
-    out[loff[LB + l] + (int64_t)t * dt] = v;
-  }
+  const int pl = d.npass - 1;
+  run_pass<T, POW2>(d.radix[pl], n, LB, d.ns[pl], fast, tw, sign, tid, nthr,
+                    [&](int l, int t) -> C { return at(src, l, t); }, gstore);
 }
 
 template <typename T, bool POW2>
@@ -331,7 +372,7 @@ int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
   dim3 grid((unsigned)nblk), block(d.threads);
   bool pow2 = true;
   for (int p = 0; p < d.npass; ++p)
-    pow2 = pow2 && (d.radix[p] == 2 || d.radix[p] == 4 || d.radix[p] == 8);
+    pow2 = pow2 && (d.radix[p] == 2 || d.radix[p] == 4 || d.radix[p] == 8 || d.radix[p] == 16);
   if (fp64) {
     if (pow2)
       hipLaunchKernelGGL((fft_lines_kernel<double, true>), grid, block, lds, stream, d,
@@ -353,17 +394,20 @@ int pnh_fft_launch(const FftDesc& d, int fp64, const void* in, void* out,
 
 size_t pnh_fft_lds_bytes(const FftDesc& d, int fp64) {
   const size_t esz = fp64 ? 16 : 8;
-  return 2 * (size_t)d.lines_per_block * (d.n + d.lds_pad) * esz +
+  const int nbuf = d.npass >= 3 ? 2 : (d.npass == 2 ? 1 : 0);
+  return (size_t)nbuf * d.lines_per_block * pnh_fft_lds_line(d.n) * esz +
          3 * (size_t)d.lines_per_block * sizeof(int64_t);
 }
 
-// Factor n into radices (8,4,2 first, then 3,5, then generic odd primes).
+// Factor n into radices: 16 first, then 8, 4, 2, then 3, 5, then the other
+// small odd primes.
 int pnh_fft_factor(int n, int* radix, int* ns, int* npass) {
   int m = n, cnt = 0;
   int prod = 1;
   while (m > 1) {
     int R;
-    if (m % 8 == 0) R = 8;
+    if (m % 16 == 0) R = 16;
+    else if (m % 8 == 0) R = 8;
     else if (m % 4 == 0) R = 4;
     else if (m % 2 == 0) R = 2;
     else if (m % 3 == 0) R = 3;

@@ -308,3 +308,61 @@ def test_samples_on_grid_edges():
     A = oracle.OracleNUFFT()
     A.plan(om, Nd, Kd, Jd)
     assert rel(emulate_y2k(st, Kd, Jd, y), A.y2k(y)) < 1e-14
+
+
+def _factor(n):
+    """The device FFT's radix schedule (16 first, then 8, 4, 2, 3, 5, 7, 11, 13)."""
+    out = []
+    m = n
+    while m > 1:
+        for R in (16, 8, 4, 2, 3, 5, 7, 11, 13):
+            if m % R == 0:
+                out.append(R)
+                m //= R
+                break
+        else:
+            raise ValueError(n)
+    return out
+
+
+def _dft16(v, sign):
+    """The kernel's 16-point DFT: 4-point DFTs over n1 of x[4 n1 + n2],
+    twiddles W16^(n2 k1), 4-point DFTs over n2."""
+    W4 = np.exp(sign * 2j * np.pi * np.outer(np.arange(4), np.arange(4)) / 4)
+    a = np.array([W4 @ v[n2::4] for n2 in range(4)])            # a[n2][k1]
+    a = a * np.exp(sign * 2j * np.pi * np.outer(np.arange(4), np.arange(4)) / 16)
+    out = np.empty(16, complex)
+    for k1 in range(4):
+        out[k1::4] = W4 @ a[:, k1]
+    return out
+
+
+@pytest.mark.parametrize('n', [16, 64, 128, 256, 384, 512, 1000, 4096, 13 * 16])
+@pytest.mark.parametrize('sign', [-1, 1])
+def test_fft_stockham_schedule(n, sign):
+    """The device FFT's Stockham passes (butterfly (j) reads j + r n/R, twiddle
+    w^(r k n/(ns R)) with k = j % ns, writes (j // ns) ns R + k + r ns)
+    reproduce numpy's FFT for every radix schedule it uses."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    tw = np.exp(-2j * np.pi * np.arange(n) / n)
+    v = x.copy()
+    ns = 1
+    for R in _factor(n):
+        nb = n // R
+        out = np.empty(n, complex)
+        for j in range(nb):
+            k = j % ns
+            e = v[j + np.arange(R) * nb].copy()
+            if ns > 1:
+                w = tw[np.arange(R) * k * (n // (ns * R))]
+                e = e * (np.conj(w) if sign > 0 else w)
+            if R == 16:
+                e = _dft16(e, sign)
+            else:
+                e = np.exp(sign * 2j * np.pi * np.outer(np.arange(R), np.arange(R)) / R) @ e
+            out[(j // ns) * ns * R + k + np.arange(R) * ns] = e
+        v = out
+        ns *= R
+    ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+    assert np.allclose(v, ref, rtol=1e-12, atol=1e-10 * np.abs(ref).max())
