@@ -90,6 +90,7 @@ typedef struct pnh_plan_desc {
   const int32_t* tile_slots; /* [n_tiles][2]: first partial slot, slot count; the
                                 slots of one tile are consecutive, in row order */
   int64_t n_slots;           /* partial slots (one per item) */
+  int32_t max_tile_slots;    /* largest slot count of any tile (sizes the gather) */
   int32_t threads;           /* gridding workgroup size: multiple of 64 in
                                 64..1024, or 0 to let the library choose */
 } pnh_plan_desc;

@@ -47,6 +47,7 @@ class PlanDesc(ctypes.Structure):
         ('n_tiles', ctypes.c_int64),
         ('tile_slots', _vp),
         ('n_slots', ctypes.c_int64),
+        ('max_tile_slots', ctypes.c_int32),
         ('threads', ctypes.c_int32),
     ]
 

@@ -289,6 +289,8 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   REQUIRE(d.n_items >= 0 && (d.n_items == 0 || d.items), "gridding work items are required");
   REQUIRE(d.n_slots >= d.n_items, "n_slots below n_items");
   REQUIRE(d.M == 0 || (d.u_bound > 0.0 && d.u_bound < 1e300), "u_bound must be positive");
+  REQUIRE(d.max_tile_slots >= 0, "max_tile_slots must be >= 0");
+  p->bins.max_tile_slots = d.max_tile_slots;
   p->bins.u_bound = d.u_bound;
   p->bins.n_items = d.n_items;
   p->bins.items = d.items;
@@ -635,9 +637,10 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
 
   Blob blob;
   const size_t off_probs = blob.add(probs.data(), probs.size() * sizeof(BatchProblem));
-  // gridding gather: one block per tile
+  // gridding gather: pnh_gather_parts blocks per tile
   std::vector<int64_t> nb_tiles(n);
-  for (int i = 0; i < n; ++i) nb_tiles[i] = plans[i].bins.n_tiles;
+  for (int i = 0; i < n; ++i)
+    nb_tiles[i] = plans[i].bins.n_tiles * pnh_gather_parts(plans[i].bins);
   size_t og_tab, og_first;
   add_table(blob, nb_tiles, &og_tab, &og_first, &b->gather_t.n_blocks);
   // FFT passes per kind and axis

@@ -30,10 +30,21 @@ struct BinArgs {
   int64_t n_tiles;             // prod(tgrid)
   double u_bound;              // bound of |coefficient| over all rows and offsets
   int threads;                 // workgroup size of the gridding kernel
+  int max_tile_slots;          // largest slot count of any tile
   int64_t n_items;
   const int* items;            // [n_items][4] tile id, first row, rows, slot
   const int* tile_slots;       // [n_tiles][2] first slot, slot count
 };
+
+// Workgroups per tile in the gridding gather pass: one, or one per 256
+// cells when some tile has several slots (so that the slots of the heavy
+// tiles of a dense trajectory region are not summed by one workgroup).
+__host__ __device__ inline int pnh_gather_parts(const BinArgs& b) {
+  if (b.max_tile_slots <= 1) return 1;
+  int cells = 1;
+  for (int d = 0; d < PNH_MAXD; ++d) cells *= b.tile[d] > 0 ? b.tile[d] : 1;
+  return (cells + 255) / 256;
+}
 
 // Lanes per sample used by k2y (and by the CSR table layout): one lane per
 // support offset on the last axis, rounded to a divisor-friendly width.

@@ -632,6 +632,7 @@ k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
 // contiguously and every slot cell is read exactly once over the launch.
 template <typename T, int D, int NBC>
 __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& b, int tid,
+                                            int first, int count,
                                             const typename cplx<T>::t* __restrict__ part,
                                             typename cplx<T>::t* __restrict__ grid, int c0) {
   typedef typename cplx<T>::t C;
@@ -671,7 +672,8 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
   const int E1 = D > 1 ? E[1] : 1, E2 = D > 2 ? E[2] : 1;
   const int P1 = D > 1 ? b.pad[1] : 1, P2 = D > 2 ? b.pad[2] : 1;
   const int n = E[0] * E1 * E2;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  const int end = first + count < n ? first + count : n;
+  for (int i = first + threadIdx.x; i < end; i += blockDim.x) {
     int l[PNH_MAXD];
     {
       int rest = i;
@@ -707,13 +709,17 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
   }
 }
 
+// blockIdx.x = tile * parts + part: one workgroup per tile, or (parts > 1)
+// one per 256 consecutive cells of a tile.
 template <typename T, int D, int NBC>
 __global__ void __launch_bounds__(256)
-y2k_gather_kernel(InterpArgs a, BinArgs b, const void* __restrict__ part_,
+y2k_gather_kernel(InterpArgs a, BinArgs b, int parts, const void* __restrict__ part_,
                   void* __restrict__ grid_, int c0, const int* __restrict__ skip) {
   typedef typename cplx<T>::t C;
   if (skip && *skip) return;
-  gather_tile<T, D, NBC>(a, b, blockIdx.x, (const C*)part_, (C*)grid_, c0);
+  const int t = blockIdx.x / parts, q = blockIdx.x - t * parts;
+  const int span = parts == 1 ? (1 << 30) : 256;
+  gather_tile<T, D, NBC>(a, b, t, q * 256, span, (const C*)part_, (C*)grid_, c0);
 }
 
 size_t y2k_lds_bytes(int64_t pad_cells, int nbc) {
@@ -824,10 +830,11 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
       else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, lds, skip, s);
       if (rc) return rc;
     }
-    dim3 gg((unsigned)b.n_tiles);
-    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
-    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
-    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(256), 0, s, a, b, part, grid, c0, skip);
+    const int parts = pnh_gather_parts(b);
+    dim3 gg((unsigned)(b.n_tiles * parts));
+    if (nbc == 4) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 4>), gg, dim3(256), 0, s, a, b, parts, part, grid, c0, skip);
+    else if (nbc == 2) hipLaunchKernelGGL((y2k_gather_kernel<T, D, 2>), gg, dim3(256), 0, s, a, b, parts, part, grid, c0, skip);
+    else hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), gg, dim3(256), 0, s, a, b, parts, part, grid, c0, skip);
     PNH_LAUNCH_CHECK();
     c0 += nbc;
   }
@@ -1003,7 +1010,7 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
                                (C*)part_ + probs[p].part_off, 0);
 }
 
-// Gather of every problem's grid: one block per tile, blocks by table.
+// Gather of every problem's grid: pnh_gather_parts blocks per tile, blocks by table.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
 gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
@@ -1016,7 +1023,10 @@ gather_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t,
   const int lb = (int)(blockIdx.x - t.first[p]);
   const InterpArgs a = probs[p].ia;
   const BinArgs b = probs[p].bins;
-  gather_tile<T, D, 1>(a, b, lb, (const C*)part_ + probs[p].part_off,
+  const int parts = pnh_gather_parts(b);
+  const int tile = lb / parts, q = lb - tile * parts;
+  const int span = parts == 1 ? (1 << 30) : 256;
+  gather_tile<T, D, 1>(a, b, tile, q * 256, span, (const C*)part_ + probs[p].part_off,
                        (C*)k_ + probs[p].k_off, 0);
 }
 

@@ -249,6 +249,7 @@ class NUFFT:
         d.n_tiles = int(st['tile_slots'].shape[0])
         d.tile_slots = t['tile_slots'].data_ptr()
         d.n_slots = int(st['items'].shape[0])
+        d.max_tile_slots = int(st['tile_slots'][:, 1].max()) if st['tile_slots'].size else 0
         d.threads = self._threads
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):

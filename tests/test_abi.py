@@ -103,6 +103,7 @@ def _fake_desc(ndim, Jd, tile, K=64, N=32, M=100):
         setattr(d, f, 0x1000)
     d.n_items = 4
     d.n_slots = 4
+    d.max_tile_slots = 1
     d.n_tiles = (-(-K // tile)) ** ndim if tile < K else 1
     d.u_bound = 1.0
     return d
