@@ -116,14 +116,22 @@ def cpu_baseline(w, m_sample, seed=7):
 
 
 def bench_batched(torch, W, maxiter=20, reps=3):
-    """Batched uneven 2-D CG (64 problems) vs looping the plain operator."""
+    """Batched uneven 2-D CG (64 problems, gridding knobs autotuned on the
+    batched CG) vs looping the plain operator, each problem planned with the
+    defaults as a user of the reference would (one NUFFT per problem)."""
     from pynufft_amd.batch import BatchNUFFT
+    from pynufft_amd.nufft import NUFFT
     ps = W.batched2d()
     t0 = time.perf_counter()
     B = BatchNUFFT()
-    B.plan(ps)
+    B.plan(ps, tune=True)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
+    plain_ops = []
+    for p in ps:
+        op = NUFFT()
+        op.plan(p['om'], p['Nd'], p['Kd'], p['Jd'])
+        plain_ops.append(op)
     xs = [torch.from_numpy(p['x']).to(B.device) for p in ps]
     y = B.forward(xs)
     ys = B.unpack_samples(y)
@@ -137,9 +145,9 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / reps, out
 
-    progress('batched: planned %d problems in %.1f s' % (len(ps), t_plan))
+    progress('batched: planned %d problems in %.1f s (tuned %s)' % (len(ps), t_plan, B.tuned))
     t_plain, plain = timed(lambda: [op.solve(ys[i], 'cg', maxiter=maxiter, rtol=0.0)
-                                    for i, op in enumerate(B.ops)])
+                                    for i, op in enumerate(plain_ops)])
     t_batch, xb = timed(lambda: B.solve(y, maxiter=maxiter, rtol=0.0))
     t_graph, xg = timed(lambda: B.solve_graph(y, maxiter=maxiter, rtol=0.0))
     xgl = B.unpack_images(xg)
@@ -153,6 +161,8 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         'batched_graph_solves_per_s': n / t_graph,
         'speedup_graph_vs_plain': t_plain / t_graph,
         'max_rel_diff_vs_plain': diff, 'plan_s': t_plan,
+        'tuned': {'tile': B.tuned['tile'], 'chunk': B.tuned['chunk'],
+                  'threads': B.tuned['threads']},
     }
 
 

@@ -12,11 +12,14 @@ How fast y2k runs depends on three knobs of the plan:
   many CUs.
 
 ``tune`` times y2k on the device (HIP events on the launch stream) for a short
-list of valid candidates and returns the fastest.  The FP64 host design is
+list of valid candidates and returns the fastest.  ``tune_batch`` does the same
+for a ``BatchNUFFT`` by timing a few iterations of its batched FP64 CG, the
+workload the batch exists for.  The FP64 host design is
 computed once; each candidate only redoes the row sort.  Results are cached
 per device and geometry in a JSON file (``PYNUFFT_AMD_TUNE_CACHE``, default
 ``~/.cache/pynufft_amd/tune.json``); ``cache=False`` skips it.
 """
+import hashlib
 import json
 import os
 
@@ -156,4 +159,73 @@ def tune(om, Nd, Kd, Jd, batch=None, host_plan=None, device=None, reps=5,
     if cache:
         _store(key, {'tile': list(best['tile']), 'threads': best['threads'],
                      'chunk': best['chunk'], 'ms': best['ms']})
+    return best
+
+
+def resort(st, Kd, Jd, tile, chunk):
+    """A host plan with its rows re-sorted for another tile / rows per item."""
+    tile = helper.check_tile(tile, Kd, Jd) if tile is not None else helper.tile_shape(Kd, Jd)
+    perm, items, ts = helper.sort_and_bin(st['base'], Kd, Jd, tile, chunk)
+    s2 = dict(st)
+    s2.update(perm=perm, items=items, tile_slots=ts, tile=tile)
+    return s2
+
+
+def batch_candidates(problems):
+    """(tile, chunk, threads) triples for a batch: each problem's own tile
+    (None) or a common smaller or larger one where valid for every problem,
+    with rows per item and workgroup sizes around the defaults."""
+    d = len(problems[0][2])
+    tiles = [None] + {1: [(512,), (2048,)], 2: [(16, 16), (64, 64)],
+                      3: [(16, 16, 8), (8, 8, 8)]}[d]
+    out = []
+    for t in tiles:
+        if t is not None:
+            try:
+                for (_, _, Kd, Jd) in problems:
+                    helper.check_tile(t, Kd, Jd)
+            except ValueError:
+                continue
+        for chunk in (None, 1024):
+            for thr in (0, 512):
+                out.append((t, chunk, thr))
+    return out
+
+
+def tune_batch(batch, problems, host_plans, iters=3, reps=2, cache=True, verbose=False):
+    """Fastest (tile, chunk, threads) for a BatchNUFFT: each candidate is
+    planned (rows re-sorted from the given host designs) and timed on
+    ``iters`` iterations of the batched CG.  Returns the winner's dict."""
+    sig = hashlib.sha1(repr([(np.asarray(om).shape, float(np.asarray(om)[:64].sum()),
+                              Nd, Kd, Jd)
+                             for (om, Nd, Kd, Jd) in problems]).encode()).hexdigest()[:16]
+    key = '%s|batch|%s' % (torch.cuda.get_device_name(batch.device), sig)
+    if cache:
+        hit = _load().get(key)
+        if hit is not None:
+            return {'tile': tuple(hit['tile']) if hit['tile'] else None,
+                    'chunk': hit['chunk'], 'threads': int(hit['threads']),
+                    'ms': float(hit['ms'])}
+    g = torch.Generator().manual_seed(12345)
+    best = None
+    import concurrent.futures
+    pool = concurrent.futures.ThreadPoolExecutor(8)
+    for tile, chunk, thr in batch_candidates(problems):
+        sts = list(pool.map(lambda a: resort(a[1], a[0][2], a[0][3], tile, chunk),
+                            zip(problems, host_plans)))
+        batch._build(problems, sts, thr)
+        y = torch.complex(torch.randn(batch.n_samples, generator=g),
+                          torch.randn(batch.n_samples, generator=g)).to(batch.device,
+                                                                        torch.complex64)
+        ms = _time(lambda: batch.solve(y, maxiter=iters, rtol=0.0), reps)
+        if verbose:
+            print('tune_batch: tile %s chunk %s threads %d -> %.3f ms' % (tile, chunk, thr, ms),
+                  flush=True)
+        if best is None or ms < best['ms']:
+            best = {'tile': tile, 'chunk': chunk, 'threads': thr, 'ms': ms}
+    pool.shutdown()
+    batch.release()
+    if cache:
+        _store(key, {'tile': list(best['tile']) if best['tile'] else None,
+                     'chunk': best['chunk'], 'threads': best['threads'], 'ms': best['ms']})
     return best

@@ -33,25 +33,39 @@ class BatchNUFFT:
         self._graphs = {}
         self._ws = {}
 
-    def plan(self, problems, tile=None, chunk=None, threads=0):
+    def plan(self, problems, tile=None, chunk=None, threads=0, tune=False):
         """``problems``: sequence of (om, Nd, Kd, Jd) (or dicts with those keys).
         ``tile``/``chunk``/``threads``: gridding knobs applied to every
-        problem (default: each problem's own tile, see ``NUFFT.plan``)."""
+        problem (default: each problem's own tile, see ``NUFFT.plan``).
+        ``tune=True`` picks them by timing a short batched CG over a list of
+        candidates (``pynufft_amd.autotune.tune_batch``)."""
         self.release()
-        probs = [(p['om'], p['Nd'], p['Kd'], p['Jd']) if isinstance(p, dict) else p
-                 for p in problems]
+        probs = [(p['om'], tuple(p['Nd']), tuple(p['Kd']), tuple(p['Jd']))
+                 if isinstance(p, dict) else
+                 (p[0], tuple(p[1]), tuple(p[2]), tuple(p[3])) for p in problems]
         if not probs:
             raise ValueError('need at least one problem')
-        # host FP64 designs in parallel (numpy releases the GIL), then upload
+        # host FP64 designs in parallel (numpy releases the GIL)
         with concurrent.futures.ThreadPoolExecutor(8) as pool:
             sts = list(pool.map(
-                lambda p: helper.plan(p[0], tuple(p[1]), tuple(p[2]), tuple(p[3]),
-                                      tile=tile, chunk_rows=chunk), probs))
+                lambda p: helper.plan(p[0], p[1], p[2], p[3], tile=tile, chunk_rows=chunk),
+                probs))
+        if tune:
+            from . import autotune
+            cfg = autotune.tune_batch(self, probs, sts)
+            tile, chunk, threads = cfg['tile'], cfg['chunk'], cfg['threads']
+            sts = [autotune.resort(st, p[2], p[3], tile, chunk) for p, st in zip(probs, sts)]
+            self.tuned = cfg
+        self._build(probs, sts, threads)
+        return 0
+
+    def _build(self, probs, sts, threads):
+        """Upload every problem and create the batch tables."""
+        self.release()
         self.ops = []
         for (om, Nd, Kd, Jd), st in zip(probs, sts):
             op = NUFFT(self.device)
-            op.plan(om, tuple(Nd), tuple(Kd), tuple(Jd), tile=tile, chunk=chunk,
-                    host_plan=st, threads=threads)
+            op.plan(om, Nd, Kd, Jd, host_plan=st, threads=threads)
             op._ensure_fp64()
             self.ops.append(op)
         n = len(self.ops)
@@ -75,7 +89,6 @@ class BatchNUFFT:
         self.n_image, self.n_samples, self.n_grid = ni.value, ns.value, ng.value
         self._x_off = np.concatenate([[0], np.cumsum([int(np.prod(d)) for d in self.Nd])])
         self._y_off = np.concatenate([[0], np.cumsum(self.M)])
-        return 0
 
     def release(self):
         for g in getattr(self, '_graphs', {}).values():

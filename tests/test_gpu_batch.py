@@ -81,3 +81,26 @@ def test_batch_cg_graph_replays(batch):
         # eager work between replays
         B.adjoint(B.forward(xs))
         torch.cuda.synchronize()
+
+
+def test_autotuned_batch_matches_plain(hip, tmp_path, monkeypatch):
+    """BatchNUFFT.plan(tune=True) times the candidates on the batched CG,
+    caches the winner, and its solves match the per-problem solves."""
+    import torch
+    from pynufft_amd import autotune
+    from pynufft_amd.batch import BatchNUFFT
+    monkeypatch.setenv('PYNUFFT_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    ps = W.batched2d(n=5, seed0=910)
+    B = BatchNUFFT()
+    B.plan(ps, tune=True)
+    assert (tmp_path / 'tune.json').exists()
+    probs = [(p['om'], p['Nd'], p['Kd'], p['Jd']) for p in ps]
+    assert (B.tuned['tile'], B.tuned['chunk'], B.tuned['threads']) in \
+        autotune.batch_candidates(probs)
+    xs = [torch.from_numpy(p['x']).to(B.device) for p in ps]
+    y = B.forward(xs)
+    xb = B.unpack_images(B.solve(y, maxiter=8, rtol=0.0))
+    ys = B.unpack_samples(y)
+    for i, op in enumerate(B.ops):
+        xp = op.solve(ys[i], 'cg', maxiter=8, rtol=0.0)
+        assert float(torch.linalg.vector_norm(xb[i] - xp) / torch.linalg.vector_norm(xp)) < 1e-5
