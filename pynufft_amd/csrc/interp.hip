@@ -279,8 +279,9 @@ __device__ __forceinline__ T block_max(T v, T* red) {
   return r;
 }
 
-// One work item (tile id, first row, rows, slot).  LDS: the accumulator
-// [pad_cells][NBC][re, im] as int64, then reduction scratch.  JN bounds the
+// One work item (tile id, first row, rows, slot).  LDS: the accumulator as
+// int64, a real plane [pad_cells][NBC] then an imaginary plane, then
+// reduction scratch.  JN bounds the
 // unrolled support loops of the leading axes; JE > 0 states that they are
 // exactly JE long (no per-offset guards).
 template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
@@ -558,15 +559,19 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
               row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
             }
           if (D == 2) {
+            C s0[NBC];
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) s0[c] = mk<T>((T)0, (T)0);
 #pragma unroll
             for (int j0 = 0; j0 < JN; ++j0) {
               if (JE > 0 || j0 < J0) {
-                const C w = cmulf(u0c[j0], ul);
                 const int idx = row0[j0] + xl;
 #pragma unroll
-                for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+                for (int c = 0; c < NBC; ++c) s0[c] = cfma(u0c[j0], box[idx * NBC + c], s0[c]);
               }
             }
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) acc[c] = cmulf(ul, s0[c]);
           } else {
             const int x1b = a.base[a.M + r] - f.T0[1];
             const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
@@ -578,21 +583,31 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
                 u1c[j] = u1[j];
                 off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
               }
+            // acc = ul * sum_j0 u0[j0] * (sum_j1 u1[j1] * box[j0, j1]): one
+            // complex multiply-add per support cell
+            C s0[NBC];
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) s0[c] = mk<T>((T)0, (T)0);
 #pragma unroll
             for (int j0 = 0; j0 < JN; ++j0) {
               if (JE > 0 || j0 < J0) {
-                const C a0 = cmulf(u0c[j0], ul);
+                C s1[NBC];
+#pragma unroll
+                for (int c = 0; c < NBC; ++c) s1[c] = mk<T>((T)0, (T)0);
 #pragma unroll
                 for (int j1 = 0; j1 < JN; ++j1) {
                   if (JE > 0 || j1 < J1) {
-                    const C w = cmulf(a0, u1c[j1]);
                     const int idx = row0[j0] + off1[j1];
 #pragma unroll
-                    for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
+                    for (int c = 0; c < NBC; ++c) s1[c] = cfma(u1c[j1], box[idx * NBC + c], s1[c]);
                   }
                 }
+#pragma unroll
+                for (int c = 0; c < NBC; ++c) s0[c] = cfma(u0c[j0], s1[c], s0[c]);
               }
             }
+#pragma unroll
+            for (int c = 0; c < NBC; ++c) acc[c] = cmulf(ul, s0[c]);
           }
         }
       }
