@@ -197,6 +197,39 @@ def bench_cg_radial(torch, W, maxiter=30, reps=3, cpu=True):
     return out
 
 
+def bench_other_workloads(torch, W, steps=20):
+    """forward+adjoint pairs/s of the other configurations, each planned
+    with the defaults: plain 2-D 64^2 (M 4 096), radial 2-D 256^2 (M 205 824)
+    and the 8-coil radial 256^2 (M 131 072) through forward_one2many +
+    adjoint_many2one."""
+    from pynufft_amd.nufft import NUFFT
+    out = {}
+    for name in ('plain2d', 'radial2d', 'multicoil2d'):
+        w = getattr(W, name)()
+        A = NUFFT()
+        if name == 'multicoil2d':
+            A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'], batch=w['S'].shape[-1])
+            A.set_sense(w['S'])
+            x = torch.from_numpy(w['x']).to(A.device)
+            fn = lambda: A.adjoint_many2one(A.forward_one2many(x))
+        else:
+            A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
+            x = torch.from_numpy(w['x']).to(A.device)
+            fn = lambda: A.adjoint(A.forward(x))
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        out[name] = {'pairs_per_s': 1.0 / dt, 'ms_per_pair': 1000.0 * dt,
+                     'Nd': list(w['Nd']), 'M': int(w['om'].shape[0]),
+                     'coils': int(w['S'].shape[-1]) if name == 'multicoil2d' else 1}
+        A.release()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -293,6 +326,10 @@ def main():
     progress('single-problem CG (radial 2-D)')
     cg_single = bench_cg_radial(torch, W, cpu=not args.no_cpu_baseline) \
         if not args.no_batched else None
+    others = None
+    if not args.no_batched:
+        progress('other workloads')
+        others = bench_other_workloads(torch, W)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -320,6 +357,7 @@ def main():
         'plan_s': t_plan,
         'batched_cg': batched,
         'cg_single': cg_single,
+        'other_workloads': others,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))
