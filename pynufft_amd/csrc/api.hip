@@ -128,7 +128,6 @@ int build_pass(const pnh_plan* p, int a, bool inverse, bool src_image,
   if (pnh_fft_factor(f->n, f->radix, f->ns, &f->npass)) return -2;
   const size_t esz = fp64 ? 16 : 8;
   const int n = f->n;
-  f->lds_pad = 0;
   // LDS per line: line buffers (none for one pass, one for two, two for
   // more) plus 24 B of per-line offsets
   const int nbuf = f->npass >= 3 ? 2 : (f->npass == 2 ? 1 : 0);

@@ -33,7 +33,6 @@ struct FftDesc {
   int radix[PNH_FFT_MAX_PASS];
   int ns[PNH_FFT_MAX_PASS];
   int lines_per_block;
-  int lds_pad;           // unused (line padding is fixed by pnh_fft_lds_line)
   int threads;
   int inner_major;      // 1: threads walk lines fastest (strided axis)
 };
