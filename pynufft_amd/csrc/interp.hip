@@ -699,21 +699,50 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
     C s[NBC];
 #pragma unroll
     for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
+    if (b.max_tile_slots == 1) {
+      // one slot per source at most (and some slot exists): all loads
+      // issued unconditionally (sources that do not apply read the slot
+      // array's first cell), the sum selects, so the loads overlap
+      C v[1 << D][NBC];
+      bool use[1 << D];
 #pragma unroll
-    for (int mask = 0; mask < (1 << D); ++mask) {
-      bool ok = src[mask].y > 0;
-      int pidx = 0;
+      for (int mask = 0; mask < (1 << D); ++mask) {
+        bool ok = src[mask].y > 0;
+        int pidx = 0;
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        const bool bit = (mask >> d) & 1;
-        ok &= !bit || l[d] < below[d];
-        pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
+        for (int d = 0; d < D; ++d) {
+          const bool bit = (mask >> d) & 1;
+          ok &= !bit || l[d] < below[d];
+          pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
+        }
+        use[mask] = ok;
+        const C* p = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC : 0);
+#pragma unroll
+        for (int cc = 0; cc < NBC; ++cc) v[mask][cc] = p[cc];
       }
-      if (!ok) continue;
-      for (int k = 0; k < src[mask].y; ++k) {
-        const C* p = part + ((int64_t)(src[mask].x + k) * b.pad_cells + pidx) * NBC;
 #pragma unroll
-        for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
+      for (int mask = 0; mask < (1 << D); ++mask)
+        if (use[mask]) {
+#pragma unroll
+          for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], v[mask][cc]);
+        }
+    } else {
+#pragma unroll
+      for (int mask = 0; mask < (1 << D); ++mask) {
+        bool ok = src[mask].y > 0;
+        int pidx = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const bool bit = (mask >> d) & 1;
+          ok &= !bit || l[d] < below[d];
+          pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
+        }
+        if (!ok) continue;
+        for (int k = 0; k < src[mask].y; ++k) {
+          const C* p = part + ((int64_t)(src[mask].x + k) * b.pad_cells + pidx) * NBC;
+#pragma unroll
+          for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], p[cc]);
+        }
       }
     }
     int64_t g = 0;

@@ -317,3 +317,28 @@ def test_non_finite_samples_reach_their_cells(hip):
     support = np.abs(B.y2k(yz)) > 0
     assert np.all(np.isnan(k[support]))
     assert np.isfinite(k).sum() > 0.5 * k.size
+
+
+@pytest.mark.parametrize('nb', [1, 2, 3, 4])
+@pytest.mark.parametrize('name', ['tiny1d', 'tiny2d_j3', 'tiny3d_mixed'])
+def test_adjoint_matches_staged_path(hip, name, nb):
+    """adjoint / selfadjoint agree with their stages (y2k -> k2xx -> xx2x)
+    and with single-coil plans, for coil counts gridded in one pass (1, 2,
+    4) and in two (3), with default tiles and with split tiles (padding that
+    wraps across the grid edge) and 3-row work items (several slots per
+    tile)."""
+    from pynufft_amd.nufft import NUFFT
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    M = g['om'].shape[0]
+    rng = np.random.default_rng(nb)
+    y = (rng.standard_normal((M, nb)) + 1j * rng.standard_normal((M, nb))).astype(np.complex64)
+    for kw in ({}, {'tile': _split_tiles(Kd, Jd), 'chunk': 3}):
+        A = NUFFT()
+        A.plan(g['om'], Nd, Kd, Jd, batch=nb, **kw)
+        staged = A.xx2x(A.k2xx(A.y2k(y)))
+        assert rel(A.adjoint(y), staged) < 1e-6
+        x = A.adjoint(y)
+        assert rel(A.selfadjoint(x), A.xx2x(A.k2xx(A.y2k(A.forward(x))))) < 1e-6
+        for c in range(nb):
+            assert rel(A.adjoint(y)[..., c], make(g, **kw).adjoint(y[:, c])) < 1e-6
