@@ -43,7 +43,9 @@ enum pnh_op {
   PNH_OP_SELFADJOINT = 2,
   PNH_OP_Y2K = 3,
   PNH_OP_K2Y2K = 4,
-  PNH_OP_SOLVE_CG = 5
+  PNH_OP_SOLVE_CG = 5,
+  PNH_OP_FORWARD_ONE2MANY = 6,
+  PNH_OP_ADJOINT_MANY2ONE = 7
 };
 
 /*
@@ -154,6 +156,18 @@ int pnh_selfadjoint(const pnh_plan* plan, const void* x, void* x_out, void* ws,
  * trajectory gradient of pynufft_amd.autograd. */
 int pnh_forward_grad_om(const pnh_plan* plan, const void* k, const void* g,
                         float* om_grad, void* stream);
+/* Multi-coil operators of a plan with batch = ncoil; sense is the coil
+ * sensitivity array, Nd x ncoil complex64 with the coil index fastest.
+ * forward_one2many: y[:, c] = forward(x * sense[..., c]) for one Nd image x
+ * (reference linalg/nufft_cpu.py:177-189; device kernel cDistribute,
+ * src/re_subroutine.py:355-374).
+ * adjoint_many2one: x = mean over c of adjoint(y)[..., c] * conj(sense[..., c])
+ * (reference linalg/nufft_cpu.py:191-203, numpy.mean over the coil axis;
+ * device kernel cMerge, src/re_subroutine.py:378-439). */
+int pnh_forward_one2many(const pnh_plan* plan, const void* x, const void* sense, void* y,
+                         void* ws, void* stream);
+int pnh_adjoint_many2one(const pnh_plan* plan, const void* y, const void* sense, void* x,
+                         void* ws, void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

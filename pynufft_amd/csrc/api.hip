@@ -353,6 +353,12 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
     case PNH_OP_SELFADJOINT: *bytes = grid + ys + part; break;
     case PNH_OP_Y2K: *bytes = part; break;
     case PNH_OP_K2Y2K: *bytes = ys + part; break;
+    case PNH_OP_FORWARD_ONE2MANY:
+      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + grid;
+      break;
+    case PNH_OP_ADJOINT_MANY2ONE:
+      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + grid + part;
+      break;
     case PNH_OP_SOLVE_CG: {
       const size_t g2 = align_up((size_t)p->Ktot * p->nb * 16);
       const size_t y2 = align_up((size_t)p->M * p->nb * 16);
@@ -451,6 +457,28 @@ int pnh_selfadjoint(const pnh_plan* p, const void* x, void* x_out, void* ws, voi
   if (p->M > 0 && (rc = do_k2y(p, 0, grid, ytmp, nullptr, s))) return rc;
   if ((rc = do_y2k(p, 0, ytmp, grid, part, nullptr, s))) return rc;
   return run_passes_inverse(p, grid, x_out, true, 0, nullptr, s);
+}
+
+int pnh_forward_one2many(const pnh_plan* p, const void* x, const void* sense, void* y,
+                         void* ws, void* stream) {
+  REQUIRE(p && x && sense && ws && (y || p->M == 0), "null argument");
+  char* w = (char*)ws;
+  void* stack = w;
+  void* fws = w + align_up((size_t)p->Ntot * p->nb * 8);
+  int rc = pnh_coil_expand(x, sense, stack, p->Ntot, p->nb, (hipStream_t)stream);
+  if (rc) return rc;
+  return pnh_forward(p, stack, y, fws, stream);
+}
+
+int pnh_adjoint_many2one(const pnh_plan* p, const void* y, const void* sense, void* x,
+                         void* ws, void* stream) {
+  REQUIRE(p && x && sense && ws && (y || p->M == 0), "null argument");
+  char* w = (char*)ws;
+  void* stack = w;
+  void* aws = w + align_up((size_t)p->Ntot * p->nb * 8);
+  int rc = pnh_adjoint(p, y, stack, aws, stream);
+  if (rc) return rc;
+  return pnh_coil_combine(stack, sense, x, p->Ntot, p->nb, (hipStream_t)stream);
 }
 
 int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* om_grad,

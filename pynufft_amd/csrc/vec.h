@@ -24,6 +24,12 @@ int pnh_scale_sn(int fp64, const void* in, void* out, const void* sn,
 int pnh_c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s);
 int pnh_cast_div_sn(const void* in, void* out, const float* sn, int64_t n_img,
                     int nb, hipStream_t s);
+// out[p*nb + c] = x[p] * sense[p*nb + c] (complex64)
+int pnh_coil_expand(const void* x, const void* sense, void* out, int64_t n_img, int nb,
+                    hipStream_t s);
+// out[p] = mean_c xc[p*nb + c] * conj(sense[p*nb + c]) (complex64)
+int pnh_coil_combine(const void* xc, const void* sense, void* out, int64_t n_img, int nb,
+                     hipStream_t s);
 int pnh_zero(int fp64, void* p, int64_t n, hipStream_t s);
 int pnh_cg_dot(int mode, const void* a, const void* b, int64_t n_cells, int nb,
                void* part, const int* skip, hipStream_t s);

@@ -45,6 +45,36 @@ __global__ void cast_div_sn_kernel(const double2* __restrict__ in, float2* __res
   }
 }
 
+// out[p*nb + c] = x[p] * sense[p*nb + c]: one image broadcast to the coils
+// (complex64 product as numpy forms it: (ac - bd, ad + bc)).
+__global__ void coil_expand_kernel(const float2* __restrict__ x, const float2* __restrict__ sense,
+                                   float2* __restrict__ out, int64_t n_img, int nb) {
+  const int64_t n = n_img * nb;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = x[i / nb], s = sense[i];
+    out[i] = make_float2(v.x * s.x - v.y * s.y, v.x * s.y + v.y * s.x);
+  }
+}
+
+// out[p] = (sum_c xc[p*nb + c] * conj(sense[p*nb + c])) / nb: the coil
+// images combined with the conjugate sensitivities and averaged over coils
+// (complex64, coils summed in order).
+__global__ void coil_combine_kernel(const float2* __restrict__ xc,
+                                    const float2* __restrict__ sense, float2* __restrict__ out,
+                                    int64_t n_img, int nb) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n_img;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = 0; c < nb; ++c) {
+      const float2 v = xc[p * nb + c], s = sense[p * nb + c];
+      acc.x += v.x * s.x + v.y * s.y;
+      acc.y += v.y * s.x - v.x * s.y;
+    }
+    out[p] = make_float2(acc.x / (float)nb, acc.y / (float)nb);
+  }
+}
+
 template <typename C>
 __global__ void zero_kernel(C* __restrict__ p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -238,6 +268,24 @@ int pnh_cast_div_sn(const void* in, void* out, const float* sn, int64_t n_img,
                     int nb, hipStream_t s) {
   hipLaunchKernelGGL(cast_div_sn_kernel, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
                      (const double2*)in, (float2*)out, sn, n_img, nb);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_coil_expand(const void* x, const void* sense, void* out, int64_t n_img, int nb,
+                    hipStream_t s) {
+  if (n_img * nb == 0) return 0;
+  hipLaunchKernelGGL(coil_expand_kernel, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
+                     (const float2*)x, (const float2*)sense, (float2*)out, n_img, nb);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_coil_combine(const void* xc, const void* sense, void* out, int64_t n_img, int nb,
+                     hipStream_t s) {
+  if (n_img == 0) return 0;
+  hipLaunchKernelGGL(coil_combine_kernel, dim3(grid_for(n_img)), dim3(256), 0, s,
+                     (const float2*)xc, (const float2*)sense, (float2*)out, n_img, nb);
   PNH_LAUNCH_CHECK();
   return 0;
 }

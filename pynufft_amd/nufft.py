@@ -77,6 +77,7 @@ class NUFFT:
         self.batch = None
         self._plan = None
         self._ws = {}
+        self._sense = None
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
@@ -116,6 +117,7 @@ class NUFFT:
             raise ValueError('batch must be >= 1')
         self.batch = None if batch is None else int(batch)
         self._nb = 1 if batch is None else int(batch)
+        self._sense = None
         M = int(st['M'])
         self.M = (M,)
         self.Kdprod = int(np.prod(Kd))
@@ -501,7 +503,7 @@ class NUFFT:
     # ------------------------------------------------------ multi-coil --
     def set_sense(self, coil_profile):
         """Coil sensitivities S, shape Nd + (batch,) (reference
-        ``linalg/nufft_cpu.py:163-175``, ``set_sense``)."""
+        ``linalg/nufft_cpu.py:166-175``, ``set_sense``)."""
         if self.batch is None:
             raise ValueError('plan with batch=<number of coils> first')
         S = torch.as_tensor(coil_profile).to(self.device, torch.complex64)
@@ -509,26 +511,52 @@ class NUFFT:
             raise ValueError('coil profile must have shape %s' % (self.Nd + (self.batch,),))
         self._sense = S.contiguous()
 
+    def reset_sense(self):
+        """All coil sensitivities set to 1 (reference ``reset_sense``,
+        ``linalg/nufft_cpu.py:163-164``)."""
+        if self.batch is None:
+            raise ValueError('plan with batch=<number of coils> first')
+        self._sense = torch.ones(self.Nd + (self.batch,), dtype=_c64, device=self.device)
+
+    def _need_sense(self):
+        self._need_plan()
+        if self._sense is None:
+            raise ValueError('set_sense(coil_profile) first')
+
     def forward_one2many(self, x):
-        """y[:, c] = forward(S_c x) (reference ``linalg/nufft_cpu.py:177-189``)."""
+        """y[:, c] = forward(x * S[..., c]) for one image x of shape Nd
+        (reference ``linalg/nufft_cpu.py:177-189``; coil broadcast by the
+        library's coil-expand kernel, then the batched forward)."""
+        self._need_sense()
         is_np = not isinstance(x, torch.Tensor)
-        xi = torch.as_tensor(x).to(self.device, torch.complex64).reshape(self.Nd)
-        y = self.forward(xi[..., None] * self._sense)
-        return y.cpu().numpy() if is_np else y
+        if is_np:
+            x = np.asarray(x)
+        if int(np.prod(x.shape)) != int(np.prod(self.Nd)):
+            raise ValueError('expected an image of shape %s, got %s' % (self.Nd, tuple(x.shape)))
+        xt = torch.as_tensor(x).to(self.device, _c64).reshape(self.Nd).contiguous()
+        out = self._empty('y')
+        ws = self._workspace(_lib.OP_FORWARD_ONE2MANY, _lib.C64)
+        self._call(self._libh.pnh_forward_one2many, _ptr(xt), _ptr(self._sense), _ptr(out),
+                   _ptr(ws), self._stream())
+        return self._out(out, is_np)
 
     def adjoint_many2one(self, y):
-        """sum_c conj(S_c) adjoint(y[:, c]) (reference ``linalg/nufft_cpu.py:191-203``)."""
-        is_np = not isinstance(y, torch.Tensor)
-        xt = self.adjoint(torch.as_tensor(y).to(self.device, torch.complex64))
-        out = (xt * self._sense.conj()).sum(dim=-1)
-        return out.cpu().numpy() if is_np else out
+        """mean over coils of adjoint(y)[..., c] * conj(S[..., c]) (reference
+        ``linalg/nufft_cpu.py:191-203``, ``numpy.mean`` over the coil axis)."""
+        self._need_sense()
+        yt, is_np = self._in(y, 'y')
+        out = torch.empty(self.Nd, dtype=_c64, device=self.device)
+        ws = self._workspace(_lib.OP_ADJOINT_MANY2ONE, _lib.C64)
+        self._call(self._libh.pnh_adjoint_many2one, _ptr(yt), _ptr(self._sense), _ptr(out),
+                   _ptr(ws), self._stream())
+        return self._out(out, is_np)
 
     def selfadjoint_one2many2one(self, x):
         """adjoint_many2one(forward_one2many(x))."""
         is_np = not isinstance(x, torch.Tensor)
-        xi = torch.as_tensor(x).to(self.device, torch.complex64)
+        xi = torch.as_tensor(np.asarray(x) if is_np else x).to(self.device, _c64)
         out = self.adjoint_many2one(self.forward_one2many(xi))
-        return out.cpu().numpy() if is_np else out
+        return self._out(out, is_np)
 
     # ----------------------------------------------------------- misc --
     def to_device(self, x, shape=None):
