@@ -24,6 +24,13 @@ OracleNUFFT stages     golden forward/adjoint/selfadjoint/solve outputs;
                        adjoint identity; FFT vs brute-force DFT
 cg_scipy               golden 30-iteration image; dense FP64 solve (tiny)
 OracleNUDFT            brute-force triple loop on tiny inputs
+laplacian_spectrum     closed form sum_d 2 cos(2 pi k_d / K_d) - 2
+precompute_sp,         golden W and selfadjoint2 of the reference
+  selfadjoint2         (tests/golden/solvers.npz)
+solve 'dc', 'L1TVOLS', golden outputs of the reference's solve for each
+  'lsqr', 'lsmr',      solver on four shapes (tests/golden/solvers.npz)
+  'bicgstab', 'bicg',
+  'gmres', 'lgmres'
 =====================  ======================================================
 """
 import numpy as np
@@ -246,6 +253,71 @@ def cg_scipy(matvec, b, maxiter, rtol=1e-5, atol=0.0):
     return scipy.sparse.linalg.cg(A, b, maxiter=maxiter, rtol=rtol, atol=atol)
 
 
+def laplacian_spectrum(Kd):
+    """fftn of the periodic discrete Laplacian stencil on the Kd grid: -2d at
+    the origin, 1 at the two neighbours along each axis, in complex64.
+    Reference src/_helper/helper.py:11-45 (create_laplacian_kernel)."""
+    d = len(Kd)
+    stencil = np.zeros(Kd, dtype=_C64)
+    origin = (0,) * d
+    stencil[origin] = -2.0 * d
+    for p in range(d):
+        for step in (1, -1):
+            at = list(origin)
+            at[p] = step
+            stencil[tuple(at)] = 1
+    return np.fft.fftn(stencil)
+
+
+def diff_back(x, p):
+    """x[i - e_p] - x[i], periodic (reference solve_cpu.py:11-23 cDiff with
+    helper.py:46-70 indxmap_diff's d_indx)."""
+    return np.roll(x, 1, axis=p) - x
+
+
+def diff_fwd(x, p):
+    """x[i + e_p] - x[i], periodic (cDiff with indxmap_diff's dt_indx)."""
+    return np.roll(x, -1, axis=p) - x
+
+
+def l1tvols(A, y, maxiter, rho):
+    """L1 total-variation regularised least squares by split Bregman.
+    Reference linalg/solve_cpu.py:42-163 (L1TVOLS) with mu = 1, lambda = rho:
+    each outer iteration solves (mu W - lambda Lap) x = rhs in k-space, takes
+    the periodic image gradient, shrinks its magnitude by 1/lambda, updates the
+    Bregman variables and the data residual AHyk += AHy - AHA x.  complex64
+    arithmetic as numpy runs it there."""
+    mu = 1.0
+    lam = rho * mu
+    uker = mu * A.kspace_density() - lam * laplacian_spectrum(A.Kd)
+    AHy = A.adjoint(y)
+    x = np.zeros_like(AHy)
+    AHyk = np.zeros_like(AHy)
+    d = len(A.Nd)
+    dd = [np.zeros(A.Nd, dtype=_C64) for _ in range(d)]
+    bb = [np.zeros(A.Nd, dtype=_C64) for _ in range(d)]
+    for _ in range(maxiter):
+        rhs = mu * AHyk
+        for p in range(d):
+            rhs += lam * diff_fwd(dd[p] - bb[p], p)
+        x = A.k2xx((A.xx2k(rhs) + 1e-7) / (uker + 1e-7))
+        zz = [diff_back(x, p) for p in range(d)]
+        zf = A.selfadjoint(x) - AHy
+        mag = np.zeros_like(zz[0])
+        t = []
+        for p in range(d):
+            t.append(zz[p] + bb[p])
+            mag = np.hypot(np.hypot(t[p].real, t[p].imag), mag.real)
+        mag += 1e-5
+        thr = 1 / lam
+        r = (mag > thr) * (mag - thr) / mag
+        for p in range(d):
+            dd[p] = t[p] * r
+            bb[p] += zz[p] - dd[p]
+        AHyk = AHyk - zf
+    return x
+
+
 class OracleNUFFT:
     """CPU NUFFT with the reference's stage semantics.
 
@@ -313,14 +385,56 @@ class OracleNUFFT:
     def selfadjoint(self, x):
         return self.xx2x(self.k2xx(self.k2y2k(self.xx2k(self.x2xx(x)))))
 
-    def solve(self, y, solver=None, maxiter=None, rtol=1e-5, atol=0.0):
-        """'cg' branch of reference linalg/solve_cpu.py:256-288."""
-        if solver != 'cg':
-            raise KeyError('oracle implements only solver="cg"')
+    def precompute_sp(self):
+        """W = |xx2k(adjoint(1))| as (W W*)^0.5 in complex64.  Reference
+        _nufft_class_methods_cpu.py:126-145 (_precompute_sp_cpu)."""
+        W = self.xx2k(self.adjoint(np.ones(self.st['M'], dtype=_C64)))
+        self.W = (W * W.conj()) ** 0.5
 
-        def spHsp(v):
-            return self.k2y2k(v.reshape(self.Kd)).ravel()
-        k, info = cg_scipy(spHsp, self.y2k(y).ravel(), maxiter, rtol, atol)
+    def selfadjoint2(self, x):
+        """k2xx(W * xx2k(x)), no apodization.  Reference
+        _nufft_class_methods_cpu.py:216-222 (_selfadjoint2_cpu)."""
+        return self.k2xx(self.W * self.xx2k(x))
+
+    def kspace_density(self):
+        """|xx2k(adjoint(1))|.  Reference solve_cpu.py:25-40."""
+        return np.abs(self.xx2k(self.adjoint(np.ones(self.st['M'], dtype=_C64))))
+
+    def pipe_density(self, maxiter):
+        """W <- W / forward(adjoint(W)) from W = 1.  Reference
+        solve_cpu.py:165-183 (_pipe_density)."""
+        W = np.ones(self.st['M'], dtype=_C64)
+        for _ in range(maxiter):
+            W = W / self.forward(self.adjoint(W))
+        return W
+
+    def solve(self, y, solver=None, *args, **kwargs):
+        """Reference linalg/solve_cpu.py:185-288: 'dc' (density-compensated
+        adjoint), 'L1TVOLS', 'lsqr'/'lsmr' on P (k-space unknowns), and the
+        scipy Krylov solvers on P^H P; results are crop(ifftn(k)) / sn."""
+        if solver == 'dc':
+            return self.adjoint(self.pipe_density(*args, **kwargs) * y)
+        if solver == 'L1TVOLS':
+            return l1tvols(self, y, *args, **kwargs)
+        LO = scipy.sparse.linalg.LinearOperator
+        if solver in ('lsqr', 'lsmr'):
+            A = LO((self.st['M'], self.Kdprod), dtype=np.complex128,
+                   matvec=lambda k: self.k2y(k.reshape(self.Kd)).ravel(),
+                   rmatvec=lambda v: self.y2k(v.reshape(self.st['M'])).ravel())
+            fn = {'lsqr': scipy.sparse.linalg.lsqr, 'lsmr': scipy.sparse.linalg.lsmr}[solver]
+            k = fn(A, y.flatten(), *args, **kwargs)[0]
+        else:
+            fns = {'cg': scipy.sparse.linalg.cg, 'bicgstab': scipy.sparse.linalg.bicgstab,
+                   'bicg': scipy.sparse.linalg.bicg, 'gmres': scipy.sparse.linalg.gmres,
+                   'lgmres': scipy.sparse.linalg.lgmres}
+            if solver not in fns:
+                raise KeyError('unknown solver %r' % (solver,))
+
+            def spHsp(v):
+                return self.k2y2k(v.reshape(self.Kd)).ravel()
+            A = LO((self.Kdprod, self.Kdprod), matvec=spHsp, rmatvec=spHsp,
+                   dtype=np.complex128)
+            k = fns[solver](A, self.y2k(y).ravel(), *args, **kwargs)[0]
         xx = self.k2xx(k.reshape(self.Kd))
         return xx / self.sn
 

@@ -218,9 +218,41 @@ def make_small3d(ref):
          cg_iters=np.array(30), **versions())
 
 
+SOLVER_CASES = ('tiny1d', 'tiny2d', 'tiny3d', 'plain2d')
+L1TV_RHO = (0.05, 1.0)
+
+
+def make_solvers(ref):
+    """The Toeplitz weights and AHA of ``_precompute_sp_cpu`` /
+    ``_selfadjoint2_cpu`` and the solvers other than 'cg' (``solve_cpu.solve``:
+    'dc', 'L1TVOLS', 'lsqr', 'lsmr', 'bicgstab', 'bicg', 'gmres', 'lgmres'),
+    on the inputs of existing fixtures; keys are ``<case>__<output>``."""
+    out = {}
+    for case in SOLVER_CASES:
+        g = np.load(os.path.join(HERE, case + '.npz'))
+        Nd, Kd, Jd = (tuple(int(v) for v in g[k]) for k in ('Nd', 'Kd', 'Jd'))
+        x = g['x']
+        yin = g['adjoint_in']
+        A = ref.NUFFT()
+        A.plan(g['om'], Nd, Kd, Jd)
+        A._precompute_sp_cpu()
+        out[case + '__W'] = A.W
+        out[case + '__selfadjoint2'] = A._selfadjoint2_cpu(x)
+        out[case + '__dc3'] = A.solve(yin, 'dc', maxiter=3)
+        for rho in L1TV_RHO:
+            out[case + '__l1tvols5_rho%g' % rho] = A.solve(yin, 'L1TVOLS', maxiter=5, rho=rho)
+        out[case + '__lsqr8'] = A.solve(yin, 'lsqr', iter_lim=8)
+        out[case + '__lsmr8'] = A.solve(yin, 'lsmr', maxiter=8)
+        for solver in ('bicgstab', 'bicg'):
+            out[case + '__%s6' % solver] = A.solve(yin, solver, maxiter=6)
+        out[case + '__gmres'] = A.solve(yin, 'gmres', restart=4, maxiter=2)
+        out[case + '__lgmres'] = A.solve(yin, 'lgmres', maxiter=2)
+    save('solvers', **out)
+
+
 CASES = {'tiny': make_tiny, 'grid_j1': make_grid_j1, 'plain2d': make_plain2d,
          'radial2d': make_radial2d, 'small3d': make_small3d,
-         'uniform3d': make_uniform3d}
+         'uniform3d': make_uniform3d, 'solvers': make_solvers}
 
 
 def main():

@@ -126,3 +126,41 @@ def test_uniform3d_rows_match_reference():
 def test_workload_generators_stable():
     g = golden('small3d')
     assert np.array_equal(np.array(W.fingerprint(W.uniform_om(20000, 3, 303))), g['om_fp'])
+
+
+SOLVER_CASES = ['tiny1d', 'tiny2d', 'tiny3d', 'plain2d']
+
+
+@pytest.mark.parametrize('name', SOLVER_CASES)
+def test_toeplitz_and_solvers_match_reference(name):
+    """precompute_sp / selfadjoint2 and every solver other than 'cg' against
+    the reference's own outputs (tests/golden/solvers.npz)."""
+    g = golden(name)
+    s = golden('solvers')
+    Nd, Kd, Jd = geometry(g)
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    x, yin = g['x'], g['adjoint_in']
+    A.precompute_sp()
+    assert np.array_equal(A.W, s[name + '__W'])
+    assert np.array_equal(A.selfadjoint2(x), s[name + '__selfadjoint2'])
+    assert np.array_equal(A.solve(yin, 'dc', maxiter=3), s[name + '__dc3'])
+    for rho in (0.05, 1.0):
+        assert np.array_equal(A.solve(yin, 'L1TVOLS', maxiter=5, rho=rho),
+                              s[name + '__l1tvols5_rho%g' % rho])
+    assert rel(A.solve(yin, 'lsqr', iter_lim=8), s[name + '__lsqr8']) < 1e-6
+    assert rel(A.solve(yin, 'lsmr', maxiter=8), s[name + '__lsmr8']) < 1e-6
+    for solver in ('bicgstab', 'bicg'):
+        assert rel(A.solve(yin, solver, maxiter=6), s[name + '__%s6' % solver]) < 1e-6
+    assert rel(A.solve(yin, 'gmres', restart=4, maxiter=2), s[name + '__gmres']) < 1e-6
+    assert rel(A.solve(yin, 'lgmres', maxiter=2), s[name + '__lgmres']) < 1e-6
+
+
+def test_laplacian_spectrum_closed_form():
+    """fftn of the periodic Laplacian stencil is sum_d 2 cos(2 pi k_d/K_d) - 2
+    (real); a wrong sign or a missing neighbour breaks it."""
+    for Kd in [(8,), (6, 10), (4, 6, 8)]:
+        lap = oracle.laplacian_spectrum(Kd)
+        k = np.meshgrid(*[np.arange(K) for K in Kd], indexing='ij')
+        ref = sum(2 * np.cos(2 * np.pi * k[d] / Kd[d]) - 2 for d in range(len(Kd)))
+        assert np.abs(lap - ref).max() < 1e-5
