@@ -45,7 +45,11 @@ enum pnh_op {
   PNH_OP_K2Y2K = 4,
   PNH_OP_SOLVE_CG = 5,
   PNH_OP_FORWARD_ONE2MANY = 6,
-  PNH_OP_ADJOINT_MANY2ONE = 7
+  PNH_OP_ADJOINT_MANY2ONE = 7,
+  PNH_OP_PRECOMPUTE_SP = 8,
+  PNH_OP_SELFADJOINT2 = 9,
+  PNH_OP_SOLVE_DC = 10,
+  PNH_OP_SOLVE_L1TVOLS = 11
 };
 
 /*
@@ -168,6 +172,27 @@ int pnh_forward_one2many(const pnh_plan* plan, const void* x, const void* sense,
                          void* ws, void* stream);
 int pnh_adjoint_many2one(const pnh_plan* plan, const void* y, const void* sense, void* x,
                          void* ws, void* stream);
+/* Toeplitz-style self-adjoint (complex64).  precompute_sp writes
+ * W = |xx2k(adjoint(1))| (first coil) as float32 [prod(Kd)]; selfadjoint2
+ * returns k2xx(W * xx2k(x)) with no apodization, each coil weighted by W.
+ * Reference nufft/_nufft_class_methods_cpu.py:126-145 (_precompute_sp_cpu)
+ * and :216-222 (_selfadjoint2_cpu). */
+int pnh_precompute_sp(const pnh_plan* plan, float* W, void* ws, void* stream);
+int pnh_selfadjoint2(const pnh_plan* plan, const void* x, const float* W, void* x_out,
+                     void* ws, void* stream);
+/* solve(y, 'dc', maxiter): pipe density W <- W / forward(adjoint(W)) from
+ * W = 1, maxiter times, then x = adjoint(W * y) (per coil).  As in the
+ * reference, W is complex128 and the interpolation runs in FP64 (the plan
+ * needs u_c128); the FFTs are complex64 forward, complex128 inverse.
+ * Reference linalg/solve_cpu.py:165-183 (_pipe_density) and :214-235. */
+int pnh_solve_dc(const pnh_plan* plan, const void* y, void* x, int maxiter, void* ws,
+                 void* stream);
+/* solve(y, 'L1TVOLS', maxiter, rho): L1 total-variation regularised least
+ * squares by split Bregman with mu = 1, lambda = rho (complex64, batch 1).
+ * Reference linalg/solve_cpu.py:42-163 (L1TVOLS), src/_helper/helper.py:11-70
+ * (Laplacian kernel, periodic difference maps). */
+int pnh_solve_l1tvols(const pnh_plan* plan, const void* y, void* x, int maxiter, double rho,
+                      void* ws, void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

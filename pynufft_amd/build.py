@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'libpynufft_hip.so')
-SOURCES = ['api.hip', 'fft.hip', 'interp.hip', 'vec.hip']
+SOURCES = ['api.hip', 'fft.hip', 'interp.hip', 'vec.hip', 'solvers.hip']
 ARCH = os.environ.get('PYNUFFT_HIP_ARCH', 'gfx950')
 
 

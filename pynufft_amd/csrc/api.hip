@@ -9,6 +9,7 @@
 #include "common.h"
 #include "fft.h"
 #include "interp.h"
+#include "solvers.h"
 #include "vec.h"
 
 #define PNH_VERSION "pynufft_hip 0.1.0 (gfx950)"
@@ -340,6 +341,34 @@ int pnh_plan_destroy(pnh_plan* plan) {
   return 0;
 }
 
+// Scratch of density(): ones [M nb], image [N nb], adjoint scratch (grid, partials).
+static size_t density_ws_bytes(const pnh_plan* p) {
+  return align_up((size_t)p->M * p->nb * 8) + align_up((size_t)p->Ntot * p->nb * 8) +
+         align_up((size_t)p->Ktot * p->nb * 8) + align_up(partial_bytes(p, 0));
+}
+
+// Scratch of the L1TVOLS solve: uker and |density| (float [K]), images AHy,
+// AHyk, rhs, q and dd/bb per axis, then self-adjoint scratch (grid, samples,
+// partials), which also serves density() and the k-space solve.
+static size_t l1tv_ws_bytes(const pnh_plan* p) {
+  const size_t img = align_up((size_t)p->Ntot * 8);
+  const size_t sa = align_up((size_t)p->Ktot * 8) + align_up((size_t)p->M * 8) +
+                    align_up(partial_bytes(p, 0));
+  const size_t dens = density_ws_bytes(p);
+  return 2 * align_up((size_t)p->Ktot * 4) + (4 + 2 * (size_t)p->D) * img +
+         (sa > dens ? sa : dens);
+}
+
+// Scratch of the 'dc' solve: W, E and the weighted samples (complex128
+// [M nb] each), a complex128 and a complex64 grid, the complex128 image, the
+// FP64 gridding partials.
+static size_t dc_ws_bytes(const pnh_plan* p) {
+  const int64_t my = p->M * p->nb;
+  return 3 * align_up((size_t)my * 16) + align_up((size_t)p->Ktot * p->nb * 16) +
+         align_up((size_t)p->Ktot * p->nb * 8) + align_up((size_t)p->Ntot * p->nb * 16) +
+         align_up(partial_bytes(p, 1));
+}
+
 int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
   REQUIRE(p && bytes, "null argument");
   const int fp64 = dtype == PNH_C128;
@@ -358,6 +387,14 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
       break;
     case PNH_OP_ADJOINT_MANY2ONE:
       *bytes = align_up((size_t)p->Ntot * p->nb * 8) + grid + part;
+      break;
+    case PNH_OP_PRECOMPUTE_SP:
+      *bytes = density_ws_bytes(p);
+      break;
+    case PNH_OP_SELFADJOINT2: *bytes = align_up((size_t)p->Ktot * p->nb * 8); break;
+    case PNH_OP_SOLVE_DC: *bytes = dc_ws_bytes(p); break;
+    case PNH_OP_SOLVE_L1TVOLS:
+      *bytes = l1tv_ws_bytes(p);
       break;
     case PNH_OP_SOLVE_CG: {
       const size_t g2 = align_up((size_t)p->Ktot * p->nb * 16);
@@ -479,6 +516,126 @@ int pnh_adjoint_many2one(const pnh_plan* p, const void* y, const void* sense, vo
   int rc = pnh_adjoint(p, y, stack, aws, stream);
   if (rc) return rc;
   return pnh_coil_combine(stack, sense, x, p->Ntot, p->nb, (hipStream_t)stream);
+}
+
+// W[i] = |xx2k(adjoint(1))[i]| of the first coil (float [Ktot]).
+static int density(const pnh_plan* p, float* W, char* w, hipStream_t s) {
+  void* ones = w;
+  void* img = w + align_up((size_t)p->M * p->nb * 8);
+  char* aws = (char*)img + align_up((size_t)p->Ntot * p->nb * 8);
+  int rc;
+  if ((rc = pnh_fill_c64(ones, p->M * p->nb, 1.f, 0.f, s))) return rc;
+  if ((rc = pnh_adjoint(p, ones, img, aws, s))) return rc;
+  if ((rc = run_passes_forward(p, img, aws, false, 0, nullptr, s))) return rc;
+  return pnh_abs_first_coil(aws, W, p->Ktot, p->nb, s);
+}
+
+int pnh_precompute_sp(const pnh_plan* p, float* W, void* ws, void* stream) {
+  REQUIRE(p && W && ws, "null argument");
+  int rc = check_dtype_ready(p, PNH_C64);
+  if (rc) return rc;
+  return density(p, W, (char*)ws, (hipStream_t)stream);
+}
+
+int pnh_selfadjoint2(const pnh_plan* p, const void* x, const float* W, void* x_out,
+                     void* ws, void* stream) {
+  REQUIRE(p && x && W && x_out && ws, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  if ((rc = run_passes_forward(p, x, ws, false, 0, nullptr, s))) return rc;
+  if ((rc = pnh_scale_sn(0, ws, ws, W, p->Ktot, p->nb, s))) return rc;
+  return run_passes_inverse(p, ws, x_out, false, 0, nullptr, s);
+}
+
+// x (complex64) = xx2x(k2xx(y2k(v))) for complex128 samples v, with the
+// reference's precision: FP64 gridding and inverse FFT, rounded to complex64
+// before the apodization.
+static int adjoint_c128(const pnh_plan* p, const void* v, void* x, void* g64, void* img64,
+                        void* part, hipStream_t s) {
+  int rc;
+  if ((rc = do_y2k(p, 1, v, g64, part, nullptr, s))) return rc;
+  if ((rc = run_passes_inverse(p, g64, img64, false, 1, nullptr, s))) return rc;
+  return pnh_cast_mul_sn(img64, x, p->d.sn_f32, p->Ntot, p->nb, s);
+}
+
+int pnh_solve_dc(const pnh_plan* p, const void* y, void* x, int maxiter, void* ws,
+                 void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(maxiter >= 0, "maxiter must be >= 0");
+  int rc = check_dtype_ready(p, PNH_C64);
+  if (rc) return rc;
+  if ((rc = check_dtype_ready(p, PNH_C128))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t my = p->M * p->nb;
+  char* w = (char*)ws;
+  void* W = w; w += align_up((size_t)my * 16);
+  void* E = w; w += align_up((size_t)my * 16);
+  void* Wy = w; w += align_up((size_t)my * 16);
+  void* g64 = w; w += align_up((size_t)p->Ktot * p->nb * 16);
+  void* g32 = w; w += align_up((size_t)p->Ktot * p->nb * 8);
+  void* img64 = w; w += align_up((size_t)p->Ntot * p->nb * 16);
+  void* part = w;
+  // W = 1; W <- W / forward(adjoint(W)): the forward's FFT in complex64 and
+  // its interpolation in FP64 (complex128 result), as the reference runs them
+  if ((rc = pnh_fill_c128(W, my, 1.0, 0.0, s))) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    if ((rc = adjoint_c128(p, W, x, g64, img64, part, s))) return rc;
+    if ((rc = run_passes_forward(p, x, g32, true, 0, nullptr, s))) return rc;
+    if ((rc = pnh_c64_to_c128(g32, g64, p->Ktot * p->nb, s))) return rc;
+    if (p->M > 0 && (rc = do_k2y(p, 1, g64, E, nullptr, s))) return rc;
+    if ((rc = pnh_cdiv_c128(W, E, my, s))) return rc;
+  }
+  if ((rc = pnh_cmul_c128_c64(W, y, Wy, my, s))) return rc;
+  return adjoint_c128(p, Wy, x, g64, img64, part, s);
+}
+
+int pnh_solve_l1tvols(const pnh_plan* p, const void* y, void* x, int maxiter, double rho,
+                      void* ws, void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(maxiter >= 0, "maxiter must be >= 0");
+  REQUIRE(p->nb == 1, "L1TVOLS solves one coil (plan without batch)");
+  REQUIRE(rho > 0.0, "rho must be > 0");
+  int rc = check_dtype_ready(p, PNH_C64);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t N = p->Ntot, K = p->Ktot;
+  const size_t img = align_up((size_t)N * 8);
+  char* w = (char*)ws;
+  float* uker = (float*)w; w += align_up((size_t)K * 4);
+  float* wabs = (float*)w; w += align_up((size_t)K * 4);
+  void* AHy = w; w += img;
+  void* AHyk = w; w += img;
+  void* rhs = w; w += img;
+  void* q = w; w += img;
+  void* dd = w; w += p->D * img;
+  void* bb = w; w += p->D * img;
+  char* sws = w;          // self-adjoint scratch: grid first
+  const float mu = 1.0f, lam = (float)(rho * 1.0);
+  ShapeArgs gk, gn;
+  gk.D = gn.D = p->D;
+  for (int a = 0; a < 3; ++a) {
+    gk.dim[a] = a < p->D ? p->K[a] : 1;
+    gn.dim[a] = a < p->D ? p->N[a] : 1;
+  }
+  gk.n = K;
+  gn.n = N;
+  if ((rc = density(p, wabs, sws, s))) return rc;
+  if ((rc = pnh_l1tv_uker(wabs, uker, gk, mu, lam, s))) return rc;
+  if ((rc = pnh_adjoint(p, y, AHy, sws, s))) return rc;
+  if ((rc = pnh_fill_c64(AHyk, N, 0.f, 0.f, s))) return rc;
+  if ((rc = pnh_fill_c64(dd, p->D * N, 0.f, 0.f, s))) return rc;
+  if ((rc = pnh_fill_c64(bb, p->D * N, 0.f, 0.f, s))) return rc;
+  if ((rc = pnh_fill_c64(x, N, 0.f, 0.f, s))) return rc;
+  const float thr = (float)(1.0 / (double)lam);
+  for (int it = 0; it < maxiter; ++it) {
+    if ((rc = pnh_l1tv_rhs(AHyk, dd, bb, rhs, gn, mu, lam, s))) return rc;
+    if ((rc = run_passes_forward(p, rhs, sws, false, 0, nullptr, s))) return rc;
+    if ((rc = pnh_l1tv_kdiv(sws, uker, K, s))) return rc;
+    if ((rc = run_passes_inverse(p, sws, x, false, 0, nullptr, s))) return rc;
+    if ((rc = pnh_selfadjoint(p, x, q, sws, s))) return rc;
+    if ((rc = pnh_l1tv_shrink(x, q, AHy, AHyk, dd, bb, gn, thr, s))) return rc;
+  }
+  return 0;
 }
 
 int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* om_grad,

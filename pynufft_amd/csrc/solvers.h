@@ -1,0 +1,39 @@
+// Element-wise kernels of the Toeplitz self-adjoint and of the 'dc' and
+// 'L1TVOLS' solvers (complex64; host launch wrappers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// A C-order array of D <= 3 dimensions (n = prod(dim)).
+struct ShapeArgs {
+  int D;
+  int64_t dim[3];
+  int64_t n;
+};
+
+// p[i] = re + i im
+int pnh_fill_c64(void* p, int64_t n, float re, float im, hipStream_t s);
+// w[i] /= e[i] (complex, Smith's scaling)
+int pnh_cdiv_c64(void* w, const void* e, int64_t n, hipStream_t s);
+// complex128 variants: fill, w /= e, out = a (c128) * b (c64)
+int pnh_fill_c128(void* p, int64_t n, double re, double im, hipStream_t s);
+int pnh_cdiv_c128(void* w, const void* e, int64_t n, hipStream_t s);
+int pnh_cmul_c128_c64(const void* a, const void* b, void* out, int64_t n, hipStream_t s);
+// out = complex64(in) * sn[i / nb] (complex128 image -> apodized complex64)
+int pnh_cast_mul_sn(const void* in, void* out, const float* sn, int64_t n_img, int nb,
+                    hipStream_t s);
+// out[i] = a[i] * b[i]
+int pnh_cmul_c64(const void* a, const void* b, void* out, int64_t n, hipStream_t s);
+// w[i] = |k[i * nb]| (first coil)
+int pnh_abs_first_coil(const void* k, float* w, int64_t n, int nb, hipStream_t s);
+// uker = mu w - lam * (sum_d 2 cos(2 pi k_d / K_d) - 2) on the grid sh
+int pnh_l1tv_uker(const float* w, float* uker, const ShapeArgs& sh, float mu, float lam,
+                  hipStream_t s);
+// k = (k + 1e-7) / (uker + 1e-7)
+int pnh_l1tv_kdiv(void* k, const float* uker, int64_t n, hipStream_t s);
+// rhs = mu AHyk + lam sum_p D+_p(dd_p - bb_p) (periodic forward differences)
+int pnh_l1tv_rhs(const void* AHyk, const void* dd, const void* bb, void* rhs,
+                 const ShapeArgs& sh, float mu, float lam, hipStream_t s);
+// gradient D-_p x, shrinkage by thr, Bregman update of dd/bb, AHyk -= q - AHy
+int pnh_l1tv_shrink(const void* x, const void* q, const void* AHy, void* AHyk, void* dd,
+                    void* bb, const ShapeArgs& sh, float thr, hipStream_t s);

@@ -78,6 +78,7 @@ class NUFFT:
         self._plan = None
         self._ws = {}
         self._sense = None
+        self.W = None
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
@@ -118,6 +119,7 @@ class NUFFT:
         self.batch = None if batch is None else int(batch)
         self._nb = 1 if batch is None else int(batch)
         self._sense = None
+        self.W = None
         M = int(st['M'])
         self.M = (M,)
         self.Kdprod = int(np.prod(Kd))
@@ -461,20 +463,31 @@ class NUFFT:
         return self._out(out, is_np)
 
     def solve(self, y, solver=None, *args, **kwargs):
-        """Solve with ``solver='cg'`` (reference ``linalg/solve_cpu.py:185-288``).
+        """Solve for the image (reference ``linalg/solve_cpu.py:185-288``).
 
-        CG on the k-space normal equations P^H P k = P^H y with scipy's
-        semantics (x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the start
-        of an iteration, ``maxiter`` default 10 * prod(Kd)), all in FP64 on the
-        device; returns crop(ifftn(k)) / sn.  After the call ``self.cg_iters``
-        holds the iterations used (per coil with ``batch``).
+        * ``'cg'``: CG on the k-space normal equations P^H P k = P^H y with
+          scipy's semantics (x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at
+          the start of an iteration, ``maxiter`` default 10 * prod(Kd)), all in
+          FP64 on the device; returns crop(ifftn(k)) / sn.  After the call
+          ``self.cg_iters`` holds the iterations used (per coil with
+          ``batch``).
+        * ``'dc'``: density-compensated adjoint, ``maxiter`` pipe-density
+          iterations (``solve_cpu.py:165-183``).
+        * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
+          ``maxiter`` outer iterations, regularisation ``rho``
+          (``solve_cpu.py:42-163``; plans without ``batch``).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg')
+            raise KeyError('solver must be one of: cg, dc, L1TVOLS')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
+        if solver == 'dc':
+            return self._solve_dc(y, *args, **kwargs)
+        if solver == 'L1TVOLS':
+            return self._solve_l1tvols(y, *args, **kwargs)
         if solver != 'cg':
-            raise NotImplementedError("solver %r is not provided; use 'cg'" % (solver,))
+            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc' or 'L1TVOLS'"
+                                      % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -493,6 +506,61 @@ class NUFFT:
         self._call(self._libh.pnh_solve_cg, _ptr(yt), _ptr(out), int(maxiter),
                    float(rtol), float(atol), _ptr(ws), _ptr(iters), self._stream())
         self._cg_iters_dev = iters
+        return self._out(out, is_np)
+
+    def _solve_dc(self, y, maxiter):
+        """x = adjoint(W y), W from ``maxiter`` pipe-density iterations
+        (complex128 weights, FP64 interpolation, as the reference)."""
+        self._need_plan()
+        self._ensure_fp64()
+        if int(maxiter) < 0:
+            raise ValueError('maxiter must be >= 0')
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_DC, _lib.C64)
+        self._call(self._libh.pnh_solve_dc, _ptr(yt), _ptr(out), int(maxiter), _ptr(ws),
+                   self._stream())
+        return self._out(out, is_np)
+
+    def _solve_l1tvols(self, y, maxiter, rho):
+        """Split-Bregman L1-TV least squares (mu = 1, lambda = rho)."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('L1TVOLS solves plans without batch')
+        if int(maxiter) < 0:
+            raise ValueError('maxiter must be >= 0')
+        if not float(rho) > 0:
+            raise ValueError('rho must be > 0')
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_L1TVOLS, _lib.C64)
+        self._call(self._libh.pnh_solve_l1tvols, _ptr(yt), _ptr(out), int(maxiter),
+                   float(rho), _ptr(ws), self._stream())
+        return self._out(out, is_np)
+
+    def precompute_sp(self):
+        """Toeplitz weights W = |xx2k(adjoint(1))| on the Kd grid (float32,
+        first coil), kept as ``self.W`` (reference ``_precompute_sp_cpu``,
+        ``_nufft_class_methods_cpu.py:126-145``)."""
+        self._need_plan()
+        W = torch.empty(self.Kd, dtype=torch.float32, device=self.device)
+        ws = self._workspace(_lib.OP_PRECOMPUTE_SP, _lib.C64)
+        self._call(self._libh.pnh_precompute_sp, _ptr(W), _ptr(ws), self._stream())
+        self.W = W
+        return W
+
+    def selfadjoint2(self, x):
+        """k2xx(W * xx2k(x)) with the Toeplitz weights, no apodization
+        (reference ``_selfadjoint2_cpu``, ``:216-222``); computes W first
+        when missing."""
+        self._need_plan()
+        if getattr(self, 'W', None) is None:
+            self.precompute_sp()
+        xt, is_np = self._in(x, 'x')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SELFADJOINT2, _lib.C64)
+        self._call(self._libh.pnh_selfadjoint2, _ptr(xt), _ptr(self.W), _ptr(out), _ptr(ws),
+                   self._stream())
         return self._out(out, is_np)
 
     @property

@@ -125,3 +125,35 @@ def test_plan_argument_errors():
         oracle.plan(om, (8, 8), (16, 16), (4,))
     with pytest.raises(TypeError):
         oracle.kb_shape(17, 2.0)
+
+
+class _FftNoise(oracle.OracleNUFFT):
+    """Oracle whose complex64 FFT output carries a relative perturbation of
+    size eps (the spread between two complex64 FFT implementations)."""
+
+    def __init__(self, eps, seed):
+        super().__init__()
+        self.eps = eps
+        self.rng = np.random.default_rng(seed)
+
+    def xx2k(self, xx):
+        k = super().xx2k(xx)
+        scale = self.eps * np.sqrt(np.mean(np.abs(k) ** 2))
+        noise = self.rng.standard_normal(k.shape) + 1j * self.rng.standard_normal(k.shape)
+        return (k + scale * noise).astype(np.complex64)
+
+
+def test_dc_conditioning():
+    """How far the reference's 3-iteration pipe-density solve moves when its
+    complex64 FFT output changes by 1e-7 relative: well conditioned on the
+    tiny cases, amplified on the plain 2-D trajectory (the reason the device
+    'dc' comparison for that case is held to 2e-4 rather than 1e-5)."""
+    s = golden('solvers')
+    moved = {}
+    for name in ['tiny2d', 'tiny3d', 'plain2d']:
+        g = golden(name)
+        A = _FftNoise(1e-7, 1)
+        A.plan(g['om'], *geometry(g))
+        moved[name] = rel(A.solve(g['adjoint_in'], 'dc', maxiter=3), s[name + '__dc3'])
+    assert moved['tiny2d'] < 1e-6 and moved['tiny3d'] < 1e-6
+    assert 1e-5 < moved['plain2d'] < 2e-4, moved
