@@ -18,6 +18,7 @@ Error behaviour follows the reference: non-tuple Nd/Kd/Jd raise TypeError,
 length mismatches raise KeyError (helper.py:644-654).
 """
 import concurrent.futures
+import hashlib
 import math
 import os
 
@@ -334,11 +335,80 @@ def coefficient_bound(ud):
     return float(prod.max()) * (1.0 + 2.0 ** -20)
 
 
-def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None):
+_CACHE_VERSION = 1
+
+
+def _cache_file(cache_dir, om, Nd, Kd, Jd):
+    h = hashlib.sha256()
+    h.update(repr((_CACHE_VERSION, tuple(Nd), tuple(Kd), tuple(Jd), om.shape)).encode())
+    h.update(np.ascontiguousarray(om).tobytes())
+    return os.path.join(cache_dir, 'plan_%s.npz' % h.hexdigest()[:32])
+
+
+def _design_load(path, dd):
+    """The geometry part of a plan (alpha, beta, sn_d, factors, first cells)
+    from a cache file, or None when it is missing or unreadable."""
+    try:
+        with np.load(path, allow_pickle=False) as f:
+            return {'alpha': [f['alpha_%d' % a] for a in range(dd)],
+                    'beta': [int(f['beta_%d' % a]) for a in range(dd)],
+                    'snd': [f['snd_%d' % a] for a in range(dd)],
+                    'ud': [f['ud_%d' % a] for a in range(dd)],
+                    'base': f['base']}
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def _design_store(path, design):
+    arrays = {'base': design['base']}
+    for a in range(len(design['ud'])):
+        arrays['alpha_%d' % a] = np.asarray(design['alpha'][a])
+        arrays['beta_%d' % a] = np.asarray(design['beta'][a])
+        arrays['snd_%d' % a] = np.asarray(design['snd'][a])
+        arrays['ud_%d' % a] = design['ud'][a]
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = '%s.tmp%d.npz' % (path[:-4], os.getpid())
+        np.savez(tmp, **arrays)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def _design(om, Nd, Kd, Jd):
+    """Per-axis min-max design in FP64: alpha/beta, sn_d, factors u_d and
+    first support cells (reference helper.py:620-802 up to the CSR build)."""
+    dd = len(Nd)
+    M = om.shape[0]
+    out = {'alpha': [], 'beta': [], 'snd': [], 'ud': []}
+    based = []
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for a in range(dd):
+            alpha, beta = fit_scaling(Nd[a], Jd[a], Kd[a])
+            out['alpha'].append(alpha)
+            out['beta'].append(beta)
+            out['snd'].append(scaling_1d(Nd[a], Kd[a], alpha, beta))
+            Tinv = overlap_pinv(Nd[a], Jd[a], Kd[a], alpha, beta)
+            u, b = minmax_axis(om[:, a], Nd[a], Jd[a], Kd[a], alpha, beta, Tinv,
+                               pool=pool if M > (1 << 17) else None)
+            out['ud'].append(u)
+            based.append(b)
+    out['base'] = np.stack(based, axis=0) if M else np.zeros((dd, 0), np.int64)
+    return out
+
+
+def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None, cache_dir=None):
     """FP64 host plan: per-axis design, per-sample factors, sort and bins.
 
     Mirrors ``helper.plan(om, Nd, Kd, Jd, ft_axes)`` (reference
     helper.py:620-802).  Returns a dict; arrays are numpy.
+
+    ``cache_dir`` (default: the ``PYNUFFT_AMD_PLAN_CACHE`` environment
+    variable, unset = no cache) keeps the FP64 design of each geometry
+    (trajectory bytes, Nd, Kd, Jd) in an ``.npz`` file there and reuses it
+    on the next plan of the same geometry; the row sort and work list are
+    always rebuilt.
     """
     if type(Nd) != tuple:
         raise TypeError('Nd must be tuple, e.g. (256, 256)')
@@ -365,21 +435,19 @@ def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None):
         if Jd[a] > Kd[a]:
             raise ValueError('Jd must be <= Kd on every axis')
     M = om.shape[0]
-    st = {'Nd': Nd, 'Kd': Kd, 'Jd': Jd, 'M': np.int32(M), 'om': om,
-          'tol': 0, 'alpha': [], 'beta': []}
-    snd, ud, based = [], [], []
-    workers = max(1, min(16, len(os.sched_getaffinity(0))))
-    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
-        for a in range(dd):
-            alpha, beta = fit_scaling(Nd[a], Jd[a], Kd[a])
-            st['alpha'].append(alpha)
-            st['beta'].append(beta)
-            snd.append(scaling_1d(Nd[a], Kd[a], alpha, beta))
-            Tinv = overlap_pinv(Nd[a], Jd[a], Kd[a], alpha, beta)
-            u, b = minmax_axis(om[:, a], Nd[a], Jd[a], Kd[a], alpha, beta, Tinv,
-                               pool=pool if M > (1 << 17) else None)
-            ud.append(u)
-            based.append(b)
+    st = {'Nd': Nd, 'Kd': Kd, 'Jd': Jd, 'M': np.int32(M), 'om': om, 'tol': 0}
+    if cache_dir is None:
+        cache_dir = os.environ.get('PYNUFFT_AMD_PLAN_CACHE') or None
+    design = None
+    if cache_dir is not None:
+        path = _cache_file(cache_dir, om, Nd, Kd, Jd)
+        design = _design_load(path, dd)
+    if design is None:
+        design = _design(om, Nd, Kd, Jd)
+        if cache_dir is not None:
+            _design_store(path, design)
+    st['alpha'], st['beta'] = design['alpha'], design['beta']
+    snd, ud, base = design['snd'], design['ud'], design['base']
     # full scaling factor = real part of the outer product (helper.py:558-584)
     sn = np.reshape(1.0, (1,) * dd)
     for a in range(dd):
@@ -388,7 +456,6 @@ def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None):
         sn = sn * np.reshape(snd[a], shp)
     st['sn'] = sn.real
     st['snd'] = snd
-    base = np.stack(based, axis=0) if M else np.zeros((dd, 0), np.int64)
     tile = check_tile(tile, Kd, Jd) if tile is not None else tile_shape(Kd, Jd)
     perm, items, tile_slots = sort_and_bin(base, Kd, Jd, tile, chunk_rows)
     st.update(ud=ud, base=base, perm=perm, tile=tile, items=items,

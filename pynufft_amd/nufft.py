@@ -82,7 +82,7 @@ class NUFFT:
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
-             tile=None, chunk=None, host_plan=None, threads=0, tune=False):
+             tile=None, chunk=None, host_plan=None, threads=0, tune=False, cache_dir=None):
         """Design the interpolator on the host in FP64 and upload it.
 
         Reference ``_plan_cpu`` (``_nufft_class_methods_cpu.py:27-124``) and
@@ -97,11 +97,14 @@ class NUFFT:
         ``tune=True`` measures candidates on the device and keeps the fastest
         (``pynufft_amd.autotune``).
         ``host_plan`` may pass a ``helper.plan`` result computed elsewhere
-        (e.g. in a thread pool).  Returns 0.
+        (e.g. in a thread pool).  ``cache_dir`` (or the
+        ``PYNUFFT_AMD_PLAN_CACHE`` environment variable) keeps the FP64
+        design per geometry on disk (``helper.plan``).  Returns 0.
         """
         self.release()
         st = host_plan if host_plan is not None else \
-            helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_rows=chunk)
+            helper.plan(om, Nd, Kd, Jd, ft_axes=ft_axes, tile=tile, chunk_rows=chunk,
+                        cache_dir=cache_dir)
         if tune:
             from . import autotune
             cfg = autotune.tune(om, Nd, Kd, Jd, batch=batch, host_plan=st, device=self.device)

@@ -366,3 +366,35 @@ def test_fft_stockham_schedule(n, sign):
         ns *= R
     ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
     assert np.allclose(v, ref, rtol=1e-12, atol=1e-10 * np.abs(ref).max())
+
+
+def test_plan_cache_round_trip(tmp_path, monkeypatch):
+    """helper.plan(cache_dir=...) stores the FP64 design per geometry and a
+    second plan of the same geometry reuses it (no design computed) with
+    identical results; another trajectory or shape misses; an unreadable
+    file is recomputed."""
+    from pynufft_amd import helper
+    g = golden('tiny2d')
+    Nd, Kd, Jd = geometry(g)
+    a = helper.plan(g['om'], Nd, Kd, Jd, cache_dir=str(tmp_path))
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1
+
+    def no_design(*args):
+        raise AssertionError('design recomputed')
+    monkeypatch.setattr(helper, '_design', no_design)
+    b = helper.plan(g['om'], Nd, Kd, Jd, cache_dir=str(tmp_path))
+    for k in ('sn', 'base', 'perm', 'items', 'tile_slots'):
+        assert np.array_equal(a[k], b[k]), k
+    for d in range(len(Nd)):
+        assert np.array_equal(a['ud'][d], b['ud'][d])
+        assert np.array_equal(a['alpha'][d], b['alpha'][d]) and a['beta'][d] == b['beta'][d]
+    assert a['u_bound'] == b['u_bound']
+    with pytest.raises(AssertionError):
+        helper.plan(g['om'][:-1], Nd, Kd, Jd, cache_dir=str(tmp_path))
+    with pytest.raises(AssertionError):
+        helper.plan(g['om'], Nd, tuple(k + 2 for k in Kd), Jd, cache_dir=str(tmp_path))
+    monkeypatch.undo()
+    files[0].write_bytes(b'not a plan')
+    c = helper.plan(g['om'], Nd, Kd, Jd, cache_dir=str(tmp_path))
+    assert np.array_equal(a['ud'][0], c['ud'][0])
