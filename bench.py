@@ -166,35 +166,58 @@ def bench_batched(torch, W, maxiter=20, reps=3):
     }
 
 
-def bench_cg_radial(torch, W, maxiter=30, reps=3, cpu=True):
-    """Single-problem FP64 CG (radial 2-D 256^2/512^2, M 205 824, 30
-    iterations, rtol 0) on the GPU, and the CPU oracle's scipy CG."""
+def _time_host(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps
+
+
+def bench_radial_solvers(torch, W, maxiter=30, reps=3, cpu=True):
+    """Solvers on the radial 2-D config (256^2/512^2, M 205 824), GPU and
+    CPU oracle: the FP64 CG (30 iterations, rtol 0), 'dc' (3 pipe-density
+    iterations), 'L1TVOLS' (10 outer iterations, rho 1) and the Toeplitz
+    selfadjoint2 next to selfadjoint.  Returns (cg, others)."""
     from pynufft_amd.nufft import NUFFT
     w = W.radial2d()
     A = NUFFT()
     A.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
     x = torch.from_numpy(w['x']).to(A.device)
     y = A.forward(x)
-    A.solve(y, 'cg', maxiter=maxiter, rtol=0.0)
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for _ in range(reps):
-        A.solve(y, 'cg', maxiter=maxiter, rtol=0.0)
-    torch.cuda.synchronize()
-    t_gpu = (time.perf_counter() - t) / reps
-    out = {'workload': 'radial2d', 'cg_iterations': maxiter, 'gpu_solves_per_s': 1.0 / t_gpu,
-           'gpu_ms_per_solve': 1000.0 * t_gpu}
+    A.precompute_sp()
+    t_cg = _time_host(torch, lambda: A.solve(y, 'cg', maxiter=maxiter, rtol=0.0), reps)
+    cg = {'workload': 'radial2d', 'cg_iterations': maxiter, 'gpu_solves_per_s': 1.0 / t_cg,
+          'gpu_ms_per_solve': 1000.0 * t_cg}
+    runs = {
+        'dc3': (lambda: A.solve(y, 'dc', maxiter=3), lambda B, yc, xc: B.solve(yc, 'dc', maxiter=3)),
+        'l1tvols10_rho1': (lambda: A.solve(y, 'L1TVOLS', maxiter=10, rho=1.0),
+                           lambda B, yc, xc: B.solve(yc, 'L1TVOLS', maxiter=10, rho=1.0)),
+        'selfadjoint2': (lambda: A.selfadjoint2(x), lambda B, yc, xc: B.selfadjoint2(xc)),
+        'selfadjoint': (lambda: A.selfadjoint(x), lambda B, yc, xc: B.selfadjoint(xc)),
+    }
+    others = {'workload': 'radial2d'}
+    for name, (fn, _) in runs.items():
+        others[name] = {'gpu_ms': 1000.0 * _time_host(torch, fn, 5)}
     if cpu:
         import oracle
         B = oracle.OracleNUFFT()
         B.plan(w['om'], w['Nd'], w['Kd'], w['Jd'])
-        yc = y.cpu().numpy()
+        yc, xc = y.cpu().numpy(), w['x']
+        B.precompute_sp()
         t = time.perf_counter()
         B.solve(yc, 'cg', maxiter=maxiter, rtol=0.0)
-        t_cpu = time.perf_counter() - t
-        out['cpu_oracle_solves_per_s'] = 1.0 / t_cpu
-        out['cpu_cores'] = 1
-    return out
+        cg['cpu_oracle_solves_per_s'] = 1.0 / (time.perf_counter() - t)
+        cg['cpu_cores'] = 1
+        for name, (_, fc) in runs.items():
+            t = time.perf_counter()
+            fc(B, yc, xc)
+            others[name]['cpu_oracle_ms'] = 1000.0 * (time.perf_counter() - t)
+        others['cpu_cores'] = 1
+    A.release()
+    return cg, others
 
 
 def bench_other_workloads(torch, W, steps=20):
@@ -323,9 +346,9 @@ def main():
         batched = bench_batched(torch, W)
         progress('batched done')
 
-    progress('single-problem CG (radial 2-D)')
-    cg_single = bench_cg_radial(torch, W, cpu=not args.no_cpu_baseline) \
-        if not args.no_batched else None
+    progress('single-problem solvers (radial 2-D)')
+    cg_single, solvers = bench_radial_solvers(torch, W, cpu=not args.no_cpu_baseline) \
+        if not args.no_batched else (None, None)
     others = None
     if not args.no_batched:
         progress('other workloads')
@@ -357,6 +380,7 @@ def main():
         'plan_s': t_plan,
         'batched_cg': batched,
         'cg_single': cg_single,
+        'solvers_radial2d': solvers,
         'other_workloads': others,
         'cpu_baseline': cpu,
     }
