@@ -49,7 +49,8 @@ enum pnh_op {
   PNH_OP_PRECOMPUTE_SP = 8,
   PNH_OP_SELFADJOINT2 = 9,
   PNH_OP_SOLVE_DC = 10,
-  PNH_OP_SOLVE_L1TVOLS = 11
+  PNH_OP_SOLVE_L1TVOLS = 11,
+  PNH_OP_SOLVE_LSQR = 12
 };
 
 /*
@@ -193,6 +194,14 @@ int pnh_solve_dc(const pnh_plan* plan, const void* y, void* x, int maxiter, void
  * (Laplacian kernel, periodic difference maps). */
 int pnh_solve_l1tvols(const pnh_plan* plan, const void* y, void* x, int maxiter, double rho,
                       void* ws, void* stream);
+/* solve(y, 'lsqr', ...): scipy.sparse.linalg.lsqr on A = P (k2y, adjoint
+ * y2k) with k-space unknowns, x0 = 0, in FP64 on the device (batch 1), then
+ * x = crop(ifftn(k)) / sn.  iter_lim <= 0 means scipy's default 2 prod(Kd).
+ * info_out (device int32[2], may be NULL) receives istop and the iteration
+ * count.  Reference linalg/solve_cpu.py:236-256 and scipy's lsqr. */
+int pnh_solve_lsqr(const pnh_plan* plan, const void* y, void* x, int iter_lim, double damp,
+                   double atol, double btol, double conlim, void* ws, int32_t* info_out,
+                   void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

@@ -369,6 +369,15 @@ static size_t dc_ws_bytes(const pnh_plan* p) {
          align_up(partial_bytes(p, 1));
 }
 
+// Scratch of the LSQR solve: u and A v (complex128 [M]), v, w, x and A^H u
+// (complex128 [K]), FP64 gridding partials, the complex128 image, block
+// partials and the scalar state.
+static size_t lsqr_ws_bytes(const pnh_plan* p) {
+  return 2 * align_up((size_t)p->M * 16) + 4 * align_up((size_t)p->Ktot * 16) +
+         align_up(partial_bytes(p, 1)) + align_up((size_t)p->Ntot * 16) +
+         align_up((size_t)PNH_LSQR_BLOCKS * 8) + align_up(sizeof(LsqrState));
+}
+
 int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
   REQUIRE(p && bytes, "null argument");
   const int fp64 = dtype == PNH_C128;
@@ -395,6 +404,9 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
     case PNH_OP_SOLVE_DC: *bytes = dc_ws_bytes(p); break;
     case PNH_OP_SOLVE_L1TVOLS:
       *bytes = l1tv_ws_bytes(p);
+      break;
+    case PNH_OP_SOLVE_LSQR:
+      *bytes = lsqr_ws_bytes(p);
       break;
     case PNH_OP_SOLVE_CG: {
       const size_t g2 = align_up((size_t)p->Ktot * p->nb * 16);
@@ -634,6 +646,70 @@ int pnh_solve_l1tvols(const pnh_plan* p, const void* y, void* x, int maxiter, do
     if ((rc = run_passes_inverse(p, sws, x, false, 0, nullptr, s))) return rc;
     if ((rc = pnh_selfadjoint(p, x, q, sws, s))) return rc;
     if ((rc = pnh_l1tv_shrink(x, q, AHy, AHyk, dd, bb, gn, thr, s))) return rc;
+  }
+  return 0;
+}
+
+int pnh_solve_lsqr(const pnh_plan* p, const void* y, void* x, int iter_lim, double damp,
+                   double atol, double btol, double conlim, void* ws, int32_t* info_out,
+                   void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(p->nb == 1, "lsqr solves one coil (plan without batch)");
+  REQUIRE(damp >= 0.0, "damp must be >= 0");
+  int rc = check_dtype_ready(p, PNH_C128);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t M = p->M, K = p->Ktot;
+  if (iter_lim <= 0) iter_lim = (int)(2 * K < (1LL << 30) ? 2 * K : (1LL << 30));
+  char* w = (char*)ws;
+  void* u = w; w += align_up((size_t)M * 16);
+  void* tM = w; w += align_up((size_t)M * 16);
+  void* v = w; w += align_up((size_t)K * 16);
+  void* wv = w; w += align_up((size_t)K * 16);
+  void* xk = w; w += align_up((size_t)K * 16);
+  void* tK = w; w += align_up((size_t)K * 16);
+  void* part = w; w += align_up(partial_bytes(p, 1));
+  void* img64 = w; w += align_up((size_t)p->Ntot * 16);
+  double* red = (double*)w; w += align_up((size_t)PNH_LSQR_BLOCKS * 8);
+  LsqrState* st = (LsqrState*)w;
+  const int* done = &st->done;
+  const int* skip_v = &st->skip_v;
+  // beta u = b, alfa v = A^H u, w = v, x = 0
+  if ((rc = pnh_lsqr_init(st, atol, btol, conlim, damp, iter_lim, s))) return rc;
+  if ((rc = pnh_fill_c128(xk, K, 0.0, 0.0, s))) return rc;
+  if (M > 0 && (rc = pnh_c64_to_c128(y, u, M, s))) return rc;
+  if ((rc = pnh_lsqr_norm2(u, M, red, nullptr, s))) return rc;
+  if ((rc = pnh_lsqr_scalar(st, red, 0, s))) return rc;
+  if ((rc = pnh_lsqr_scale(u, &st->beta, M, nullptr, done, s))) return rc;
+  if ((rc = do_y2k(p, 1, u, v, part, done, s))) return rc;
+  if ((rc = pnh_lsqr_norm2(v, K, red, done, s))) return rc;
+  if ((rc = pnh_lsqr_scalar(st, red, 1, s))) return rc;
+  if ((rc = pnh_lsqr_scale(v, &st->alfa, K, wv, done, s))) return rc;
+  for (int it = 0; it < iter_lim; ++it) {
+    // beta u = A v - alfa u; alfa v = A^H u - beta v
+    if (M > 0 && (rc = do_k2y(p, 1, v, tM, done, s))) return rc;
+    if ((rc = pnh_lsqr_axpby(u, tM, &st->alfa, M, red, done, s))) return rc;
+    if ((rc = pnh_lsqr_scalar(st, red, 2, s))) return rc;
+    if ((rc = pnh_lsqr_scale(u, &st->beta, M, nullptr, skip_v, s))) return rc;
+    if ((rc = do_y2k(p, 1, u, tK, part, skip_v, s))) return rc;
+    if ((rc = pnh_lsqr_axpby(v, tK, &st->beta, K, red, skip_v, s))) return rc;
+    if ((rc = pnh_lsqr_scalar(st, red, 3, s))) return rc;
+    if ((rc = pnh_lsqr_scale(v, &st->alfa, K, nullptr, skip_v, s))) return rc;
+    if ((rc = pnh_lsqr_xw(xk, wv, v, st, K, red, s))) return rc;
+    if ((rc = pnh_lsqr_scalar(st, red, 4, s))) return rc;
+    if ((it & 63) == 63 && it + 1 < iter_lim) {
+      // long runs: stop launching once the device has converged
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
+  }
+  if ((rc = run_passes_inverse(p, xk, img64, false, 1, nullptr, s))) return rc;
+  if ((rc = pnh_cast_div_sn(img64, x, p->d.sn_f32, p->Ntot, 1, s))) return rc;
+  if (info_out) {
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->istop, sizeof(int), hipMemcpyDeviceToDevice, s));
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->itn, sizeof(int), hipMemcpyDeviceToDevice, s));
   }
   return 0;
 }

@@ -37,3 +37,30 @@ int pnh_l1tv_rhs(const void* AHyk, const void* dd, const void* bb, void* rhs,
 // gradient D-_p x, shrinkage by thr, Bregman update of dd/bb, AHyk -= q - AHy
 int pnh_l1tv_shrink(const void* x, const void* q, const void* AHy, void* AHyk, void* dd,
                     void* bb, const ShapeArgs& sh, float thr, hipStream_t s);
+
+// ---- LSQR (scipy.sparse.linalg.lsqr semantics), FP64, scalars on the device
+#define PNH_LSQR_BLOCKS 512
+
+struct LsqrState {
+  double alfa, beta, rhobar, phibar, anorm, acond, ddnorm, res2, xnorm, xxnorm, z, cs2, sn2;
+  double bnorm, t1, t2, rho, theta, phi, psi, tau;
+  double atol, btol, ctol, damp;
+  int itn, iter_lim, istop, done, skip_v;
+};
+
+// part[b] = block partials of sum |a|^2 (PNH_LSQR_BLOCKS blocks)
+int pnh_lsqr_norm2(const void* a, int64_t n, double* part, const int* skip, hipStream_t s);
+// a = b - (*coef) a with partials of |a|^2
+int pnh_lsqr_axpby(void* a, const void* b, const double* coef, int64_t n, double* part,
+                   const int* skip, hipStream_t s);
+// a /= *norm when *norm > 0; copy (optional) = a
+int pnh_lsqr_scale(void* a, const double* norm, int64_t n, void* copy, const int* skip,
+                   hipStream_t s);
+// partials of |w / rho|^2, x += t1 w, w = v + t2 w
+int pnh_lsqr_xw(void* x, void* w, const void* v, const LsqrState* st, int64_t n, double* part,
+                hipStream_t s);
+// state reset with the caller's tolerances (ctol = 1 / conlim, 0 if conlim <= 0)
+int pnh_lsqr_init(LsqrState* st, double atol, double btol, double conlim, double damp,
+                  int iter_lim, hipStream_t s);
+// scalar recurrence step `phase` (see solvers.hip)
+int pnh_lsqr_scalar(LsqrState* st, const double* part, int phase, hipStream_t s);

@@ -253,3 +253,260 @@ int pnh_l1tv_shrink(const void* x, const void* q, const void* AHy, void* AHyk, v
          (float2*)AHyk, (float2*)dd, (float2*)bb, sh, thr);
   return 0;
 }
+
+// ------------------------------------------------------------------ LSQR --
+// scipy.sparse.linalg.lsqr (Paige & Saunders) on A = P (k2y), A^H = y2k, in
+// FP64 on the device.  Every scalar of the recurrence lives in LsqrState and
+// is updated by one thread; the vector kernels read what they need, so the
+// host only launches.  Reductions are deterministic (fixed grid, fixed tree).
+namespace {
+
+__device__ double sum_partials(const double* __restrict__ part) {
+  double s = 0.0;
+  for (int b = 0; b < PNH_LSQR_BLOCKS; ++b) s += part[b];
+  return s;
+}
+
+// per-block partial of sum |scale * a|^2 (scale == 0 means 1)
+__device__ void block_sum_store(double v, double* __restrict__ part) {
+  __shared__ double red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256)
+norm2_partials_kernel(const double2* __restrict__ a, int64_t n, double* __restrict__ part,
+                      const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) acc += a[i].x * a[i].x + a[i].y * a[i].y;
+  block_sum_store(acc, part);
+}
+
+// a = b - c * a (c = *coef), partial |a|^2
+__global__ void __launch_bounds__(256)
+lsqr_axpby_kernel(double2* __restrict__ a, const double2* __restrict__ b,
+                  const double* __restrict__ coef, int64_t n, double* __restrict__ part,
+                  const int* __restrict__ skip) {
+  if (*skip) return;
+  const double c = *coef;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 v = make_double2(b[i].x - c * a[i].x, b[i].y - c * a[i].y);
+    a[i] = v;
+    acc += v.x * v.x + v.y * v.y;
+  }
+  block_sum_store(acc, part);
+}
+
+// a *= 1 / *norm when *norm > 0; optionally copy the result to w
+__global__ void lsqr_scale_kernel(double2* __restrict__ a, const double* __restrict__ norm,
+                                  int64_t n, double2* __restrict__ w,
+                                  const int* __restrict__ skip) {
+  if (*skip) return;
+  const double nv = *norm;
+  const double inv = nv > 0.0 ? 1.0 / nv : 1.0;
+  GRID_STRIDE(i, n) {
+    double2 v = a[i];
+    if (nv > 0.0) { v.x *= inv; v.y *= inv; a[i] = v; }
+    if (w) w[i] = v;
+  }
+}
+
+// partial |w / rho|^2 (old w), x += t1 w, w = v + t2 w
+__global__ void __launch_bounds__(256)
+lsqr_xw_kernel(double2* __restrict__ x, double2* __restrict__ w, const double2* __restrict__ v,
+               const LsqrState* __restrict__ st, int64_t n, double* __restrict__ part) {
+  if (st->done) return;
+  const double t1 = st->t1, t2 = st->t2, ir = 1.0 / st->rho;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 wo = w[i];
+    const double dx = ir * wo.x, dy = ir * wo.y;
+    acc += dx * dx + dy * dy;
+    double2 xv = x[i];
+    xv.x += t1 * wo.x;
+    xv.y += t1 * wo.y;
+    x[i] = xv;
+    w[i] = make_double2(v[i].x + t2 * wo.x, v[i].y + t2 * wo.y);
+  }
+  block_sum_store(acc, part);
+}
+
+__device__ double sgn(double a) { return a > 0.0 ? 1.0 : (a < 0.0 ? -1.0 : 0.0); }
+
+// stable Givens rotation (scipy _sym_ortho)
+__device__ void sym_ortho(double a, double b, double& c, double& s, double& r) {
+  if (b == 0.0) { c = sgn(a); s = 0.0; r = fabs(a); }
+  else if (a == 0.0) { c = 0.0; s = sgn(b); r = fabs(b); }
+  else if (fabs(b) > fabs(a)) {
+    const double tau = a / b;
+    s = sgn(b) / sqrt(1.0 + tau * tau);
+    c = s * tau;
+    r = b / s;
+  } else {
+    const double tau = b / a;
+    c = sgn(a) / sqrt(1.0 + tau * tau);
+    s = c * tau;
+    r = a / c;
+  }
+}
+
+__global__ void lsqr_init_kernel(LsqrState* __restrict__ st, double atol, double btol,
+                                 double conlim, double damp, int iter_lim) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  LsqrState z = {};
+  z.atol = atol;
+  z.btol = btol;
+  z.ctol = conlim > 0.0 ? 1.0 / conlim : 0.0;
+  z.damp = damp;
+  z.iter_lim = iter_lim;
+  z.cs2 = -1.0;
+  *st = z;
+}
+
+// phase: 0 = bnorm/beta from |b|^2; 1 = alfa of the start vector (v = A^H u);
+// 2 = beta of an iteration; 3 = alfa of an iteration, then the rotation;
+// 4 = after the x/w update: norm estimates and the stopping tests.
+__global__ void lsqr_scalar_kernel(LsqrState* __restrict__ st, const double* __restrict__ part,
+                                   int phase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) return;
+  const double S = sum_partials(part);
+  if (phase == 0) {
+    st->bnorm = st->beta = sqrt(S);
+    if (!(st->beta > 0.0)) { st->alfa = 0.0; st->done = 1; st->skip_v = 1; }
+    else st->skip_v = 0;
+    return;
+  }
+  if (phase == 1) {
+    st->alfa = sqrt(S);
+    st->rhobar = st->alfa;
+    st->phibar = st->beta;
+    if (st->alfa * st->beta == 0.0) st->done = 1;
+    return;
+  }
+  if (phase == 2) {
+    st->itn += 1;
+    st->beta = sqrt(S);
+    if (st->beta > 0.0) {
+      st->anorm = sqrt(st->anorm * st->anorm + st->alfa * st->alfa + st->beta * st->beta +
+                       st->damp * st->damp);
+      st->skip_v = 0;
+    } else {
+      st->skip_v = 1;
+    }
+    return;
+  }
+  if (phase == 3) {
+    if (!st->skip_v) st->alfa = sqrt(S);
+    double rhobar1, psi;
+    if (st->damp > 0.0) {
+      rhobar1 = sqrt(st->rhobar * st->rhobar + st->damp * st->damp);
+      const double cs1 = st->rhobar / rhobar1, sn1 = st->damp / rhobar1;
+      psi = sn1 * st->phibar;
+      st->phibar = cs1 * st->phibar;
+    } else {
+      rhobar1 = st->rhobar;
+      psi = 0.0;
+    }
+    double cs, sn, rho;
+    sym_ortho(rhobar1, st->beta, cs, sn, rho);
+    const double theta = sn * st->alfa;
+    st->rhobar = -cs * st->alfa;
+    const double phi = cs * st->phibar;
+    st->phibar = sn * st->phibar;
+    st->tau = sn * phi;
+    st->t1 = phi / rho;
+    st->t2 = -theta / rho;
+    st->rho = rho;
+    st->theta = theta;
+    st->phi = phi;
+    st->psi = psi;
+    return;
+  }
+  // phase 4
+  const double eps = 2.220446049250313e-16;
+  st->ddnorm += S;
+  const double rho = st->rho, theta = st->theta, phi = st->phi;
+  const double delta = st->sn2 * rho;
+  const double gambar = -st->cs2 * rho;
+  const double rhs = phi - delta * st->z;
+  const double zbar = rhs / gambar;
+  st->xnorm = sqrt(st->xxnorm + zbar * zbar);
+  const double gamma = sqrt(gambar * gambar + theta * theta);
+  st->cs2 = gambar / gamma;
+  st->sn2 = theta / gamma;
+  st->z = rhs / gamma;
+  st->xxnorm += st->z * st->z;
+  st->acond = st->anorm * sqrt(st->ddnorm);
+  const double res1 = st->phibar * st->phibar;
+  st->res2 += st->psi * st->psi;
+  const double rnorm = sqrt(res1 + st->res2);
+  const double arnorm = st->alfa * fabs(st->tau);
+  const double test1 = rnorm / st->bnorm;
+  const double test2 = arnorm / (st->anorm * rnorm + eps);
+  const double test3 = 1.0 / (st->acond + eps);
+  const double t1 = test1 / (1.0 + st->anorm * st->xnorm / st->bnorm);
+  const double rtol = st->btol + st->atol * st->anorm * st->xnorm / st->bnorm;
+  int istop = 0;
+  if (st->itn >= st->iter_lim) istop = 7;
+  if (1.0 + test3 <= 1.0) istop = 6;
+  if (1.0 + test2 <= 1.0) istop = 5;
+  if (1.0 + t1 <= 1.0) istop = 4;
+  if (test3 <= st->ctol) istop = 3;
+  if (test2 <= st->atol) istop = 2;
+  if (test1 <= rtol) istop = 1;
+  st->istop = istop;
+  if (istop) { st->done = 1; st->skip_v = 1; }
+}
+
+}  // namespace
+
+int pnh_lsqr_norm2(const void* a, int64_t n, double* part, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(norm2_partials_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s,
+                     (const double2*)a, n, part, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsqr_axpby(void* a, const void* b, const double* coef, int64_t n, double* part,
+                   const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(lsqr_axpby_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (double2*)a,
+                     (const double2*)b, coef, n, part, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsqr_scale(void* a, const double* norm, int64_t n, void* copy, const int* skip,
+                   hipStream_t s) {
+  LAUNCH(lsqr_scale_kernel, n, (double2*)a, norm, n, (double2*)copy, skip);
+  return 0;
+}
+
+int pnh_lsqr_xw(void* x, void* w, const void* v, const LsqrState* st, int64_t n, double* part,
+                hipStream_t s) {
+  hipLaunchKernelGGL(lsqr_xw_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (double2*)x,
+                     (double2*)w, (const double2*)v, st, n, part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsqr_init(LsqrState* st, double atol, double btol, double conlim, double damp,
+                  int iter_lim, hipStream_t s) {
+  hipLaunchKernelGGL(lsqr_init_kernel, dim3(1), dim3(64), 0, s, st, atol, btol, conlim, damp,
+                     iter_lim);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsqr_scalar(LsqrState* st, const double* part, int phase, hipStream_t s) {
+  hipLaunchKernelGGL(lsqr_scalar_kernel, dim3(1), dim3(64), 0, s, st, part, phase);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}

@@ -479,18 +479,24 @@ class NUFFT:
         * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
           ``maxiter`` outer iterations, regularisation ``rho``
           (``solve_cpu.py:42-163``; plans without ``batch``).
+        * ``'lsqr'``: scipy's LSQR on P with k-space unknowns in FP64
+          (``solve_cpu.py:236-256``; options ``damp, atol, btol, conlim,
+          iter_lim``; plans without ``batch``).  ``self.lsqr_info`` then holds
+          (istop, iterations).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg, dc, L1TVOLS')
+            raise KeyError('solver must be one of: cg, dc, L1TVOLS, lsqr')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
         if solver == 'dc':
             return self._solve_dc(y, *args, **kwargs)
         if solver == 'L1TVOLS':
             return self._solve_l1tvols(y, *args, **kwargs)
+        if solver == 'lsqr':
+            return self._solve_lsqr(y, *args, **kwargs)
         if solver != 'cg':
-            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc' or 'L1TVOLS'"
-                                      % (solver,))
+            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc', 'L1TVOLS' "
+                                      "or 'lsqr'" % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -540,6 +546,30 @@ class NUFFT:
         self._call(self._libh.pnh_solve_l1tvols, _ptr(yt), _ptr(out), int(maxiter),
                    float(rho), _ptr(ws), self._stream())
         return self._out(out, is_np)
+
+    def _solve_lsqr(self, y, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8, iter_lim=None,
+                    show=False, calc_var=False, x0=None):
+        """scipy.sparse.linalg.lsqr semantics on P (k2y / y2k), x0 = 0."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('lsqr solves plans without batch')
+        if calc_var or x0 is not None:
+            raise NotImplementedError('lsqr: calc_var and x0 are not supported')
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_LSQR, _lib.C128)
+        info = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._call(self._libh.pnh_solve_lsqr, _ptr(yt), _ptr(out),
+                   int(iter_lim) if iter_lim is not None else 0, float(damp), float(atol),
+                   float(btol), float(conlim), _ptr(ws), _ptr(info), self._stream())
+        self._lsqr_info = info
+        return self._out(out, is_np)
+
+    @property
+    def lsqr_info(self):
+        """(istop, iterations) of the last 'lsqr' solve."""
+        return tuple(int(v) for v in self._lsqr_info.cpu().numpy())
 
     def precompute_sp(self):
         """Toeplitz weights W = |xx2k(adjoint(1))| on the Kd grid (float32,
