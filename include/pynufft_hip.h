@@ -50,7 +50,8 @@ enum pnh_op {
   PNH_OP_SELFADJOINT2 = 9,
   PNH_OP_SOLVE_DC = 10,
   PNH_OP_SOLVE_L1TVOLS = 11,
-  PNH_OP_SOLVE_LSQR = 12
+  PNH_OP_SOLVE_LSQR = 12,
+  PNH_OP_SOLVE_LSMR = 13
 };
 
 /*
@@ -200,6 +201,12 @@ int pnh_solve_l1tvols(const pnh_plan* plan, const void* y, void* x, int maxiter,
  * info_out (device int32[2], may be NULL) receives istop and the iteration
  * count.  Reference linalg/solve_cpu.py:236-256 and scipy's lsqr. */
 int pnh_solve_lsqr(const pnh_plan* plan, const void* y, void* x, int iter_lim, double damp,
+                   double atol, double btol, double conlim, void* ws, int32_t* info_out,
+                   void* stream);
+/* solve(y, 'lsmr', ...): scipy.sparse.linalg.lsmr on A = P, as
+ * pnh_solve_lsqr (maxiter <= 0 means scipy's default min(M, prod(Kd))).
+ * Reference linalg/solve_cpu.py:236-256 and scipy's lsmr. */
+int pnh_solve_lsmr(const pnh_plan* plan, const void* y, void* x, int maxiter, double damp,
                    double atol, double btol, double conlim, void* ws, int32_t* info_out,
                    void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in

@@ -408,6 +408,9 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
     case PNH_OP_SOLVE_LSQR:
       *bytes = lsqr_ws_bytes(p);
       break;
+    case PNH_OP_SOLVE_LSMR:
+      *bytes = lsqr_ws_bytes(p) + align_up((size_t)p->Ktot * 16) + align_up(sizeof(LsmrState));
+      break;
     case PNH_OP_SOLVE_CG: {
       const size_t g2 = align_up((size_t)p->Ktot * p->nb * 16);
       const size_t y2 = align_up((size_t)p->M * p->nb * 16);
@@ -699,6 +702,69 @@ int pnh_solve_lsqr(const pnh_plan* p, const void* y, void* x, int iter_lim, doub
     if ((rc = pnh_lsqr_scalar(st, red, 4, s))) return rc;
     if ((it & 63) == 63 && it + 1 < iter_lim) {
       // long runs: stop launching once the device has converged
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
+  }
+  if ((rc = run_passes_inverse(p, xk, img64, false, 1, nullptr, s))) return rc;
+  if ((rc = pnh_cast_div_sn(img64, x, p->d.sn_f32, p->Ntot, 1, s))) return rc;
+  if (info_out) {
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->istop, sizeof(int), hipMemcpyDeviceToDevice, s));
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->itn, sizeof(int), hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
+int pnh_solve_lsmr(const pnh_plan* p, const void* y, void* x, int maxiter, double damp,
+                   double atol, double btol, double conlim, void* ws, int32_t* info_out,
+                   void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(p->nb == 1, "lsmr solves one coil (plan without batch)");
+  REQUIRE(damp >= 0.0, "damp must be >= 0");
+  int rc = check_dtype_ready(p, PNH_C128);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t M = p->M, K = p->Ktot;
+  if (maxiter <= 0) maxiter = (int)(M < K ? (M > 0 ? M : 1) : K);
+  char* w = (char*)ws;
+  void* u = w; w += align_up((size_t)M * 16);
+  void* tM = w; w += align_up((size_t)M * 16);
+  void* v = w; w += align_up((size_t)K * 16);
+  void* h = w; w += align_up((size_t)K * 16);
+  void* hbar = w; w += align_up((size_t)K * 16);
+  void* xk = w; w += align_up((size_t)K * 16);
+  void* tK = w; w += align_up((size_t)K * 16);
+  void* part = w; w += align_up(partial_bytes(p, 1));
+  void* img64 = w; w += align_up((size_t)p->Ntot * 16);
+  double* red = (double*)w; w += align_up((size_t)PNH_LSQR_BLOCKS * 8);
+  LsmrState* st = (LsmrState*)w;
+  const int* done = &st->done;
+  const int* skip_v = &st->skip_v;
+  if ((rc = pnh_lsmr_init(st, atol, btol, conlim, damp, maxiter, s))) return rc;
+  if ((rc = pnh_fill_c128(xk, K, 0.0, 0.0, s))) return rc;
+  if ((rc = pnh_fill_c128(hbar, K, 0.0, 0.0, s))) return rc;
+  if (M > 0 && (rc = pnh_c64_to_c128(y, u, M, s))) return rc;
+  if ((rc = pnh_lsqr_norm2(u, M, red, nullptr, s))) return rc;
+  if ((rc = pnh_lsmr_scalar(st, red, 0, s))) return rc;
+  if ((rc = pnh_lsqr_scale(u, &st->beta, M, nullptr, done, s))) return rc;
+  if ((rc = do_y2k(p, 1, u, v, part, done, s))) return rc;
+  if ((rc = pnh_lsqr_norm2(v, K, red, done, s))) return rc;
+  if ((rc = pnh_lsmr_scalar(st, red, 1, s))) return rc;
+  if ((rc = pnh_lsqr_scale(v, &st->alpha, K, h, done, s))) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    if (M > 0 && (rc = do_k2y(p, 1, v, tM, done, s))) return rc;
+    if ((rc = pnh_lsqr_axpby(u, tM, &st->alpha, M, red, done, s))) return rc;
+    if ((rc = pnh_lsmr_scalar(st, red, 2, s))) return rc;
+    if ((rc = pnh_lsqr_scale(u, &st->beta, M, nullptr, skip_v, s))) return rc;
+    if ((rc = do_y2k(p, 1, u, tK, part, skip_v, s))) return rc;
+    if ((rc = pnh_lsqr_axpby(v, tK, &st->beta, K, red, skip_v, s))) return rc;
+    if ((rc = pnh_lsmr_scalar(st, red, 3, s))) return rc;
+    if ((rc = pnh_lsqr_scale(v, &st->alpha, K, nullptr, skip_v, s))) return rc;
+    if ((rc = pnh_lsmr_hx(xk, h, hbar, v, st, K, red, s))) return rc;
+    if ((rc = pnh_lsmr_scalar(st, red, 4, s))) return rc;
+    if ((it & 63) == 63 && it + 1 < maxiter) {
       int flag = 0;
       PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
       PNH_CHECK_HIP(hipStreamSynchronize(s));

@@ -64,3 +64,20 @@ int pnh_lsqr_init(LsqrState* st, double atol, double btol, double conlim, double
                   int iter_lim, hipStream_t s);
 // scalar recurrence step `phase` (see solvers.hip)
 int pnh_lsqr_scalar(LsqrState* st, const double* part, int phase, hipStream_t s);
+
+// ---- LSMR (scipy.sparse.linalg.lsmr semantics), FP64, scalars on the device
+struct LsmrState {
+  double alpha, beta, alphabar, rho, rhobar, cbar, sbar, zetabar, zeta;
+  double betadd, betad, rhodold, tautildeold, thetatilde, d;
+  double normA2, maxrbar, minrbar, normA, condA, normx, normb, normr;
+  double c1, c2, c3;
+  double atol, btol, ctol, damp;
+  int itn, maxiter, istop, done, skip_v;
+};
+
+int pnh_lsmr_init(LsmrState* st, double atol, double btol, double conlim, double damp,
+                  int maxiter, hipStream_t s);
+// hbar = h + c1 hbar; x += c2 hbar; h = v + c3 h; partials of |x|^2
+int pnh_lsmr_hx(void* x, void* h, void* hbar, const void* v, const LsmrState* st, int64_t n,
+                double* part, hipStream_t s);
+int pnh_lsmr_scalar(LsmrState* st, const double* part, int phase, hipStream_t s);

@@ -510,3 +510,182 @@ int pnh_lsqr_scalar(LsqrState* st, const double* part, int phase, hipStream_t s)
   PNH_LAUNCH_CHECK();
   return 0;
 }
+
+// ------------------------------------------------------------------ LSMR --
+// scipy.sparse.linalg.lsmr (Fong & Saunders) on the same bidiagonalization;
+// scalars in LsmrState, one thread.
+namespace {
+
+__global__ void lsmr_init_kernel(LsmrState* __restrict__ st, double atol, double btol,
+                                 double conlim, double damp, int maxiter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  LsmrState z = {};
+  z.atol = atol;
+  z.btol = btol;
+  z.ctol = conlim > 0.0 ? 1.0 / conlim : 0.0;
+  z.damp = damp;
+  z.maxiter = maxiter;
+  *st = z;
+}
+
+// hbar = h + c1 hbar; x += c2 hbar; h = v + c3 h; partials of |x|^2
+__global__ void __launch_bounds__(256)
+lsmr_hx_kernel(double2* __restrict__ x, double2* __restrict__ h, double2* __restrict__ hbar,
+               const double2* __restrict__ v, const LsmrState* __restrict__ st, int64_t n,
+               double* __restrict__ part) {
+  if (st->done) return;
+  const double c1 = st->c1, c2 = st->c2, c3 = st->c3;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 hv = h[i];
+    double2 hb = hbar[i];
+    hb = make_double2(hv.x + c1 * hb.x, hv.y + c1 * hb.y);
+    hbar[i] = hb;
+    double2 xv = x[i];
+    xv.x += c2 * hb.x;
+    xv.y += c2 * hb.y;
+    x[i] = xv;
+    acc += xv.x * xv.x + xv.y * xv.y;
+    h[i] = make_double2(v[i].x + c3 * hv.x, v[i].y + c3 * hv.y);
+  }
+  block_sum_store(acc, part);
+}
+
+// phase 0: normb, beta; 1: alpha of the start vector and the initial
+// scalars; 2: beta of an iteration; 3: alpha of an iteration and every
+// rotation and estimate except ||x||; 4: ||x|| and the stopping tests.
+__global__ void lsmr_scalar_kernel(LsmrState* __restrict__ st, const double* __restrict__ part,
+                                   int phase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) return;
+  const double S = sum_partials(part);
+  if (phase == 0) {
+    st->normb = st->beta = sqrt(S);
+    if (!(st->beta > 0.0)) { st->alpha = 0.0; st->done = 1; st->skip_v = 1; }
+    return;
+  }
+  if (phase == 1) {
+    st->alpha = sqrt(S);
+    st->zetabar = st->alpha * st->beta;
+    st->alphabar = st->alpha;
+    st->rho = 1.0;
+    st->rhobar = 1.0;
+    st->cbar = 1.0;
+    st->sbar = 0.0;
+    st->betadd = st->beta;
+    st->betad = 0.0;
+    st->rhodold = 1.0;
+    st->tautildeold = 0.0;
+    st->thetatilde = 0.0;
+    st->zeta = 0.0;
+    st->d = 0.0;
+    st->normA2 = st->alpha * st->alpha;
+    st->maxrbar = 0.0;
+    st->minrbar = 1e100;
+    st->normA = sqrt(st->normA2);
+    st->condA = 1.0;
+    st->normx = 0.0;
+    st->normr = st->beta;
+    if (st->alpha * st->beta == 0.0) st->done = 1;
+    return;
+  }
+  if (phase == 2) {
+    st->itn += 1;
+    st->beta = sqrt(S);
+    st->skip_v = st->beta > 0.0 ? 0 : 1;
+    return;
+  }
+  if (phase == 3) {
+    if (!st->skip_v) st->alpha = sqrt(S);
+    const double alpha = st->alpha, beta = st->beta;
+    double chat, shat, alphahat;
+    sym_ortho(st->alphabar, st->damp, chat, shat, alphahat);
+    const double rhoold = st->rho;
+    double c, s, rho;
+    sym_ortho(alphahat, beta, c, s, rho);
+    const double thetanew = s * alpha;
+    st->alphabar = c * alpha;
+    const double rhobarold = st->rhobar;
+    const double zetaold = st->zeta;
+    const double thetabar = st->sbar * rho;
+    const double rhotemp = st->cbar * rho;
+    double cbar, sbar, rhobar;
+    sym_ortho(st->cbar * rho, thetanew, cbar, sbar, rhobar);
+    st->cbar = cbar;
+    st->sbar = sbar;
+    st->rhobar = rhobar;
+    st->zeta = cbar * st->zetabar;
+    st->zetabar = -sbar * st->zetabar;
+    st->c1 = -(thetabar * rho / (rhoold * rhobarold));
+    st->c2 = st->zeta / (rho * rhobar);
+    st->c3 = -(thetanew / rho);
+    st->rho = rho;
+    // estimate of ||r||
+    const double betaacute = chat * st->betadd;
+    const double betacheck = -shat * st->betadd;
+    const double betahat = c * betaacute;
+    st->betadd = -s * betaacute;
+    const double thetatildeold = st->thetatilde;
+    double ctildeold, stildeold, rhotildeold;
+    sym_ortho(st->rhodold, thetabar, ctildeold, stildeold, rhotildeold);
+    st->thetatilde = stildeold * rhobar;
+    st->rhodold = ctildeold * rhobar;
+    st->betad = -stildeold * st->betad + ctildeold * betahat;
+    st->tautildeold = (zetaold - thetatildeold * st->tautildeold) / rhotildeold;
+    const double taud = (st->zeta - st->thetatilde * st->tautildeold) / st->rhodold;
+    st->d = st->d + betacheck * betacheck;
+    const double bt = st->betad - taud;
+    st->normr = sqrt(st->d + bt * bt + st->betadd * st->betadd);
+    // estimates of ||A|| and cond(A)
+    st->normA2 = st->normA2 + beta * beta;
+    st->normA = sqrt(st->normA2);
+    st->normA2 = st->normA2 + alpha * alpha;
+    st->maxrbar = fmax(st->maxrbar, rhobarold);
+    if (st->itn > 1) st->minrbar = fmin(st->minrbar, rhobarold);
+    st->condA = fmax(st->maxrbar, rhotemp) / fmin(st->minrbar, rhotemp);
+    return;
+  }
+  // phase 4
+  st->normx = sqrt(S);
+  const double normar = fabs(st->zetabar);
+  const double test1 = st->normr / st->normb;
+  const double test2 = (st->normA * st->normr) != 0.0 ? normar / (st->normA * st->normr)
+                                                      : INFINITY;
+  const double test3 = 1.0 / st->condA;
+  const double t1 = test1 / (1.0 + st->normA * st->normx / st->normb);
+  const double rtol = st->btol + st->atol * st->normA * st->normx / st->normb;
+  int istop = 0;
+  if (st->itn >= st->maxiter) istop = 7;
+  if (1.0 + test3 <= 1.0) istop = 6;
+  if (1.0 + test2 <= 1.0) istop = 5;
+  if (1.0 + t1 <= 1.0) istop = 4;
+  if (test3 <= st->ctol) istop = 3;
+  if (test2 <= st->atol) istop = 2;
+  if (test1 <= rtol) istop = 1;
+  st->istop = istop;
+  if (istop) { st->done = 1; st->skip_v = 1; }
+}
+
+}  // namespace
+
+int pnh_lsmr_init(LsmrState* st, double atol, double btol, double conlim, double damp,
+                  int maxiter, hipStream_t s) {
+  hipLaunchKernelGGL(lsmr_init_kernel, dim3(1), dim3(64), 0, s, st, atol, btol, conlim, damp,
+                     maxiter);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsmr_hx(void* x, void* h, void* hbar, const void* v, const LsmrState* st, int64_t n,
+                double* part, hipStream_t s) {
+  hipLaunchKernelGGL(lsmr_hx_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (double2*)x,
+                     (double2*)h, (double2*)hbar, (const double2*)v, st, n, part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lsmr_scalar(LsmrState* st, const double* part, int phase, hipStream_t s) {
+  hipLaunchKernelGGL(lsmr_scalar_kernel, dim3(1), dim3(64), 0, s, st, part, phase);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}

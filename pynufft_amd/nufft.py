@@ -479,13 +479,13 @@ class NUFFT:
         * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
           ``maxiter`` outer iterations, regularisation ``rho``
           (``solve_cpu.py:42-163``; plans without ``batch``).
-        * ``'lsqr'``: scipy's LSQR on P with k-space unknowns in FP64
-          (``solve_cpu.py:236-256``; options ``damp, atol, btol, conlim,
-          iter_lim``; plans without ``batch``).  ``self.lsqr_info`` then holds
-          (istop, iterations).
+        * ``'lsqr'`` / ``'lsmr'``: scipy's LSQR / LSMR on P with k-space
+          unknowns in FP64 (``solve_cpu.py:236-256``; options ``damp, atol,
+          btol, conlim`` and ``iter_lim`` / ``maxiter``; plans without
+          ``batch``).  ``self.lsqr_info`` then holds (istop, iterations).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg, dc, L1TVOLS, lsqr')
+            raise KeyError('solver must be one of: cg, dc, L1TVOLS, lsqr, lsmr')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
         if solver == 'dc':
@@ -494,9 +494,11 @@ class NUFFT:
             return self._solve_l1tvols(y, *args, **kwargs)
         if solver == 'lsqr':
             return self._solve_lsqr(y, *args, **kwargs)
+        if solver == 'lsmr':
+            return self._solve_lsmr(y, *args, **kwargs)
         if solver != 'cg':
-            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc', 'L1TVOLS' "
-                                      "or 'lsqr'" % (solver,))
+            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc', 'L1TVOLS', "
+                                      "'lsqr' or 'lsmr'" % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -566,9 +568,28 @@ class NUFFT:
         self._lsqr_info = info
         return self._out(out, is_np)
 
+    def _solve_lsmr(self, y, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8, maxiter=None,
+                    show=False, x0=None):
+        """scipy.sparse.linalg.lsmr semantics on P (k2y / y2k), x0 = 0."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('lsmr solves plans without batch')
+        if x0 is not None:
+            raise NotImplementedError('lsmr: x0 is not supported')
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_LSMR, _lib.C128)
+        info = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._call(self._libh.pnh_solve_lsmr, _ptr(yt), _ptr(out),
+                   int(maxiter) if maxiter is not None else 0, float(damp), float(atol),
+                   float(btol), float(conlim), _ptr(ws), _ptr(info), self._stream())
+        self._lsqr_info = info
+        return self._out(out, is_np)
+
     @property
     def lsqr_info(self):
-        """(istop, iterations) of the last 'lsqr' solve."""
+        """(istop, iterations) of the last 'lsqr' or 'lsmr' solve."""
         return tuple(int(v) for v in self._lsqr_info.cpu().numpy())
 
     def precompute_sp(self):

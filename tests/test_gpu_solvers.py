@@ -90,20 +90,26 @@ def test_solver_edge_cases(hip):
 
 
 @pytest.mark.parametrize('name', SOLVER_CASES)
-def test_solve_lsqr(hip, name):
-    """scipy's lsqr (8 iterations) on P in FP64 against the reference."""
+def test_solve_lsqr_lsmr(hip, name):
+    """scipy's lsqr and lsmr (8 iterations) on P in FP64 against the
+    reference."""
     A, g = make(name)
     s = golden('solvers')
     x = A.solve(g['adjoint_in'], 'lsqr', iter_lim=8)
     assert rel(x, s[name + '__lsqr8']) < TOL
     istop, itn = A.lsqr_info
     assert itn <= 8 and (istop == 7) == (itn == 8)
+    x = A.solve(g['adjoint_in'], 'lsmr', maxiter=8)
+    assert rel(x, s[name + '__lsmr8']) < TOL
+    istop, itn = A.lsqr_info
+    assert itn <= 8 and (istop == 7) == (itn == 8)
 
 
-def test_lsqr_stops_like_scipy(hip):
+@pytest.mark.parametrize('solver', ['lsqr', 'lsmr'])
+def test_lsqr_stops_like_scipy(hip, solver):
     """Run to scipy's own stopping test (default tolerances, iteration limit
     300): the device stops at the same iteration with the same istop as
-    scipy's lsqr on the oracle's operators, and agrees on the image."""
+    scipy's lsqr / lsmr on the oracle's operators, and agrees on the image."""
     import oracle
     import scipy.sparse.linalg as sla
     A, g = make('tiny2d')
@@ -114,8 +120,12 @@ def test_lsqr_stops_like_scipy(hip):
     op = sla.LinearOperator((M, K), dtype=np.complex128,
                             matvec=lambda k: B.k2y(k.reshape(B.Kd)).ravel(),
                             rmatvec=lambda v: B.y2k(v).ravel())
-    res = sla.lsqr(op, y, iter_lim=300)
-    x = A.solve(y, 'lsqr', iter_lim=300)
+    if solver == 'lsqr':
+        res = sla.lsqr(op, y, iter_lim=300)
+        x = A.solve(y, 'lsqr', iter_lim=300)
+    else:
+        res = sla.lsmr(op, y, maxiter=300)
+        x = A.solve(y, 'lsmr', maxiter=300)
     istop, itn = A.lsqr_info
     assert (istop, itn) == (res[1], res[2])
     ref = B.k2xx(res[0].reshape(B.Kd)) / B.sn
