@@ -51,8 +51,12 @@ enum pnh_op {
   PNH_OP_SOLVE_DC = 10,
   PNH_OP_SOLVE_L1TVOLS = 11,
   PNH_OP_SOLVE_LSQR = 12,
-  PNH_OP_SOLVE_LSMR = 13
+  PNH_OP_SOLVE_LSMR = 13,
+  PNH_OP_SOLVE_BICG = 14
 };
+
+/* Krylov solvers of pnh_solve_bicg. */
+enum pnh_bicg_kind { PNH_BICGSTAB = 0, PNH_BICG = 1 };
 
 /*
  * Plan descriptor: geometry plus device pointers to the interpolator built on
@@ -209,6 +213,17 @@ int pnh_solve_lsqr(const pnh_plan* plan, const void* y, void* x, int iter_lim, d
 int pnh_solve_lsmr(const pnh_plan* plan, const void* y, void* x, int maxiter, double damp,
                    double atol, double btol, double conlim, void* ws, int32_t* info_out,
                    void* stream);
+/* solve(y, 'bicgstab' | 'bicg', ...): scipy.sparse.linalg.bicgstab / bicg
+ * (no preconditioner, x0 = 0) on the k-space normal equations
+ * P^H P k = P^H y in FP64 on the device (batch 1); stops when
+ * ||r|| < max(atol, rtol ||b||) at the start of an iteration (bicgstab also
+ * after its half step), on scipy's breakdown tests, or after maxiter
+ * (<= 0: scipy's default 10 prod(Kd)); then x = crop(ifftn(k)) / sn.
+ * info_out (device int32[2], may be NULL) receives scipy's info and the
+ * iterations run.  Reference linalg/solve_cpu.py:256-288 with scipy's
+ * bicgstab / bicg. */
+int pnh_solve_bicg(const pnh_plan* plan, int kind, const void* y, void* x, int maxiter,
+                   double rtol, double atol, void* ws, int32_t* info_out, void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

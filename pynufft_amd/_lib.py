@@ -16,7 +16,8 @@ MAXD = 3
 C64, C128 = 0, 1
 (OP_FORWARD, OP_ADJOINT, OP_SELFADJOINT, OP_Y2K, OP_K2Y2K, OP_SOLVE_CG,
  OP_FORWARD_ONE2MANY, OP_ADJOINT_MANY2ONE, OP_PRECOMPUTE_SP, OP_SELFADJOINT2,
- OP_SOLVE_DC, OP_SOLVE_L1TVOLS, OP_SOLVE_LSQR, OP_SOLVE_LSMR) = range(14)
+ OP_SOLVE_DC, OP_SOLVE_L1TVOLS, OP_SOLVE_LSQR, OP_SOLVE_LSMR, OP_SOLVE_BICG) = range(15)
+BICGSTAB, BICG = 0, 1
 
 _vp = ctypes.c_void_p
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -89,6 +90,8 @@ EXPORTS = {
     'pnh_solve_lsqr': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       _vp, _vp, _vp]),
+    'pnh_solve_bicg': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]),
     'pnh_solve_lsmr': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       _vp, _vp, _vp]),

@@ -408,6 +408,11 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
     case PNH_OP_SOLVE_LSQR:
       *bytes = lsqr_ws_bytes(p);
       break;
+    case PNH_OP_SOLVE_BICG:
+      *bytes = align_up((size_t)p->M * 16) + 7 * align_up((size_t)p->Ktot * 16) +
+               align_up(partial_bytes(p, 1)) + align_up((size_t)p->Ntot * 16) +
+               align_up((size_t)2 * PNH_LSQR_BLOCKS * 16) + align_up(sizeof(KrylovState));
+      break;
     case PNH_OP_SOLVE_LSMR:
       *bytes = lsqr_ws_bytes(p) + align_up((size_t)p->Ktot * 16) + align_up(sizeof(LsmrState));
       break;
@@ -776,6 +781,83 @@ int pnh_solve_lsmr(const pnh_plan* p, const void* y, void* x, int maxiter, doubl
   if (info_out) {
     PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->istop, sizeof(int), hipMemcpyDeviceToDevice, s));
     PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->itn, sizeof(int), hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
+int pnh_solve_bicg(const pnh_plan* p, int kind, const void* y, void* x, int maxiter,
+                   double rtol, double atol, void* ws, int32_t* info_out, void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(kind == PNH_BICGSTAB || kind == PNH_BICG, "unknown Krylov solver %d", kind);
+  REQUIRE(p->nb == 1, "bicg/bicgstab solve one coil (plan without batch)");
+  REQUIRE(atol >= 0.0, "atol must be >= 0");
+  int rc = check_dtype_ready(p, PNH_C128);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t M = p->M, K = p->Ktot;
+  if (maxiter <= 0) maxiter = (int)(10 * K < (1LL << 30) ? 10 * K : (1LL << 30));
+  const bool stab = kind == PNH_BICGSTAB;
+  char* w = (char*)ws;
+  void* ym = w; w += align_up((size_t)M * 16);
+  void* xk = w; w += align_up((size_t)K * 16);
+  void* r = w; w += align_up((size_t)K * 16);
+  void* rt = w; w += align_up((size_t)K * 16);
+  void* pv = w; w += align_up((size_t)K * 16);
+  void* v = w; w += align_up((size_t)K * 16);   // bicgstab v, bicg q
+  void* t = w; w += align_up((size_t)K * 16);   // bicgstab t, bicg qt
+  void* pt = w; w += align_up((size_t)K * 16);  // bicg ptilde
+  void* part = w; w += align_up(partial_bytes(p, 1));
+  void* img64 = w; w += align_up((size_t)p->Ntot * 16);
+  void* red = w; w += align_up((size_t)2 * PNH_LSQR_BLOCKS * 16);
+  KrylovState* st = (KrylovState*)w;
+  const int* done = &st->done;
+  const int* skip_t = &st->skip_t;
+  auto normal_op = [&](const void* in, void* out, const int* skip) -> int {
+    int e;
+    if (M > 0 && (e = do_k2y(p, 1, in, ym, skip, s))) return e;
+    return do_y2k(p, 1, ym, out, part, skip, s);
+  };
+  // b = P^H y; r = b, rt = r, x = 0
+  if ((rc = pnh_krylov_init(st, stab ? 1 : 0, rtol, atol, maxiter, s))) return rc;
+  if (M > 0 && (rc = pnh_c64_to_c128(y, ym, M, s))) return rc;
+  if ((rc = do_y2k(p, 1, ym, r, part, nullptr, s))) return rc;
+  if ((rc = pnh_fill_c128(xk, K, 0.0, 0.0, s))) return rc;
+  PNH_CHECK_HIP(hipMemcpyAsync(rt, r, (size_t)K * 16, hipMemcpyDeviceToDevice, s));
+  if ((rc = pnh_krylov_vdot2(r, r, nullptr, nullptr, K, red, done, s))) return rc;
+  if ((rc = pnh_krylov_scalar(st, red, 0, s))) return rc;
+  for (int it = 0; it < maxiter; ++it) {
+    if ((rc = pnh_krylov_vdot2(r, r, rt, r, K, red, done, s))) return rc;
+    if ((rc = pnh_krylov_scalar(st, red, 1, s))) return rc;
+    if ((rc = pnh_krylov_p(pv, r, v, stab ? nullptr : pt, rt, st, K, s))) return rc;
+    if ((rc = normal_op(pv, v, done))) return rc;
+    if (stab) {
+      if ((rc = pnh_krylov_vdot2(rt, v, nullptr, nullptr, K, red, done, s))) return rc;
+      if ((rc = pnh_krylov_scalar(st, red, 2, s))) return rc;
+      if ((rc = pnh_krylov_r(r, v, st, K, red, s))) return rc;
+      if ((rc = pnh_krylov_scalar(st, red, 3, s))) return rc;
+      if ((rc = normal_op(r, t, skip_t))) return rc;
+      if ((rc = pnh_krylov_vdot2(t, r, t, t, K, red, skip_t, s))) return rc;
+      if ((rc = pnh_krylov_scalar(st, red, 4, s))) return rc;
+      if ((rc = pnh_krylov_x(xk, pv, r, t, nullptr, nullptr, st, K, s))) return rc;
+    } else {
+      if ((rc = normal_op(pt, t, done))) return rc;
+      if ((rc = pnh_krylov_vdot2(pt, v, nullptr, nullptr, K, red, done, s))) return rc;
+      if ((rc = pnh_krylov_scalar(st, red, 2, s))) return rc;
+      if ((rc = pnh_krylov_x(xk, pv, r, v, rt, t, st, K, s))) return rc;
+    }
+    if ((rc = pnh_krylov_scalar(st, red, 5, s))) return rc;
+    if ((it & 63) == 63 && it + 1 < maxiter) {
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
+  }
+  if ((rc = run_passes_inverse(p, xk, img64, false, 1, nullptr, s))) return rc;
+  if ((rc = pnh_cast_div_sn(img64, x, p->d.sn_f32, p->Ntot, 1, s))) return rc;
+  if (info_out) {
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->it, sizeof(int), hipMemcpyDeviceToDevice, s));
   }
   return 0;
 }

@@ -81,3 +81,24 @@ int pnh_lsmr_init(LsmrState* st, double atol, double btol, double conlim, double
 int pnh_lsmr_hx(void* x, void* h, void* hbar, const void* v, const LsmrState* st, int64_t n,
                 double* part, hipStream_t s);
 int pnh_lsmr_scalar(LsmrState* st, const double* part, int phase, hipStream_t s);
+
+// ---- BiCGSTAB / BiCG (scipy semantics) on P^H P, FP64, complex scalars
+struct KrylovState {
+  double2 rho, rho_prev, alpha, omega, beta;
+  double rtol, atol, tol, rhotol;
+  int stab, it, maxiter, done, info, half, skip_t;
+};
+
+int pnh_krylov_init(KrylovState* st, int stab, double rtol, double atol, int maxiter,
+                    hipStream_t s);
+// block partials of vdot(a0, b0) (and of vdot(a1, b1) when a1 is set), 2 x
+// PNH_LSQR_BLOCKS complex128 entries
+int pnh_krylov_vdot2(const void* a0, const void* b0, const void* a1, const void* b1, int64_t n,
+                     void* part, const int* skip, hipStream_t s);
+int pnh_krylov_scalar(KrylovState* st, const void* part, int phase, hipStream_t s);
+int pnh_krylov_p(void* p, const void* r, const void* v, void* pt, const void* rt,
+                 const KrylovState* st, int64_t n, hipStream_t s);
+int pnh_krylov_r(void* r, const void* v, const KrylovState* st, int64_t n, void* part,
+                 hipStream_t s);
+int pnh_krylov_x(void* x, const void* p, void* r, const void* t, void* rt, const void* qt,
+                 const KrylovState* st, int64_t n, hipStream_t s);

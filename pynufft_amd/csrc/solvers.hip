@@ -689,3 +689,254 @@ int pnh_lsmr_scalar(LsmrState* st, const double* part, int phase, hipStream_t s)
   PNH_LAUNCH_CHECK();
   return 0;
 }
+
+// --------------------------------------------------------- BiCG(STAB) --
+// scipy.sparse.linalg.bicgstab / bicg on the k-space normal operator P^H P
+// (no preconditioner), FP64, x0 = 0; complex scalars in KrylovState.
+namespace {
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double zabs(double2 a) { return hypot(a.x, a.y); }
+
+__device__ void block_sum2_store(double2 v, double2* __restrict__ part) {
+  __shared__ double2 red[256];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[threadIdx.x].x += red[threadIdx.x + st].x;
+      red[threadIdx.x].y += red[threadIdx.x + st].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// part[0][b] = block partial of vdot(a0, b0); part[1][b] of vdot(a1, b1)
+// (second pair optional)
+__global__ void __launch_bounds__(256)
+vdot2_kernel(const double2* __restrict__ a0, const double2* __restrict__ b0,
+             const double2* __restrict__ a1, const double2* __restrict__ b1, int64_t n,
+             double2* __restrict__ part, const int* __restrict__ skip) {
+  if (*skip) return;
+  double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
+  GRID_STRIDE(i, n) {
+    const double2 u = a0[i], v = b0[i];
+    s0.x += u.x * v.x + u.y * v.y;
+    s0.y += u.x * v.y - u.y * v.x;
+    if (a1) {
+      const double2 p = a1[i], q = b1[i];
+      s1.x += p.x * q.x + p.y * q.y;
+      s1.y += p.x * q.y - p.y * q.x;
+    }
+  }
+  block_sum2_store(s0, part);
+  if (a1) {
+    __syncthreads();
+    block_sum2_store(s1, part + PNH_LSQR_BLOCKS);
+  }
+}
+
+__device__ double2 sum_partials2(const double2* __restrict__ part) {
+  double2 s = make_double2(0.0, 0.0);
+  for (int b = 0; b < PNH_LSQR_BLOCKS; ++b) { s.x += part[b].x; s.y += part[b].y; }
+  return s;
+}
+
+__global__ void krylov_init_kernel(KrylovState* __restrict__ st, int stab, double rtol,
+                                   double atol, int maxiter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  KrylovState z = {};
+  z.stab = stab;
+  z.rtol = rtol;
+  z.atol = atol;
+  z.maxiter = maxiter;
+  z.rhotol = 2.220446049250313e-16 * 2.220446049250313e-16;
+  *st = z;
+}
+
+// Scalar steps.  BiCGSTAB: 0 = |b| (tolerance); 1 = |r| test, rho, breakdown
+// tests, beta; 2 = alpha = rho / vdot(rt, v); 3 = |s| test; 4 = omega;
+// 5 = end of iteration.  BiCG: 0, 1 (|r| test, rho, beta), 2 (alpha), 5.
+__device__ void krylov_step(KrylovState* __restrict__ st, const double2* __restrict__ part,
+                            int phase);
+
+// skip_t (the second matvec of BiCGSTAB and its dots) is set whenever the
+// solve is done or the half-step test converged
+__global__ void krylov_scalar_kernel(KrylovState* __restrict__ st,
+                                     const double2* __restrict__ part, int phase) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (!st->done) krylov_step(st, part, phase);
+  st->skip_t = (st->done || st->half) ? 1 : 0;
+}
+
+__device__ void krylov_step(KrylovState* __restrict__ st, const double2* __restrict__ part,
+                            int phase) {
+  const double2 S0 = sum_partials2(part);
+  if (phase == 0) {
+    const double bnrm2 = sqrt(S0.x);
+    st->tol = fmax(st->atol, st->rtol * bnrm2);
+    if (bnrm2 == 0.0) { st->done = 1; st->info = 0; }
+    return;
+  }
+  if (phase == 1) {
+    st->half = 0;
+    if (sqrt(S0.x) < st->tol) { st->done = 1; st->info = 0; return; }
+    const double2 rho = sum_partials2(part + PNH_LSQR_BLOCKS);
+    if (zabs(rho) < st->rhotol) { st->done = 1; st->info = -10; return; }
+    if (st->it > 0) {
+      if (st->stab) {
+        if (zabs(st->omega) < st->rhotol) { st->done = 1; st->info = -11; return; }
+        st->beta = zmul(cdivd(rho, st->rho_prev), cdivd(st->alpha, st->omega));
+      } else {
+        st->beta = cdivd(rho, st->rho_prev);
+      }
+    }
+    st->rho = rho;
+    return;
+  }
+  if (phase == 2) {
+    if (S0.x == 0.0 && S0.y == 0.0) { st->done = 1; st->info = -11; return; }
+    st->alpha = cdivd(st->rho, S0);
+    return;
+  }
+  if (phase == 3) {
+    st->half = sqrt(S0.x) < st->tol ? 1 : 0;
+    return;
+  }
+  if (phase == 4) {
+    if (!st->half) st->omega = cdivd(S0, sum_partials2(part + PNH_LSQR_BLOCKS));
+    return;
+  }
+  // phase 5
+  st->rho_prev = st->rho;
+  st->it += 1;
+  if (st->half) { st->done = 1; st->info = 0; }
+  else if (st->it >= st->maxiter) { st->done = 1; st->info = st->maxiter; }
+}
+
+// BiCGSTAB p: first iteration p = r, else p = (p - omega v) beta + r
+// BiCG p, pt: p = p beta + r, pt = pt conj(beta) + rt (first: copies)
+__global__ void krylov_p_kernel(double2* __restrict__ p, const double2* __restrict__ r,
+                                const double2* __restrict__ v, double2* __restrict__ pt,
+                                const double2* __restrict__ rt,
+                                const KrylovState* __restrict__ st, int64_t n) {
+  if (st->done) return;
+  const bool first = st->it == 0;
+  const double2 beta = st->beta, om = st->omega;
+  GRID_STRIDE(i, n) {
+    if (first) {
+      p[i] = r[i];
+      if (pt) pt[i] = rt[i];
+      continue;
+    }
+    double2 pv = p[i];
+    if (st->stab) {
+      const double2 ov = zmul(om, v[i]);
+      pv = make_double2(pv.x - ov.x, pv.y - ov.y);
+    }
+    pv = zmul(pv, beta);
+    p[i] = make_double2(pv.x + r[i].x, pv.y + r[i].y);
+    if (pt) {
+      const double2 q = zmul(pt[i], zconj(beta));
+      pt[i] = make_double2(q.x + rt[i].x, q.y + rt[i].y);
+    }
+  }
+}
+
+// r -= alpha v; partial |r|^2 (BiCGSTAB s = r)
+__global__ void __launch_bounds__(256)
+krylov_r_kernel(double2* __restrict__ r, const double2* __restrict__ v,
+                const KrylovState* __restrict__ st, int64_t n, double2* __restrict__ part) {
+  if (st->done) return;
+  const double2 a = st->alpha;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 av = zmul(a, v[i]);
+    const double2 rv = make_double2(r[i].x - av.x, r[i].y - av.y);
+    r[i] = rv;
+    acc += rv.x * rv.x + rv.y * rv.y;
+  }
+  block_sum2_store(make_double2(acc, 0.0), part);
+}
+
+// BiCGSTAB end of iteration: x += alpha p; unless |s| converged: x += omega s,
+// r -= omega t.  BiCG: x += alpha p; r -= alpha q; rt -= conj(alpha) qt.
+__global__ void krylov_x_kernel(double2* __restrict__ x, const double2* __restrict__ p,
+                                double2* __restrict__ r, const double2* __restrict__ t,
+                                double2* __restrict__ rt, const double2* __restrict__ qt,
+                                const KrylovState* __restrict__ st, int64_t n) {
+  if (st->done) return;
+  const double2 a = st->alpha, om = st->omega;
+  const bool stab = st->stab, half = st->half;
+  GRID_STRIDE(i, n) {
+    double2 xv = x[i];
+    const double2 ap = zmul(a, p[i]);
+    xv = make_double2(xv.x + ap.x, xv.y + ap.y);
+    if (stab) {
+      if (!half) {
+        const double2 os = zmul(om, r[i]);
+        xv = make_double2(xv.x + os.x, xv.y + os.y);
+        const double2 ot = zmul(om, t[i]);
+        r[i] = make_double2(r[i].x - ot.x, r[i].y - ot.y);
+      }
+    } else {
+      const double2 aq = zmul(a, t[i]);
+      r[i] = make_double2(r[i].x - aq.x, r[i].y - aq.y);
+      const double2 aqt = zmul(zconj(a), qt[i]);
+      rt[i] = make_double2(rt[i].x - aqt.x, rt[i].y - aqt.y);
+    }
+    x[i] = xv;
+  }
+}
+
+}  // namespace
+
+int pnh_krylov_init(KrylovState* st, int stab, double rtol, double atol, int maxiter,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(krylov_init_kernel, dim3(1), dim3(64), 0, s, st, stab, rtol, atol,
+                     maxiter);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_krylov_vdot2(const void* a0, const void* b0, const void* a1, const void* b1, int64_t n,
+                     void* part, const int* skip, hipStream_t s) {
+  hipLaunchKernelGGL(vdot2_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (const double2*)a0,
+                     (const double2*)b0, (const double2*)a1, (const double2*)b1, n,
+                     (double2*)part, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_krylov_scalar(KrylovState* st, const void* part, int phase, hipStream_t s) {
+  hipLaunchKernelGGL(krylov_scalar_kernel, dim3(1), dim3(64), 0, s, st, (const double2*)part,
+                     phase);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_krylov_p(void* p, const void* r, const void* v, void* pt, const void* rt,
+                 const KrylovState* st, int64_t n, hipStream_t s) {
+  LAUNCH(krylov_p_kernel, n, (double2*)p, (const double2*)r, (const double2*)v, (double2*)pt,
+         (const double2*)rt, st, n);
+  return 0;
+}
+
+int pnh_krylov_r(void* r, const void* v, const KrylovState* st, int64_t n, void* part,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(krylov_r_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (double2*)r,
+                     (const double2*)v, st, n, (double2*)part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_krylov_x(void* x, const void* p, void* r, const void* t, void* rt, const void* qt,
+                 const KrylovState* st, int64_t n, hipStream_t s) {
+  LAUNCH(krylov_x_kernel, n, (double2*)x, (const double2*)p, (double2*)r, (const double2*)t,
+         (double2*)rt, (const double2*)qt, st, n);
+  return 0;
+}

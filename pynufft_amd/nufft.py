@@ -474,6 +474,10 @@ class NUFFT:
           FP64 on the device; returns crop(ifftn(k)) / sn.  After the call
           ``self.cg_iters`` holds the iterations used (per coil with
           ``batch``).
+        * ``'bicgstab'`` / ``'bicg'``: scipy's BiCGSTAB / BiCG on the same
+          normal equations in FP64 (``rtol``, ``atol``, ``maxiter``; plans
+          without ``batch``); ``self.bicg_info`` then holds (scipy's info,
+          iterations).
         * ``'dc'``: density-compensated adjoint, ``maxiter`` pipe-density
           iterations (``solve_cpu.py:165-183``).
         * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
@@ -485,7 +489,7 @@ class NUFFT:
           ``batch``).  ``self.lsqr_info`` then holds (istop, iterations).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg, dc, L1TVOLS, lsqr, lsmr')
+            raise KeyError('solver must be one of: cg, bicgstab, bicg, dc, L1TVOLS, lsqr, lsmr')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
         if solver == 'dc':
@@ -496,9 +500,11 @@ class NUFFT:
             return self._solve_lsqr(y, *args, **kwargs)
         if solver == 'lsmr':
             return self._solve_lsmr(y, *args, **kwargs)
+        if solver in ('bicgstab', 'bicg'):
+            return self._solve_bicg(y, solver, *args, **kwargs)
         if solver != 'cg':
-            raise NotImplementedError("solver %r is not provided; use 'cg', 'dc', 'L1TVOLS', "
-                                      "'lsqr' or 'lsmr'" % (solver,))
+            raise NotImplementedError("solver %r is not provided; use 'cg', 'bicgstab', "
+                                      "'bicg', 'dc', 'L1TVOLS', 'lsqr' or 'lsmr'" % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -567,6 +573,32 @@ class NUFFT:
                    float(btol), float(conlim), _ptr(ws), _ptr(info), self._stream())
         self._lsqr_info = info
         return self._out(out, is_np)
+
+    def _solve_bicg(self, y, solver, maxiter=None, rtol=None, atol=0.0, tol=None, x0=None,
+                    M=None, callback=None):
+        """scipy.sparse.linalg.bicgstab / bicg semantics on P^H P, x0 = 0."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('%s solves plans without batch' % solver)
+        if x0 is not None or M is not None or callback is not None:
+            raise NotImplementedError('%s: x0, M and callback are not supported' % solver)
+        rtol = rtol if rtol is not None else (tol if tol is not None else 1e-5)
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        ws = self._workspace(_lib.OP_SOLVE_BICG, _lib.C128)
+        info = torch.zeros(2, dtype=torch.int32, device=self.device)
+        kind = _lib.BICGSTAB if solver == 'bicgstab' else _lib.BICG
+        self._call(self._libh.pnh_solve_bicg, int(kind), _ptr(yt), _ptr(out),
+                   int(maxiter) if maxiter is not None else 0, float(rtol), float(atol),
+                   _ptr(ws), _ptr(info), self._stream())
+        self._bicg_info = info
+        return self._out(out, is_np)
+
+    @property
+    def bicg_info(self):
+        """(scipy's info, iterations) of the last 'bicgstab' or 'bicg' solve."""
+        return tuple(int(v) for v in self._bicg_info.cpu().numpy())
 
     def _solve_lsmr(self, y, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8, maxiter=None,
                     show=False, x0=None):

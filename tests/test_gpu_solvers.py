@@ -130,3 +130,42 @@ def test_lsqr_stops_like_scipy(hip, solver):
     assert (istop, itn) == (res[1], res[2])
     ref = B.k2xx(res[0].reshape(B.Kd)) / B.sn
     assert rel(x, ref) < 1e-5
+
+
+@pytest.mark.parametrize('name', SOLVER_CASES)
+def test_solve_bicgstab_bicg(hip, name):
+    """scipy's bicgstab and bicg (6 iterations) on P^H P in FP64 against the
+    reference."""
+    A, g = make(name)
+    s = golden('solvers')
+    for solver in ('bicgstab', 'bicg'):
+        x = A.solve(g['adjoint_in'], solver, maxiter=6)
+        assert rel(x, s[name + '__%s6' % solver]) < TOL, solver
+
+
+@pytest.mark.parametrize('solver', ['bicgstab', 'bicg'])
+def test_bicg_stops_like_scipy(hip, solver):
+    """To scipy's own stopping test (rtol 1e-5, maxiter 200): same info and
+    iteration count as scipy on the oracle's operator, same image."""
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make('tiny2d')
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y = g['adjoint_in']
+    K = B.Kdprod
+    count = [0]
+
+    def mv(v):
+        count[0] += 1
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    fn = {'bicgstab': sla.bicgstab, 'bicg': sla.bicg}[solver]
+    k, info = fn(op, B.y2k(y).ravel(), maxiter=200)
+    x = A.solve(y, solver, maxiter=200)
+    dinfo, its = A.bicg_info
+    assert dinfo == info
+    per_it = 2
+    assert its == count[0] // per_it or its == (count[0] + 1) // per_it
+    ref = B.k2xx(k.reshape(B.Kd)) / B.sn
+    assert rel(x, ref) < 1e-5
