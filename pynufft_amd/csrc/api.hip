@@ -911,6 +911,13 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_PQ, it, maxiter, rtol, atol, s))) return rc;
     if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redp, s))) return rc;
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_RR, it, maxiter, rtol, atol, s))) return rc;
+    if ((it & 63) == 63 && it + 1 < maxiter) {
+      // long runs: stop launching once every coil has converged
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, all_done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
   }
   // x = complex64(crop(ifftn(k))) / sn, inverse FFT in FP64 (as numpy does
   // for the complex128 CG result)

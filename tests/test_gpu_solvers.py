@@ -169,3 +169,31 @@ def test_bicg_stops_like_scipy(hip, solver):
     assert its == count[0] // per_it or its == (count[0] + 1) // per_it
     ref = B.k2xx(k.reshape(B.Kd)) / B.sn
     assert rel(x, ref) < 1e-5
+
+
+def test_cg_default_maxiter_converges(hip):
+    """solve(y, 'cg') with scipy's default maxiter (10 prod(Kd)) stops at
+    scipy's iteration and returns its image; the launch loop checks the
+    device convergence flag so it does not run to the limit."""
+    import time
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make('tiny2d')
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y = g['adjoint_in']
+    its = [0]
+
+    def mv(v):
+        its[0] += 1
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    K = B.Kdprod
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    k, info = sla.cg(op, B.y2k(y).ravel())
+    assert info == 0
+    t = time.perf_counter()
+    x = A.solve(y, 'cg')
+    dt = time.perf_counter() - t
+    assert A.cg_iters == its[0]
+    assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
+    assert dt < 5.0
