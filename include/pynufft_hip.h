@@ -224,6 +224,16 @@ int pnh_solve_lsmr(const pnh_plan* plan, const void* y, void* x, int maxiter, do
  * bicgstab / bicg. */
 int pnh_solve_bicg(const pnh_plan* plan, int kind, const void* y, void* x, int maxiter,
                    double rtol, double atol, void* ws, int32_t* info_out, void* stream);
+/* solve(y, 'gmres', restart=, maxiter=, rtol=, atol=): scipy's restarted
+ * GMRES (modified Gram-Schmidt, Givens rotations, no preconditioner) on
+ * P^H P in FP64 on the device (batch 1); restart 1..64 (<= 0: scipy's 20,
+ * capped at prod(Kd)), maxiter = restart cycles (<= 0: 10 prod(Kd)).  Its
+ * scratch depends on restart: pnh_gmres_workspace_bytes.  info_out (device
+ * int32[2], may be NULL) receives scipy's info and the inner iterations.
+ * Reference linalg/solve_cpu.py:256-288 with scipy's gmres. */
+int pnh_gmres_workspace_bytes(const pnh_plan* plan, int restart, size_t* bytes);
+int pnh_solve_gmres(const pnh_plan* plan, const void* y, void* x, int restart, int maxiter,
+                    double rtol, double atol, void* ws, int32_t* info_out, void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

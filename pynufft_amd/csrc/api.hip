@@ -862,6 +862,102 @@ int pnh_solve_bicg(const pnh_plan* p, int kind, const void* y, void* x, int maxi
   return 0;
 }
 
+static int gmres_restart(const pnh_plan* p, int restart) {
+  if (restart <= 0) restart = 20;
+  if ((int64_t)restart > p->Ktot) restart = (int)p->Ktot;
+  return restart;
+}
+
+int pnh_gmres_workspace_bytes(const pnh_plan* p, int restart, size_t* bytes) {
+  REQUIRE(p && bytes, "null argument");
+  restart = gmres_restart(p, restart);
+  REQUIRE(restart >= 1 && restart <= PNH_GMRES_MAXR, "restart must be 1..%d", PNH_GMRES_MAXR);
+  const size_t kv = align_up((size_t)p->Ktot * 16);
+  *bytes = align_up((size_t)p->M * 16) + (size_t)(restart + 1) * kv + 4 * kv +
+           align_up(partial_bytes(p, 1)) + align_up((size_t)p->Ntot * 16) +
+           align_up((size_t)2 * PNH_LSQR_BLOCKS * 16) + align_up(sizeof(GmresState));
+  return 0;
+}
+
+int pnh_solve_gmres(const pnh_plan* p, const void* y, void* x, int restart, int maxiter,
+                    double rtol, double atol, void* ws, int32_t* info_out, void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(p->nb == 1, "gmres solves one coil (plan without batch)");
+  REQUIRE(atol >= 0.0, "atol must be >= 0");
+  restart = gmres_restart(p, restart);
+  REQUIRE(restart >= 1 && restart <= PNH_GMRES_MAXR, "restart must be 1..%d", PNH_GMRES_MAXR);
+  int rc = check_dtype_ready(p, PNH_C128);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t M = p->M, K = p->Ktot;
+  if (maxiter <= 0) maxiter = (int)(10 * K < (1LL << 30) ? 10 * K : (1LL << 30));
+  const size_t kv = align_up((size_t)K * 16);
+  char* w = (char*)ws;
+  void* ym = w; w += align_up((size_t)M * 16);
+  char* V = w; w += (size_t)(restart + 1) * kv;
+  void* wv = w; w += kv;
+  void* xk = w; w += kv;
+  void* r = w; w += kv;
+  void* bv = w; w += kv;
+  void* part = w; w += align_up(partial_bytes(p, 1));
+  void* img64 = w; w += align_up((size_t)p->Ntot * 16);
+  void* red = w; w += align_up((size_t)2 * PNH_LSQR_BLOCKS * 16);
+  GmresState* st = (GmresState*)w;
+  const int* done = &st->done;
+  const int* skip = &st->skip;
+  const int64_t ldv = (int64_t)(kv / 16);
+  auto vec = [&](int k) -> void* { return V + (size_t)k * kv; };
+  auto normal_op = [&](const void* in, void* out, const int* sk) -> int {
+    int e;
+    if (M > 0 && (e = do_k2y(p, 1, in, ym, sk, s))) return e;
+    return do_y2k(p, 1, ym, out, part, sk, s);
+  };
+  if ((rc = pnh_gmres_init(st, restart, rtol, atol, maxiter, s))) return rc;
+  if (M > 0 && (rc = pnh_c64_to_c128(y, ym, M, s))) return rc;
+  if ((rc = do_y2k(p, 1, ym, bv, part, nullptr, s))) return rc;
+  PNH_CHECK_HIP(hipMemcpyAsync(r, bv, (size_t)K * 16, hipMemcpyDeviceToDevice, s));
+  if ((rc = pnh_fill_c128(xk, K, 0.0, 0.0, s))) return rc;
+  if ((rc = pnh_krylov_vdot2(bv, bv, nullptr, nullptr, K, red, done, s))) return rc;
+  if ((rc = pnh_gmres_scalar(st, red, 0, 0, 0, s))) return rc;
+  for (int outer = 0; outer < maxiter; ++outer) {
+    if ((rc = pnh_krylov_vdot2(r, r, nullptr, nullptr, K, red, done, s))) return rc;
+    if ((rc = pnh_gmres_scalar(st, red, 1, 0, 0, s))) return rc;
+    if ((rc = pnh_gmres_normalize(vec(0), r, st, K, s))) return rc;
+    for (int col = 0; col < restart; ++col) {
+      if ((rc = normal_op(vec(col), wv, skip))) return rc;
+      if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
+      if ((rc = pnh_gmres_scalar(st, red, 2, col, 0, s))) return rc;
+      for (int k = 0; k <= col; ++k) {
+        if ((rc = pnh_krylov_vdot2(vec(k), wv, nullptr, nullptr, K, red, skip, s))) return rc;
+        if ((rc = pnh_gmres_scalar(st, red, 3, col, k, s))) return rc;
+        if ((rc = pnh_gmres_mgs(wv, vec(k), st, col, k, K, s))) return rc;
+      }
+      if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
+      if ((rc = pnh_gmres_scalar(st, red, 4, col, 0, s))) return rc;
+      if ((rc = pnh_gmres_normalize(vec(col + 1), wv, st, K, s))) return rc;
+    }
+    if ((rc = pnh_gmres_solve(st, s))) return rc;
+    if ((rc = pnh_gmres_x(xk, V, ldv, st, K, s))) return rc;
+    if ((rc = normal_op(xk, wv, done))) return rc;
+    if ((rc = pnh_gmres_residual(r, bv, wv, st, K, red, s))) return rc;
+    if ((rc = pnh_gmres_scalar(st, red, 6, 0, 0, s))) return rc;
+    if (outer + 1 < maxiter) {
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
+  }
+  if ((rc = run_passes_inverse(p, xk, img64, false, 1, nullptr, s))) return rc;
+  if ((rc = pnh_cast_div_sn(img64, x, p->d.sn_f32, p->Ntot, 1, s))) return rc;
+  if (info_out) {
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->inner_iter, sizeof(int),
+                                 hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
 int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* om_grad,
                         void* stream) {
   REQUIRE(p && k && (g || p->M == 0) && (om_grad || p->M == 0), "null argument");

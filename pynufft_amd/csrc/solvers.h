@@ -102,3 +102,36 @@ int pnh_krylov_r(void* r, const void* v, const KrylovState* st, int64_t n, void*
                  hipStream_t s);
 int pnh_krylov_x(void* x, const void* p, void* r, const void* t, void* rt, const void* qt,
                  const KrylovState* st, int64_t n, hipStream_t s);
+
+// ---- GMRES (scipy semantics: restarted, modified Gram-Schmidt) on P^H P
+#define PNH_GMRES_MAXR 64
+
+struct GmresState {
+  double2 S[PNH_GMRES_MAXR + 1];
+  double2 H[PNH_GMRES_MAXR][PNH_GMRES_MAXR + 1];   // H[col][row]
+  double gc[PNH_GMRES_MAXR];
+  double2 gs[PNH_GMRES_MAXR];
+  double2 y[PNH_GMRES_MAXR];
+  double rtol, atol, tol, bnrm2, h0, vnorm, presid, ptol, ptol_max_factor, rnorm;
+  int restart, maxiter, outer, inner_iter, col_end, breakdown, skip, done, info;
+};
+
+int pnh_gmres_init(GmresState* st, int restart, double rtol, double atol, int maxiter,
+                   hipStream_t s);
+// scalar step `phase` for column col / basis vector k (see solvers.hip)
+int pnh_gmres_scalar(GmresState* st, const void* part, int phase, int col, int k,
+                     hipStream_t s);
+// y = H^-1 S of the cycle (one thread)
+int pnh_gmres_solve(GmresState* st, hipStream_t s);
+// w -= H[col][k] v_k
+int pnh_gmres_mgs(void* w, const void* vk, const GmresState* st, int col, int k, int64_t n,
+                  hipStream_t s);
+// out = in / vnorm (skipped once the cycle has ended)
+int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n,
+                        hipStream_t s);
+// x += sum_{k <= col_end} y_k V[k]
+int pnh_gmres_x(void* x, const void* V, int64_t ldv, const GmresState* st, int64_t n,
+                hipStream_t s);
+// r = b - t with partials of |r|^2
+int pnh_gmres_residual(void* r, const void* b, const void* t, const GmresState* st, int64_t n,
+                       void* part, hipStream_t s);

@@ -940,3 +940,258 @@ int pnh_krylov_x(void* x, const void* p, void* r, const void* t, void* rt, const
          (double2*)rt, (const double2*)qt, st, n);
   return 0;
 }
+
+// ----------------------------------------------------------------- GMRES --
+// scipy.sparse.linalg.gmres (restarted, modified Gram-Schmidt, Givens
+// rotations, no preconditioner) on P^H P in FP64.  The Hessenberg matrix,
+// rotations and right-hand side live in GmresState; the host launches every
+// column of every restart cycle and the kernels skip once a cycle or the
+// solve has ended.
+namespace {
+
+// complex Givens rotation (c real): [c s; -conj(s) c] [f; g] = [r; 0]
+__device__ void zlartg(double2 f, double2 g, double& c, double2& s, double2& r) {
+  if (g.x == 0.0 && g.y == 0.0) {
+    c = 1.0; s = make_double2(0.0, 0.0); r = f;
+    return;
+  }
+  if (f.x == 0.0 && f.y == 0.0) {
+    const double d = hypot(g.x, g.y);
+    c = 0.0;
+    s = make_double2(g.x / d, -g.y / d);
+    r = make_double2(d, 0.0);
+    return;
+  }
+  const double f2 = f.x * f.x + f.y * f.y, g2 = g.x * g.x + g.y * g.y;
+  const double h2 = f2 + g2;
+  c = sqrt(f2 / h2);
+  r = make_double2(f.x / c, f.y / c);
+  const double d = sqrt(f2 * h2);
+  const double2 fd = make_double2(f.x / d, f.y / d);
+  s = zmul(zconj(g), fd);
+}
+
+__global__ void gmres_init_kernel(GmresState* __restrict__ st, int restart, double rtol,
+                                  double atol, int maxiter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->restart = restart;
+  st->rtol = rtol;
+  st->atol = atol;
+  st->maxiter = maxiter;
+  st->ptol_max_factor = 1.0;
+  st->done = 0;
+  st->info = 0;
+  st->outer = 0;
+  st->inner_iter = 0;
+  st->skip = 0;
+}
+
+// phase 0: |b| (tolerances); 1: start of a cycle from |r|; 2: h0 = |w|;
+// 3 (col, k): H[col][k] = vdot(v_k, w); 4 (col): end of a column
+// (h1, rotations, residual estimate); 6: |r| after the cycle, restart
+// tolerance.  skip is set whenever the cycle or the solve has ended.
+__global__ void gmres_scalar_kernel(GmresState* __restrict__ st,
+                                    const double2* __restrict__ part, int phase, int col,
+                                    int k) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) { st->skip = 1; return; }
+  const double eps = 2.220446049250313e-16;
+  const double2 S0 = sum_partials2(part);
+  if (phase == 0) {
+    st->bnrm2 = sqrt(S0.x);
+    st->tol = fmax(st->atol, st->rtol * st->bnrm2);
+    if (st->bnrm2 == 0.0) { st->done = 1; st->skip = 1; st->info = 0; return; }
+    st->ptol = st->bnrm2 * fmin(st->ptol_max_factor, st->tol / st->bnrm2);
+    return;
+  }
+  if (phase == 1) {
+    const double nr = sqrt(S0.x);
+    if (st->outer == 0 && nr < st->tol) { st->done = 1; st->skip = 1; st->info = 0; return; }
+    st->vnorm = nr;
+    for (int i = 0; i <= st->restart; ++i) st->S[i] = make_double2(0.0, 0.0);
+    st->S[0] = make_double2(nr, 0.0);
+    st->breakdown = 0;
+    st->col_end = 0;
+    st->skip = 0;
+    return;
+  }
+  if (st->skip) return;
+  if (phase == 2) { st->h0 = sqrt(S0.x); return; }
+  if (phase == 3) { st->H[col][k] = S0; return; }
+  if (phase == 4) {
+    const double h1 = sqrt(S0.x);
+    st->H[col][col + 1] = make_double2(h1, 0.0);
+    st->vnorm = h1;
+    if (h1 <= eps * st->h0) {
+      st->H[col][col + 1] = make_double2(0.0, 0.0);
+      st->breakdown = 1;
+    }
+    for (int j = 0; j < col; ++j) {
+      const double c = st->gc[j];
+      const double2 s = st->gs[j];
+      const double2 n0 = st->H[col][j], n1 = st->H[col][j + 1];
+      const double2 sn1 = zmul(s, n1), csn0 = zmul(zconj(s), n0);
+      st->H[col][j] = make_double2(c * n0.x + sn1.x, c * n0.y + sn1.y);
+      st->H[col][j + 1] = make_double2(-csn0.x + c * n1.x, -csn0.y + c * n1.y);
+    }
+    double c;
+    double2 s, mag;
+    zlartg(st->H[col][col], st->H[col][col + 1], c, s, mag);
+    st->gc[col] = c;
+    st->gs[col] = s;
+    st->H[col][col] = mag;
+    st->H[col][col + 1] = make_double2(0.0, 0.0);
+    const double2 sc = zmul(zconj(s), st->S[col]);
+    const double2 tmp = make_double2(-sc.x, -sc.y);
+    st->S[col] = make_double2(c * st->S[col].x, c * st->S[col].y);
+    st->S[col + 1] = tmp;
+    st->presid = hypot(tmp.x, tmp.y);
+    st->inner_iter += 1;
+    st->col_end = col;
+    if (st->presid <= st->ptol || st->breakdown) st->skip = 1;
+    return;
+  }
+  // phase 6: rnorm after the cycle
+  const double rnorm = sqrt(S0.x);
+  st->rnorm = rnorm;
+  st->outer += 1;
+  if (rnorm <= st->tol || st->breakdown) {
+    st->done = 1;
+  } else {
+    if (st->presid <= st->ptol) st->ptol_max_factor = fmax(eps, 0.25 * st->ptol_max_factor);
+    else st->ptol_max_factor = fmin(1.0, 1.5 * st->ptol_max_factor);
+    st->ptol = st->presid * fmin(st->ptol_max_factor, st->tol / rnorm);
+    if (st->outer >= st->maxiter) st->done = 1;
+  }
+  if (st->done) st->info = rnorm <= st->tol ? 0 : st->maxiter;
+  st->skip = st->done;
+}
+
+// the cycle's triangular solve (one thread); runs whether or not the last
+// columns were skipped
+__global__ void gmres_solve_kernel(GmresState* __restrict__ st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) return;
+  const int col = st->col_end;
+  if (st->H[col][col].x == 0.0 && st->H[col][col].y == 0.0)
+    st->S[col] = make_double2(0.0, 0.0);
+  for (int i = 0; i <= col; ++i) st->y[i] = st->S[i];
+  for (int kk = col; kk > 0; --kk) {
+    if (st->y[kk].x != 0.0 || st->y[kk].y != 0.0) {
+      st->y[kk] = cdivd(st->y[kk], st->H[kk][kk]);
+      const double2 t = st->y[kk];
+      for (int i = 0; i < kk; ++i) {
+        const double2 th = zmul(t, st->H[kk][i]);
+        st->y[i] = make_double2(st->y[i].x - th.x, st->y[i].y - th.y);
+      }
+    }
+  }
+  if (st->y[0].x != 0.0 || st->y[0].y != 0.0) st->y[0] = cdivd(st->y[0], st->H[0][0]);
+}
+
+// w -= H[col][k] v_k
+__global__ void gmres_mgs_kernel(double2* __restrict__ w, const double2* __restrict__ vk,
+                                 const GmresState* __restrict__ st, int col, int k,
+                                 int64_t n) {
+  if (st->done || st->skip) return;
+  const double2 t = st->H[col][k];
+  GRID_STRIDE(i, n) {
+    const double2 tv = zmul(t, vk[i]);
+    w[i] = make_double2(w[i].x - tv.x, w[i].y - tv.y);
+  }
+}
+
+// out = in / vnorm: the next basis vector (not needed, so skipped, once the
+// cycle or the solve has ended)
+__global__ void gmres_normalize_kernel(double2* __restrict__ out, const double2* __restrict__ in,
+                                       const GmresState* __restrict__ st, int64_t n) {
+  if (st->done || st->skip) return;
+  const double nv = st->vnorm;
+  const double inv = nv > 0.0 ? 1.0 / nv : 1.0;
+  GRID_STRIDE(i, n) {
+    double2 v = in[i];
+    if (nv > 0.0) { v.x *= inv; v.y *= inv; }
+    out[i] = v;
+  }
+}
+
+// x += sum_{k <= col_end} y_k v_k
+__global__ void gmres_x_kernel(double2* __restrict__ x, const double2* __restrict__ V,
+                               int64_t ldv, const GmresState* __restrict__ st, int64_t n) {
+  if (st->done) return;
+  const int col = st->col_end;
+  GRID_STRIDE(i, n) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kk = 0; kk <= col; ++kk) {
+      const double2 t = zmul(st->y[kk], V[kk * ldv + i]);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    x[i] = make_double2(x[i].x + acc.x, x[i].y + acc.y);
+  }
+}
+
+// r = b - t; partials of |r|^2
+__global__ void __launch_bounds__(256)
+gmres_residual_kernel(double2* __restrict__ r, const double2* __restrict__ b,
+                      const double2* __restrict__ t, const GmresState* __restrict__ st,
+                      int64_t n, double2* __restrict__ part) {
+  if (st->done) return;
+  double acc = 0.0;
+  GRID_STRIDE(i, n) {
+    const double2 v = make_double2(b[i].x - t[i].x, b[i].y - t[i].y);
+    r[i] = v;
+    acc += v.x * v.x + v.y * v.y;
+  }
+  block_sum2_store(make_double2(acc, 0.0), part);
+}
+
+}  // namespace
+
+int pnh_gmres_init(GmresState* st, int restart, double rtol, double atol, int maxiter,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(gmres_init_kernel, dim3(1), dim3(64), 0, s, st, restart, rtol, atol,
+                     maxiter);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_gmres_scalar(GmresState* st, const void* part, int phase, int col, int k,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(gmres_scalar_kernel, dim3(1), dim3(64), 0, s, st, (const double2*)part,
+                     phase, col, k);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_gmres_solve(GmresState* st, hipStream_t s) {
+  hipLaunchKernelGGL(gmres_solve_kernel, dim3(1), dim3(64), 0, s, st);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_gmres_mgs(void* w, const void* vk, const GmresState* st, int col, int k, int64_t n,
+                  hipStream_t s) {
+  LAUNCH(gmres_mgs_kernel, n, (double2*)w, (const double2*)vk, st, col, k, n);
+  return 0;
+}
+
+int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n,
+                        hipStream_t s) {
+  LAUNCH(gmres_normalize_kernel, n, (double2*)out, (const double2*)in, st, n);
+  return 0;
+}
+
+int pnh_gmres_x(void* x, const void* V, int64_t ldv, const GmresState* st, int64_t n,
+                hipStream_t s) {
+  LAUNCH(gmres_x_kernel, n, (double2*)x, (const double2*)V, ldv, st, n);
+  return 0;
+}
+
+int pnh_gmres_residual(void* r, const void* b, const void* t, const GmresState* st, int64_t n,
+                       void* part, hipStream_t s) {
+  hipLaunchKernelGGL(gmres_residual_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s,
+                     (double2*)r, (const double2*)b, (const double2*)t, st, n, (double2*)part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}

@@ -478,6 +478,10 @@ class NUFFT:
           normal equations in FP64 (``rtol``, ``atol``, ``maxiter``; plans
           without ``batch``); ``self.bicg_info`` then holds (scipy's info,
           iterations).
+        * ``'gmres'``: scipy's restarted GMRES on the same normal equations
+          in FP64 (``restart`` <= 64, ``maxiter`` cycles, ``rtol``,
+          ``atol``; plans without ``batch``); ``self.bicg_info`` then holds
+          (scipy's info, inner iterations).
         * ``'dc'``: density-compensated adjoint, ``maxiter`` pipe-density
           iterations (``solve_cpu.py:165-183``).
         * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
@@ -489,7 +493,8 @@ class NUFFT:
           ``batch``).  ``self.lsqr_info`` then holds (istop, iterations).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg, bicgstab, bicg, dc, L1TVOLS, lsqr, lsmr')
+            raise KeyError('solver must be one of: cg, bicgstab, bicg, gmres, dc, L1TVOLS, lsqr, '
+                           'lsmr')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
         if solver == 'dc':
@@ -502,9 +507,12 @@ class NUFFT:
             return self._solve_lsmr(y, *args, **kwargs)
         if solver in ('bicgstab', 'bicg'):
             return self._solve_bicg(y, solver, *args, **kwargs)
+        if solver == 'gmres':
+            return self._solve_gmres(y, *args, **kwargs)
         if solver != 'cg':
             raise NotImplementedError("solver %r is not provided; use 'cg', 'bicgstab', "
-                                      "'bicg', 'dc', 'L1TVOLS', 'lsqr' or 'lsmr'" % (solver,))
+                                      "'bicg', 'gmres', 'dc', 'L1TVOLS', 'lsqr' or 'lsmr'"
+                                      % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -595,9 +603,35 @@ class NUFFT:
         self._bicg_info = info
         return self._out(out, is_np)
 
+    def _solve_gmres(self, y, restart=None, maxiter=None, rtol=None, atol=0.0, tol=None,
+                     x0=None, M=None, callback=None, callback_type=None):
+        """scipy.sparse.linalg.gmres semantics on P^H P, x0 = 0."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('gmres solves plans without batch')
+        if x0 is not None or M is not None or callback is not None:
+            raise NotImplementedError('gmres: x0, M and callback are not supported')
+        rtol = rtol if rtol is not None else (tol if tol is not None else 1e-5)
+        restart = int(restart) if restart is not None else 0
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        n = ctypes.c_size_t()
+        with torch.cuda.device(self.device):
+            _lib.check(self._libh.pnh_gmres_workspace_bytes(self._plan, restart,
+                                                            ctypes.byref(n)))
+        ws = torch.empty(max(int(n.value), 256), dtype=torch.uint8, device=self.device)
+        info = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._call(self._libh.pnh_solve_gmres, _ptr(yt), _ptr(out), restart,
+                   int(maxiter) if maxiter is not None else 0, float(rtol), float(atol),
+                   _ptr(ws), _ptr(info), self._stream())
+        self._bicg_info = info
+        return self._out(out, is_np)
+
     @property
     def bicg_info(self):
-        """(scipy's info, iterations) of the last 'bicgstab' or 'bicg' solve."""
+        """(scipy's info, iterations) of the last 'bicgstab', 'bicg' or 'gmres'
+        solve."""
         return tuple(int(v) for v in self._bicg_info.cpu().numpy())
 
     def _solve_lsmr(self, y, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8, maxiter=None,

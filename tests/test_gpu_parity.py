@@ -199,7 +199,7 @@ def test_solver_argument_errors(hip):
     with pytest.raises(TypeError):
         A.solve(y, 'cgs')
     with pytest.raises(NotImplementedError):
-        A.solve(y, 'gmres')
+        A.solve(y, 'lgmres')
     with pytest.raises(ValueError):
         A.forward(np.zeros((3, 3), np.complex64))
 

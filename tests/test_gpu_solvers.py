@@ -197,3 +197,34 @@ def test_cg_default_maxiter_converges(hip):
     assert A.cg_iters == its[0]
     assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
     assert dt < 5.0
+
+
+@pytest.mark.parametrize('name', SOLVER_CASES)
+def test_solve_gmres(hip, name):
+    """scipy's gmres (restart 4, 2 cycles) on P^H P in FP64 against the
+    reference."""
+    A, g = make(name)
+    s = golden('solvers')
+    x = A.solve(g['adjoint_in'], 'gmres', restart=4, maxiter=2)
+    assert rel(x, s[name + '__gmres']) < TOL
+
+
+def test_gmres_stops_like_scipy(hip):
+    """Default restart (20), to scipy's stopping test: same info and image
+    as scipy's gmres on the oracle's operator; restarts are exercised."""
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make('tiny1d')
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y = g['adjoint_in']
+    K = B.Kdprod
+
+    def mv(v):
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    for restart in (None, 5):
+        k, info = sla.gmres(op, B.y2k(y).ravel(), restart=restart, maxiter=40)
+        x = A.solve(y, 'gmres', restart=restart, maxiter=40)
+        assert A.bicg_info[0] == info
+        assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
