@@ -256,8 +256,8 @@ def bench_other_workloads(torch, W, steps=20):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--workload', default='uniform3d',
                     choices=['uniform3d', 'radial2d', 'plain2d'])
     ap.add_argument('--legacy', action='store_true',
