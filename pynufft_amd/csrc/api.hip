@@ -73,9 +73,11 @@ InterpArgs interp_args(const pnh_plan* p, int fp64) {
   return a;
 }
 
+// gridding scratch: item exponents, then the partial slots
 size_t partial_bytes(const pnh_plan* p, int fp64) {
   const size_t esz = fp64 ? 16 : 8;
-  return (size_t)p->d.n_slots * p->bins.pad_cells * p->slot_nbc * esz;
+  return pnh_item_exponent_bytes(p->bins.n_items) +
+         (size_t)p->d.n_slots * p->bins.pad_cells * p->slot_nbc * esz;
 }
 
 // Build the line-FFT descriptor for axis `a`.
@@ -213,8 +215,8 @@ int check_dtype_ready(const pnh_plan* p, int dtype) {
 int do_y2k(const pnh_plan* p, int fp64, const void* y, void* k, void* part,
            const int* skip, hipStream_t s) {
   InterpArgs ia = interp_args(p, fp64);
-  if (p->d.n_slots > 0 && !part) {
-    pnh_set_error("workspace required for split gridding tiles");
+  if (p->bins.n_items > 0 && !part) {
+    pnh_set_error("workspace required for gridding");
     return -1;
   }
   return pnh_y2k_launch(ia, p->bins, fp64, y, k, part, skip, s);

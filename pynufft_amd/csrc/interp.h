@@ -63,9 +63,15 @@ int pnh_k2y_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* 
 // out[m*D + d] (+)= sum_c Re(conj(g[m, c]) * sum_j d_d coef(m, j) * grid[cell(m, j), c])
 int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
                        const void* g, void* out, hipStream_t s);
+// Bytes at the head of the gridding scratch for the work items' fixed-point
+// exponents (int32 each, rounded up to 256 B).
+inline size_t pnh_item_exponent_bytes(int64_t n_items) {
+  return ((size_t)n_items * 4 + 255) & ~(size_t)255;
+}
 // grid[cell*nb + c] = sum over rows r of conj(coef(r, j)) * y[perm[r]*nb + c]
-// for every support cell of r; `part` holds n_slots * pad_cells * min(nb, 4)
-// elements of the operator's complex type.
+// for every support cell of r; `part` holds pnh_item_exponent_bytes(n_items)
+// then n_slots * pad_cells * min(nb, 4) elements of the operator's complex
+// type.
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);
