@@ -22,6 +22,9 @@ from . import helper
 
 _c64 = torch.complex64
 _c128 = torch.complex128
+# current HIP stream of a device as an integer handle, without building a
+# Stream object per call (torch's own fast path; absent -> public API)
+_raw_stream = getattr(torch._C, '_cuda_getCurrentRawStream', None)
 
 
 def lanes_per_sample(J_last):
@@ -63,8 +66,11 @@ class NUFFT:
             dev = device_indx
         else:
             dev = torch.device('cuda', int(device_indx))
+        if dev.index is None:
+            dev = torch.device('cuda', torch.cuda.current_device())
         self._libh = lib
         self.device = dev
+        self._dev_index = dev.index
         self.processor = 'hip'
         self.legacy = bool(legacy)
         self.dtype = np.complex64
@@ -286,6 +292,8 @@ class NUFFT:
             raise RuntimeError('call plan() first')
 
     def _stream(self):
+        if _raw_stream is not None:
+            return ctypes.c_void_p(_raw_stream(self._dev_index))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _workspace(self, op, dtype):
@@ -334,8 +342,12 @@ class NUFFT:
         return torch.empty(self._shape(kind), dtype=dtype, device=self.device)
 
     def _call(self, fn, *args):
-        with torch.cuda.device(self.device):
+        # the library launches on the current device: switch only when needed
+        if torch.cuda.current_device() == self._dev_index:
             _lib.check(fn(self._plan, *args))
+        else:
+            with torch.cuda.device(self.device):
+                _lib.check(fn(self._plan, *args))
 
     # --------------------------------------------------------- stages --
     def x2xx(self, x):
