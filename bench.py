@@ -189,8 +189,9 @@ def bench_radial_solvers(torch, W, maxiter=30, reps=3, cpu=True):
     y = A.forward(x)
     A.precompute_sp()
     t_cg = _time_host(torch, lambda: A.solve(y, 'cg', maxiter=maxiter, rtol=0.0), reps)
+    t_cgg = _time_host(torch, lambda: A.solve_graph(y, maxiter=maxiter, rtol=0.0), reps)
     cg = {'workload': 'radial2d', 'cg_iterations': maxiter, 'gpu_solves_per_s': 1.0 / t_cg,
-          'gpu_ms_per_solve': 1000.0 * t_cg}
+          'gpu_ms_per_solve': 1000.0 * t_cg, 'gpu_graph_solves_per_s': 1.0 / t_cgg}
     runs = {
         'dc3': (lambda: A.solve(y, 'dc', maxiter=3), lambda B, yc, xc: B.solve(yc, 'dc', maxiter=3)),
         'l1tvols10_rho1': (lambda: A.solve(y, 'L1TVOLS', maxiter=10, rho=1.0),

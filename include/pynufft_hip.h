@@ -276,6 +276,13 @@ int pnh_batch_solve_cg(const pnh_batch* batch, const void* y, void* x, int maxit
 int pnh_batch_cg_graph_create(const pnh_batch* batch, const void* y, void* x, int maxiter,
                               double rtol, double atol, void* ws, int32_t* iters_out,
                               pnh_graph** out);
+/* Capture pnh_solve_cg of one plan with these buffers into a HIP graph
+ * (every iteration launched; converged coils skip their work on the device);
+ * replay with pnh_graph_launch.  No counterpart in the reference (new): for
+ * repeated solves on one trajectory without per-kernel launch cost. */
+int pnh_cg_graph_create(const pnh_plan* plan, const void* y, void* x, int maxiter,
+                        double rtol, double atol, void* ws, int32_t* iters_out,
+                        pnh_graph** out);
 int pnh_graph_launch(pnh_graph* graph, void* stream);
 int pnh_graph_destroy(pnh_graph* graph);
 

@@ -118,6 +118,8 @@ EXPORTS = {
     'pnh_batch_cg_graph_create': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                                  ctypes.c_double, _vp, _vp,
                                                  ctypes.POINTER(_vp)]),
+    'pnh_cg_graph_create': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, _vp, _vp, ctypes.POINTER(_vp)]),
     'pnh_graph_launch': (ctypes.c_int, [_vp, _vp]),
     'pnh_graph_destroy': (ctypes.c_int, [_vp]),
 }

@@ -968,8 +968,11 @@ int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* 
   return pnh_k2y_dom_launch(ia, 0, k, g, om_grad, (hipStream_t)stream);
 }
 
-int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
-                 double atol, void* ws, int32_t* iters_out, void* stream) {
+// The CG solve; host_checks: read the convergence flag back every 64
+// iterations and stop launching (off while a graph is being captured).
+static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
+                    double atol, void* ws, int32_t* iters_out, void* stream,
+                    bool host_checks) {
   REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
   REQUIRE(maxiter >= 0, "maxiter must be >= 0");
   REQUIRE(p->nb <= PNH_CG_MAXNB, "solve supports at most %d coils", PNH_CG_MAXNB);
@@ -1009,7 +1012,7 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_PQ, it, maxiter, rtol, atol, s))) return rc;
     if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redp, s))) return rc;
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_RR, it, maxiter, rtol, atol, s))) return rc;
-    if ((it & 63) == 63 && it + 1 < maxiter) {
+    if (host_checks && (it & 63) == 63 && it + 1 < maxiter) {
       // long runs: stop launching once every coil has converged
       int flag = 0;
       PNH_CHECK_HIP(hipMemcpyAsync(&flag, all_done, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1025,6 +1028,11 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
     PNH_CHECK_HIP(hipMemcpyAsync(iters_out, sc->iters, sizeof(int) * nb,
                                  hipMemcpyDeviceToDevice, s));
   return 0;
+}
+
+int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
+                 double atol, void* ws, int32_t* iters_out, void* stream) {
+  return cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, stream, true);
 }
 
 }  // extern "C"
@@ -1431,6 +1439,40 @@ int pnh_batch_cg_graph_create(const pnh_batch* b, const void* y, void* x, int ma
     return -3;
   }
   const int rc = batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, cs);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(cs, &g);
+  (void)hipStreamDestroy(cs);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) {
+    pnh_set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+    return -3;
+  }
+  pnh_graph* gr = new (std::nothrow) pnh_graph;
+  if (!gr) { (void)hipGraphDestroy(g); pnh_set_error("out of host memory"); return -1; }
+  e = hipGraphInstantiate(&gr->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    delete gr;
+    pnh_set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+    return -3;
+  }
+  *out = gr;
+  return 0;
+}
+
+int pnh_cg_graph_create(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
+                        double atol, void* ws, int32_t* iters_out, pnh_graph** out) {
+  REQUIRE(p && x && ws && out, "null argument");
+  *out = nullptr;
+  hipStream_t cs;
+  PNH_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cs);
+    pnh_set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
+    return -3;
+  }
+  const int rc = cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, cs, false);
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);

@@ -85,6 +85,7 @@ class NUFFT:
         self._ws = {}
         self._sense = None
         self.W = None
+        self._graphs = {}
 
     # ------------------------------------------------------------ plan --
     def plan(self, om, Nd, Kd, Jd, ft_axes=None, batch=None, radix=None,
@@ -274,6 +275,9 @@ class NUFFT:
     def release(self):
         """Free the device plan and buffers (reference ``release``,
         ``_nufft_class_methods_device.py:653-683``)."""
+        for g in getattr(self, '_graphs', {}).values():
+            self._libh.pnh_graph_destroy(g[0])
+        self._graphs = {}
         if getattr(self, '_plan', None) is not None:
             self._libh.pnh_plan_destroy(self._plan)
             self._plan = None
@@ -544,6 +548,37 @@ class NUFFT:
                    float(rtol), float(atol), _ptr(ws), _ptr(iters), self._stream())
         self._cg_iters_dev = iters
         return self._out(out, is_np)
+
+    def solve_graph(self, y, maxiter=30, rtol=1e-5, atol=0.0):
+        """``solve(y, 'cg', maxiter=..., rtol=..., atol=...)`` through a
+        captured HIP graph, built on first use for these settings (input and
+        output go through buffers the graph owns); for repeated solves on one
+        trajectory without per-kernel launch cost.  Same result as
+        ``solve``."""
+        self._need_plan()
+        self._ensure_fp64()
+        key = (int(maxiter), float(rtol), float(atol))
+        g = self._graphs.get(key)
+        if g is None:
+            yb = self._empty('y')
+            xb = self._empty('x')
+            it = torch.zeros(self._nb, dtype=torch.int32, device=self.device)
+            ws = torch.empty_like(self._workspace(_lib.OP_SOLVE_CG, _lib.C128))
+            h = ctypes.c_void_p()
+            torch.cuda.synchronize(self.device)
+            with torch.cuda.device(self.device):
+                _lib.check(self._libh.pnh_cg_graph_create(
+                    self._plan, _ptr(yb), _ptr(xb), key[0], key[1], key[2], _ptr(ws),
+                    _ptr(it), ctypes.byref(h)))
+            g = (h, yb, xb, it, ws)
+            self._graphs[key] = g
+        h, yb, xb, it, ws = g
+        yt, is_np = self._in(y, 'y')
+        yb.copy_(yt)
+        with torch.cuda.device(self.device):
+            _lib.check(self._libh.pnh_graph_launch(h, self._stream()))
+        self._cg_iters_dev = it
+        return self._out(xb.clone(), is_np)
 
     def _solve_dc(self, y, maxiter):
         """x = adjoint(W y), W from ``maxiter`` pipe-density iterations

@@ -228,3 +228,23 @@ def test_gmres_stops_like_scipy(hip):
         x = A.solve(y, 'gmres', restart=restart, maxiter=40)
         assert A.bicg_info[0] == info
         assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
+
+
+@pytest.mark.parametrize('name,maxiter,rtol', [('plain2d', 12, 0.0), ('tiny2d', 60, 1e-5)])
+def test_cg_graph_replays_match_eager(hip, name, maxiter, rtol):
+    """solve_graph replays a captured CG.  Each replay, with eager solves and
+    synchronizations in between and new data every time, equals the eager
+    solve bit for bit, also when the CG converges before maxiter (the
+    graph's remaining iterations skip on the device flag)."""
+    import torch
+    A, g = make(name)
+    y = g['adjoint_in']
+    for rep in range(4):
+        yr = (y * (1.0 + 0.5 * rep)).astype(np.complex64)
+        eager = A.solve(yr, 'cg', maxiter=maxiter, rtol=rtol)
+        it_eager = A.cg_iters
+        torch.cuda.synchronize()
+        x = A.solve_graph(yr, maxiter=maxiter, rtol=rtol)
+        assert np.array_equal(x, eager), rep
+        assert A.cg_iters == it_eager
+        torch.cuda.synchronize()
