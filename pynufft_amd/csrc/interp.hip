@@ -780,21 +780,23 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
           pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
         }
         if (!ok) continue;
-        // a heavy tile has many slots: four running sums (slots k = j mod 4,
-        // in row order each) keep four loads in flight, combined as
-        // (s0 + s1) + (s2 + s3) -- a fixed order, so still reproducible
+        // a heavy tile has many slots: NQ running sums (slots k = j mod NQ,
+        // in row order each) keep NQ loads in flight; they are combined
+        // pairwise in a fixed tree, so the result stays reproducible.  Eight
+        // for one coil, four for several (register budget).
+        constexpr int NQ = NBC == 1 ? 8 : 4;
         const int ns = src[mask].y;
         const int64_t stride = b.pad_cells * NBC;
         const C* p = part + ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC;
-        C q[4][NBC];
+        C q[NQ][NBC];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < NQ; ++j)
 #pragma unroll
           for (int cc = 0; cc < NBC; ++cc) q[j][cc] = mk<T>((T)0, (T)0);
         int k = 0;
-        for (; k + 4 <= ns; k += 4) {
+        for (; k + NQ <= ns; k += NQ) {
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < NQ; ++j)
 #pragma unroll
             for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[j][cc], p[(k + j) * stride + cc]);
         }
@@ -802,8 +804,13 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
 #pragma unroll
           for (int cc = 0; cc < NBC; ++cc) q[0][cc] = cadd(q[0][cc], p[k * stride + cc]);
 #pragma unroll
-        for (int cc = 0; cc < NBC; ++cc)
-          s[cc] = cadd(s[cc], cadd(cadd(q[0][cc], q[1][cc]), cadd(q[2][cc], q[3][cc])));
+        for (int w = NQ / 2; w >= 1; w /= 2)
+#pragma unroll
+          for (int j = 0; j < w; ++j)
+#pragma unroll
+            for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[2 * j][cc], q[2 * j + 1][cc]);
+#pragma unroll
+        for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], q[0][cc]);
       }
     }
     int64_t g = 0;
