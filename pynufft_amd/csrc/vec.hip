@@ -165,6 +165,7 @@ cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
       }
     }
   }
+  if (nb == 1) s->all_done = s->done[0];   // one coil: no separate AND pass
 }
 
 // all_done = AND over coils (single thread, runs after every finalize)
@@ -319,8 +320,11 @@ int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter
                      (const double2*)part, PNH_RED_BLOCKS, nb, sc, phase, iter, maxiter,
                      rtol, atol);
   PNH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(cg_all_done_kernel, dim3(1), dim3(64), 0, s, sc, nb);
-  PNH_LAUNCH_CHECK();
+  // the p.q phase changes no coil's state; one coil sets all_done itself
+  if (nb > 1 && phase != CG_PHASE_PQ) {
+    hipLaunchKernelGGL(cg_all_done_kernel, dim3(1), dim3(64), 0, s, sc, nb);
+    PNH_LAUNCH_CHECK();
+  }
   return 0;
 }
 
