@@ -685,8 +685,10 @@ k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
 // tile.  A cell of tile t sums, in a fixed order, the slots of the tiles
 // whose padded box holds it: neighbour masks 0 .. 2^D - 1 (bit d: the tile
 // below along axis d, whose padding reaches the J_d - 1 lowest cells of t),
-// then each tile's slots in row order.  The tile's own slot is read
-// contiguously and every slot cell is read exactly once over the launch.
+// and within a mask the tile's slots (several when the tile's rows were split
+// into several work items: NQ running sums over the slots, combined in a
+// fixed tree).  The tile's own slot is read contiguously and every slot cell
+// is read exactly once over the launch.
 template <typename T, int D, int NBC>
 __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& b, int tid,
                                             int first, int count,
