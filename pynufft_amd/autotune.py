@@ -30,7 +30,7 @@ from . import helper
 
 _TILES = {
     1: [(1024,), (2048,), (4096,), (512,)],
-    2: [(32, 32), (64, 64), (64, 32), (32, 64), (16, 16), (128, 64)],
+    2: [(16, 16), (32, 32), (16, 32), (32, 16), (8, 8), (64, 64)],
     3: [(16, 16, 16), (16, 16, 8), (16, 8, 8), (8, 16, 16), (8, 8, 8), (24, 16, 8)],
 }
 _CHUNKS = (256, 512, 1024, 2048, 4096)
@@ -176,7 +176,7 @@ def batch_candidates(problems):
     (None) or a common smaller or larger one where valid for every problem,
     with rows per item and workgroup sizes around the defaults."""
     d = len(problems[0][2])
-    tiles = [None] + {1: [(512,), (2048,)], 2: [(16, 16), (64, 64)],
+    tiles = [None] + {1: [(512,), (2048,)], 2: [(8, 8), (32, 32)],
                       3: [(16, 16, 8), (8, 8, 8)]}[d]
     out = []
     for t in tiles:

@@ -228,6 +228,17 @@ int do_k2y(const pnh_plan* p, int fp64, const void* k, void* y, const int* skip,
   return pnh_k2y_launch(ia, p->bins, fp64, k, y, skip, s);
 }
 
+// k_out = P^H P k, y (samples) receives P k; k_out may alias k
+int do_k2y2k(const pnh_plan* p, int fp64, const void* k, void* y, void* k_out, void* part,
+             const int* skip, hipStream_t s) {
+  InterpArgs ia = interp_args(p, fp64);
+  if (p->bins.n_items > 0 && !part) {
+    pnh_set_error("workspace required for gridding");
+    return -1;
+  }
+  return pnh_k2y2k_launch(ia, p->bins, fp64, k, y, k_out, part, skip, s);
+}
+
 
 // Validate a descriptor and fill a host-side plan (no device work).
 int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
@@ -477,8 +488,7 @@ int pnh_k2y2k(const pnh_plan* p, int dtype, const void* k, void* k_out, void* ws
   void* ytmp = w;
   void* part = w + align_up((size_t)p->M * p->nb * esz);
   hipStream_t s = (hipStream_t)stream;
-  if (p->M > 0 && (rc = do_k2y(p, fp64, k, ytmp, nullptr, s))) return rc;
-  return do_y2k(p, fp64, ytmp, k_out, part, nullptr, s);
+  return do_k2y2k(p, fp64, k, ytmp, k_out, part, nullptr, s);
 }
 
 int pnh_forward(const pnh_plan* p, const void* x, void* y, void* ws, void* stream) {
@@ -513,8 +523,7 @@ int pnh_selfadjoint(const pnh_plan* p, const void* x, void* x_out, void* ws, voi
   void* ytmp = w + align_up((size_t)p->Ktot * p->nb * 8);
   void* part = (char*)ytmp + align_up((size_t)p->M * p->nb * 8);
   if ((rc = run_passes_forward(p, x, grid, true, 0, nullptr, s))) return rc;
-  if (p->M > 0 && (rc = do_k2y(p, 0, grid, ytmp, nullptr, s))) return rc;
-  if ((rc = do_y2k(p, 0, ytmp, grid, part, nullptr, s))) return rc;
+  if ((rc = do_k2y2k(p, 0, grid, ytmp, grid, part, nullptr, s))) return rc;
   return run_passes_inverse(p, grid, x_out, true, 0, nullptr, s);
 }
 
@@ -815,9 +824,7 @@ int pnh_solve_bicg(const pnh_plan* p, int kind, const void* y, void* x, int maxi
   const int* done = &st->done;
   const int* skip_t = &st->skip_t;
   auto normal_op = [&](const void* in, void* out, const int* skip) -> int {
-    int e;
-    if (M > 0 && (e = do_k2y(p, 1, in, ym, skip, s))) return e;
-    return do_y2k(p, 1, ym, out, part, skip, s);
+    return do_k2y2k(p, 1, in, ym, out, part, skip, s);
   };
   // b = P^H y; r = b, rt = r, x = 0
   if ((rc = pnh_krylov_init(st, stab ? 1 : 0, rtol, atol, maxiter, s))) return rc;
@@ -910,9 +917,7 @@ int pnh_solve_gmres(const pnh_plan* p, const void* y, void* x, int restart, int 
   const int64_t ldv = (int64_t)(kv / 16);
   auto vec = [&](int k) -> void* { return V + (size_t)k * kv; };
   auto normal_op = [&](const void* in, void* out, const int* sk) -> int {
-    int e;
-    if (M > 0 && (e = do_k2y(p, 1, in, ym, sk, s))) return e;
-    return do_y2k(p, 1, ym, out, part, sk, s);
+    return do_k2y2k(p, 1, in, ym, out, part, sk, s);
   };
   if ((rc = pnh_gmres_init(st, restart, rtol, atol, maxiter, s))) return rc;
   if (M > 0 && (rc = pnh_c64_to_c128(y, ym, M, s))) return rc;
@@ -1006,8 +1011,7 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_INIT, 0, maxiter, rtol, atol, s))) return rc;
   for (int it = 0; it < maxiter; ++it) {
     if ((rc = pnh_cg_p_update(vp, vr, K, nb, sc, it == 0, s))) return rc;
-    if (p->M > 0 && (rc = do_k2y(p, 1, vp, yt, all_done, s))) return rc;
-    if ((rc = do_y2k(p, 1, yt, vq, part, all_done, s))) return rc;
+    if ((rc = do_k2y2k(p, 1, vp, yt, vq, part, all_done, s))) return rc;
     if ((rc = pnh_cg_dot(0, vp, vq, K, nb, redp, all_done, s))) return rc;
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_PQ, it, maxiter, rtol, atol, s))) return rc;
     if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redp, s))) return rc;
@@ -1321,10 +1325,9 @@ int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double 
   for (int it = 0; it < maxiter; ++it) {
     if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, b->vec_t[0], ch, w.p, w.r, nullptr,
                                    nullptr, cg, it == 0, s))) return rc;
-    if ((rc = pnh_batch_k2y_launch(b->probs, b->items, b->n_items, b->D, b->J, 1,
-                                   b->pad_cells_max, b->grid_threads, w.p, w.yt, cg.done,
-                                   cg.all_done, s))) return rc;
-    if ((rc = batch_y2k(b, 1, w.yt, w.q, w.part, cg.done, cg.all_done, s))) return rc;
+    if ((rc = pnh_batch_k2y2k_launch(b->probs, b->items, b->n_items, b->gather_t, b->D, b->J,
+                                     1, b->pad_cells_max, b->grid_threads, w.p, w.yt, w.q,
+                                     w.part, cg.done, cg.all_done, s))) return rc;
     if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, b->vec_t[0], ch, w.p, w.q, nullptr,
                                    nullptr, cg, 0, s))) return rc;
     if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_PQ, it, maxiter, rtol,

@@ -47,6 +47,12 @@ int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n
                          void* y, const int* done, const int* all_done, hipStream_t s);
 // Grids every problem's items into partial slots, then gathers the grids
 // (one block per tile, by table `gather`).
+// k_out = P^H P k for every problem (fused k2y + y2k per work item, then the
+// gather); y receives P k.
+int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
+                           const BatchTable& gather, int D, const int* Jd, int fp64,
+                           int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
+                           void* part, const int* done, const int* all_done, hipStream_t s);
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
                          const BatchTable& gather, int D, const int* Jd, int fp64,
                          int64_t pad_cells, int threads, const void* y, void* k, void* part,

@@ -75,6 +75,11 @@ inline size_t pnh_item_exponent_bytes(int64_t n_items) {
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);
+// grid = P^H P k (y2k of k2y): y (the sample buffer) receives P k; one fused
+// gridding pass per work item for one coil and per-axis factors, else k2y
+// then y2k.  `part` as for pnh_y2k_launch.  grid may alias k.
+int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* k, void* y,
+                     void* grid, void* part, const int* skip, hipStream_t s);
 // LDS bytes available to one workgroup on the current device.
 size_t pnh_lds_limit();
 // Coils gridded per pass for this tile and LDS limit (0 if none fits).

@@ -1008,6 +1008,92 @@ int launch_dom_j(const InterpArgs& a, const void* grid, const void* g, void* out
 
 }  // namespace
 
+namespace {
+
+// P^H P on one work item: k2y of the item's rows from the padded box of the
+// input grid, then y2k of those rows into the same-sized accumulator (the
+// rows' support boxes lie in the tile's padded box for both).  The box and the
+// accumulator share the LDS (the box is dead once y is formed); y goes
+// through the caller's sample buffer, which the barrier makes visible to the
+// whole workgroup, so the factors and samples of the second phase come from
+// L2 rather than HBM.  One coil, per-axis factors.
+template <typename T, int D, int JN, int JE>
+__device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
+                                           long long* smem, const void* __restrict__ k_,
+                                           void* __restrict__ y_, void* __restrict__ part_) {
+  typedef typename cplx<T>::t C;
+  k2y_item<T, D, false, 1, JN, JE>(a, b, it, reinterpret_cast<C*>(smem), k_, y_, 0);
+  __syncthreads();
+  y2k_item<T, D, false, 1, JN, JE>(a, b, it, smem, y_, part_, 0, nullptr);
+}
+
+template <typename T, int D, int JN, int JE>
+__global__ void __launch_bounds__(1024)
+k2y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ k_, void* __restrict__ y_,
+                  void* __restrict__ part_, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ long long smem_kk[];
+  const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
+  k2y2k_item<T, D, JN, JE>(a, b, it, smem_kk, k_, y_, part_);
+}
+
+template <typename T, int D>
+int launch_k2y2k(const InterpArgs& a, const BinArgs& b, const void* k, void* y, void* grid,
+                 void* part, const int* skip, hipStream_t s) {
+  void* slots = (char*)part + pnh_item_exponent_bytes(b.n_items);
+  const size_t lds = y2k_lds_bytes(b.pad_cells, 1);
+  if (b.n_items > 0) {
+    int rc = dispatch_j<D, false>(a, [&](auto jn, auto je) {
+      auto kern = k2y2k_tile_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, k, y,
+                         slots, skip);
+      PNH_LAUNCH_CHECK();
+      return 0;
+    });
+    if (rc) return rc;
+  }
+  const int parts = pnh_gather_parts(b);
+  hipLaunchKernelGGL((y2k_gather_kernel<T, D, 1>), dim3((unsigned)(b.n_tiles * parts)),
+                     dim3(256), 0, s, a, b, parts, slots, grid, 0, skip);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // namespace
+
+int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* k, void* y,
+                     void* grid, void* part, const int* skip, hipStream_t s) {
+  const size_t esz = fp64 ? 16 : 8;
+  const bool fused = a.nb == 1 && !a.csr_val && b.threads >= 64 && b.threads <= 1024 &&
+                     b.threads % 64 == 0 &&
+                     y2k_lds_bytes(b.pad_cells, 1) <= pnh_lds_limit() &&
+                     (size_t)b.pad_cells * esz <= y2k_lds_bytes(b.pad_cells, 1);
+  if (!fused) {
+    int rc = a.M > 0 ? pnh_k2y_launch(a, b, fp64, k, y, skip, s) : 0;
+    return rc ? rc : pnh_y2k_launch(a, b, fp64, y, grid, part, skip, s);
+  }
+  if (b.n_items > 0 && !part) {
+    pnh_set_error("workspace required for gridding");
+    return -1;
+  }
+  if (fp64) {
+    switch (a.D) {
+      case 1: return launch_k2y2k<double, 1>(a, b, k, y, grid, part, skip, s);
+      case 2: return launch_k2y2k<double, 2>(a, b, k, y, grid, part, skip, s);
+      case 3: return launch_k2y2k<double, 3>(a, b, k, y, grid, part, skip, s);
+    }
+  } else {
+    switch (a.D) {
+      case 1: return launch_k2y2k<float, 1>(a, b, k, y, grid, part, skip, s);
+      case 2: return launch_k2y2k<float, 2>(a, b, k, y, grid, part, skip, s);
+      case 3: return launch_k2y2k<float, 3>(a, b, k, y, grid, part, skip, s);
+    }
+  }
+  pnh_set_error("unsupported ndim %d", a.D);
+  return -2;
+}
+
 int pnh_k2y_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s) {
   const bool csrv = a.csr_val != nullptr;
@@ -1137,6 +1223,26 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
                                (C*)part_ + probs[p].part_off, 0, nullptr);
 }
 
+// P^H P of every problem's work items (k2y then y2k per item, as k2y2k_item).
+template <typename T, int D, int JN, int JE>
+__global__ void __launch_bounds__(1024)
+k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
+                        const void* __restrict__ k_, void* __restrict__ y_,
+                        void* __restrict__ part_, const int* __restrict__ done,
+                        const int* __restrict__ all_done) {
+  typedef typename cplx<T>::t C;
+  if (all_done && *all_done) return;
+  const int2 pi = items[blockIdx.x];
+  const int p = pi.x;
+  if (done && done[p]) return;            // uniform over the workgroup
+  extern __shared__ long long smem_kkb[];
+  const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
+  const BinArgs b = probs[p].bins;
+  const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
+  k2y2k_item<T, D, JN, JE>(a, b, it, smem_kkb, (const C*)k_ + probs[p].k_off,
+                           (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
+}
+
 // Gather of every problem's grid: pnh_gather_parts blocks per tile, blocks by table.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
@@ -1217,6 +1323,51 @@ int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n
   if (D == 2) KB(float, 2);
   KB(float, 3);
 #undef KB
+}
+
+namespace {
+template <typename T, int D>
+int batch_k2y2k_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
+                  size_t lds, int threads, const void* k, void* y, void* part, const int* done,
+                  const int* all_done, hipStream_t s) {
+  return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
+    auto kern = k2y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, k, y,
+                       part, done, all_done);
+    PNH_LAUNCH_CHECK();
+    return 0;
+  });
+}
+}  // namespace
+
+int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
+                           const BatchTable& gather, int D, const int* Jd, int fp64,
+                           int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
+                           void* part, const int* done, const int* all_done, hipStream_t s) {
+  const size_t lds = y2k_lds_bytes(pad_cells, 1);
+  if (lds > pnh_lds_limit()) {
+    pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
+                  pnh_lds_limit());
+    return -2;
+  }
+  int rc = 0;
+#define KK(T, DD) rc = batch_k2y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, part, done, all_done, s)
+  if (n_items > 0) {
+    if (fp64) { if (D == 1) KK(double, 1); else if (D == 2) KK(double, 2); else KK(double, 3); }
+    else { if (D == 1) KK(float, 1); else if (D == 2) KK(float, 2); else KK(float, 3); }
+    if (rc) return rc;
+  }
+#undef KK
+#define GB(T, DD) hipLaunchKernelGGL((gather_batch_kernel<T, DD>), dim3((unsigned)gather.n_blocks), \
+                                     dim3(256), 0, s, probs, gather, part, k_out, done, all_done)
+  if (gather.n_blocks > 0) {
+    if (fp64) { if (D == 1) GB(double, 1); else if (D == 2) GB(double, 2); else GB(double, 3); }
+    else { if (D == 1) GB(float, 1); else if (D == 2) GB(float, 2); else GB(float, 3); }
+    PNH_LAUNCH_CHECK();
+  }
+#undef GB
+  return 0;
 }
 
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,

@@ -242,13 +242,13 @@ def _valid_near(pref, K, J):
 
 
 def tile_shape(Kd, Jd=None):
-    """Gridding tile per axis: 1024 cells (1-D), 32 x 32 (2-D) or
+    """Gridding tile per axis: 1024 cells (1-D), 16 x 16 (2-D) or
     16 x 16 x 16 (3-D) when valid for (Kd, Jd); then, while the padded
     accumulator does not fit in LDS, the axis with the widest accumulator
     steps down to its next valid tile."""
     d = len(Kd)
     Jd = tuple(Jd) if Jd is not None else (6,) * d
-    pref = {1: (1024,), 2: (32, 32), 3: (16, 16, 16)}[d]
+    pref = {1: (1024,), 2: (16, 16), 3: (16, 16, 16)}[d]
     tile = [_valid_near(p, K, J) for p, K, J in zip(pref, Kd, Jd)]
 
     def fits(t):

@@ -250,7 +250,7 @@ def test_work_list_invariants():
 
 def test_tile_choice():
     assert helper.tile_shape((256, 256, 256), (6, 6, 6)) == (16, 16, 16)
-    assert helper.tile_shape((512, 512), (6, 6)) == (32, 32)
+    assert helper.tile_shape((512, 512), (6, 6)) == (16, 16)
     # shrinks to fit LDS with a wide interpolator
     t = helper.tile_shape((256, 256, 256), (8, 8, 8))
     assert np.prod(helper.tile_pad(t, (256,) * 3, (8,) * 3)) * 16 + 256 <= helper.GRID_LDS_MAX
