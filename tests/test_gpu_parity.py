@@ -248,6 +248,9 @@ def test_random_geometries(hip, case):
     assert rel(A.k2y(k), ref.k2y(k)) < TOL64
     assert rel(A.forward(x), ref.forward(x)) < TOL
     assert rel(A.adjoint(y.astype(np.complex64)), ref.adjoint(y.astype(np.complex64))) < TOL
+    # P^H P through the fused gridding pass (FP64 and complex64)
+    assert rel(A.k2y2k(k), ref.k2y2k(k)) < TOL64
+    assert rel(A.selfadjoint(x), ref.selfadjoint(x)) < TOL
 
 
 def test_autotuned_plan_matches_oracle(hip, tmp_path, monkeypatch):
