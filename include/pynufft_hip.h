@@ -234,6 +234,17 @@ int pnh_solve_bicg(const pnh_plan* plan, int kind, const void* y, void* x, int m
 int pnh_gmres_workspace_bytes(const pnh_plan* plan, int restart, size_t* bytes);
 int pnh_solve_gmres(const pnh_plan* plan, const void* y, void* x, int restart, int maxiter,
                     double rtol, double atol, void* ws, int32_t* info_out, void* stream);
+/* solve(y, 'lgmres', maxiter=, inner_m=, outer_k=, rtol=, atol=): scipy's
+ * LGMRES (GMRES cycles of inner_m Krylov columns augmented with the last
+ * outer_k corrections, stored with their products A z; no preconditioner)
+ * on P^H P in FP64 on the device (batch 1); inner_m + outer_k <= 64,
+ * outer_k <= 8, maxiter outer iterations.  info_out (device int32[2], may be
+ * NULL) receives scipy's info and the inner iterations.
+ * Reference linalg/solve_cpu.py:256-288 with scipy's lgmres. */
+int pnh_lgmres_workspace_bytes(const pnh_plan* plan, int inner_m, int outer_k, size_t* bytes);
+int pnh_solve_lgmres(const pnh_plan* plan, const void* y, void* x, int maxiter, int inner_m,
+                     int outer_k, double rtol, double atol, void* ws, int32_t* info_out,
+                     void* stream);
 /* Conjugate-gradient solve on the k-space normal equations P^H P k = P^H y in
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result

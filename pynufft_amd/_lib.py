@@ -94,6 +94,11 @@ EXPORTS = {
                                       ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]),
     'pnh_gmres_workspace_bytes': (ctypes.c_int, [_vp, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_lgmres_workspace_bytes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_solve_lgmres': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        _vp, _vp, _vp]),
     'pnh_solve_gmres': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]),
     'pnh_solve_lsmr': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,

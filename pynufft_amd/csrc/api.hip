@@ -919,7 +919,7 @@ int pnh_solve_gmres(const pnh_plan* p, const void* y, void* x, int restart, int 
   auto normal_op = [&](const void* in, void* out, const int* sk) -> int {
     return do_k2y2k(p, 1, in, ym, out, part, sk, s);
   };
-  if ((rc = pnh_gmres_init(st, restart, rtol, atol, maxiter, s))) return rc;
+  if ((rc = pnh_gmres_init(st, restart, rtol, atol, maxiter, 0, s))) return rc;
   if (M > 0 && (rc = pnh_c64_to_c128(y, ym, M, s))) return rc;
   if ((rc = do_y2k(p, 1, ym, bv, part, nullptr, s))) return rc;
   PNH_CHECK_HIP(hipMemcpyAsync(r, bv, (size_t)K * 16, hipMemcpyDeviceToDevice, s));
@@ -929,7 +929,7 @@ int pnh_solve_gmres(const pnh_plan* p, const void* y, void* x, int restart, int 
   for (int outer = 0; outer < maxiter; ++outer) {
     if ((rc = pnh_krylov_vdot2(r, r, nullptr, nullptr, K, red, done, s))) return rc;
     if ((rc = pnh_gmres_scalar(st, red, 1, 0, 0, s))) return rc;
-    if ((rc = pnh_gmres_normalize(vec(0), r, st, K, s))) return rc;
+    if ((rc = pnh_gmres_normalize(vec(0), r, st, K, -1, s))) return rc;
     for (int col = 0; col < restart; ++col) {
       if ((rc = normal_op(vec(col), wv, skip))) return rc;
       if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
@@ -941,13 +941,134 @@ int pnh_solve_gmres(const pnh_plan* p, const void* y, void* x, int restart, int 
       }
       if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
       if ((rc = pnh_gmres_scalar(st, red, 4, col, 0, s))) return rc;
-      if ((rc = pnh_gmres_normalize(vec(col + 1), wv, st, K, s))) return rc;
+      if ((rc = pnh_gmres_normalize(vec(col + 1), wv, st, K, col, s))) return rc;
     }
     if ((rc = pnh_gmres_solve(st, s))) return rc;
     if ((rc = pnh_gmres_x(xk, V, ldv, st, K, s))) return rc;
     if ((rc = normal_op(xk, wv, done))) return rc;
     if ((rc = pnh_gmres_residual(r, bv, wv, st, K, red, s))) return rc;
     if ((rc = pnh_gmres_scalar(st, red, 6, 0, 0, s))) return rc;
+    if (outer + 1 < maxiter) {
+      int flag = 0;
+      PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));
+      PNH_CHECK_HIP(hipStreamSynchronize(s));
+      if (flag) break;
+    }
+  }
+  if ((rc = run_passes_inverse(p, xk, img64, false, 1, nullptr, s))) return rc;
+  if ((rc = pnh_cast_div_sn(img64, x, p->d.sn_f32, p->Ntot, 1, s))) return rc;
+  if (info_out) {
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out, &st->info, sizeof(int), hipMemcpyDeviceToDevice, s));
+    PNH_CHECK_HIP(hipMemcpyAsync(info_out + 1, &st->inner_iter, sizeof(int),
+                                 hipMemcpyDeviceToDevice, s));
+  }
+  return 0;
+}
+
+int pnh_lgmres_workspace_bytes(const pnh_plan* p, int inner_m, int outer_k, size_t* bytes) {
+  REQUIRE(p && bytes, "null argument");
+  REQUIRE(inner_m >= 1 && outer_k >= 0 && outer_k <= 8 && inner_m + outer_k <= PNH_GMRES_MAXR,
+          "lgmres needs inner_m >= 1, 0 <= outer_k <= 8, inner_m + outer_k <= %d",
+          PNH_GMRES_MAXR);
+  const size_t kv = align_up((size_t)p->Ktot * 16);
+  *bytes = align_up((size_t)p->M * 16) + (size_t)(inner_m + outer_k + 1) * kv +
+           2 * (size_t)outer_k * kv + 6 * kv + align_up(partial_bytes(p, 1)) +
+           align_up((size_t)p->Ntot * 16) + align_up((size_t)2 * PNH_LSQR_BLOCKS * 16) +
+           align_up(sizeof(GmresState));
+  return 0;
+}
+
+int pnh_solve_lgmres(const pnh_plan* p, const void* y, void* x, int maxiter, int inner_m,
+                     int outer_k, double rtol, double atol, void* ws, int32_t* info_out,
+                     void* stream) {
+  REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
+  REQUIRE(p->nb == 1, "lgmres solves one coil (plan without batch)");
+  REQUIRE(atol >= 0.0, "atol must be >= 0");
+  REQUIRE(maxiter >= 1, "maxiter must be >= 1");
+  REQUIRE(inner_m >= 1 && outer_k >= 0 && outer_k <= 8 && inner_m + outer_k <= PNH_GMRES_MAXR,
+          "lgmres needs inner_m >= 1, 0 <= outer_k <= 8, inner_m + outer_k <= %d",
+          PNH_GMRES_MAXR);
+  int rc = check_dtype_ready(p, PNH_C128);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t M = p->M, K = p->Ktot;
+  const int mmax = inner_m + outer_k;
+  const size_t kv = align_up((size_t)K * 16);
+  char* w = (char*)ws;
+  void* ym = w; w += align_up((size_t)M * 16);
+  char* V = w; w += (size_t)(mmax + 1) * kv;
+  char* OZ = w; w += (size_t)outer_k * kv;
+  char* OAZ = w; w += (size_t)outer_k * kv;
+  void* wv = w; w += kv;
+  void* xk = w; w += kv;
+  void* r = w; w += kv;
+  void* bv = w; w += kv;
+  void* dx = w; w += kv;
+  void* tK = w; w += kv;
+  void* part = w; w += align_up(partial_bytes(p, 1));
+  void* img64 = w; w += align_up((size_t)p->Ntot * 16);
+  void* red = w; w += align_up((size_t)2 * PNH_LSQR_BLOCKS * 16);
+  GmresState* st = (GmresState*)w;
+  const int* done = &st->done;
+  const int* skip = &st->skip;
+  const int64_t ldv = (int64_t)(kv / 16);
+  auto vec = [&](int k) -> void* { return V + (size_t)k * kv; };
+  auto normal_op = [&](const void* in, void* out, const int* sk) -> int {
+    return do_k2y2k(p, 1, in, ym, out, part, sk, s);
+  };
+  // b = P^H y; x = 0, so the first residual b - A x is b itself
+  if ((rc = pnh_gmres_init(st, mmax, rtol, atol, maxiter, 1, s))) return rc;
+  if (M > 0 && (rc = pnh_c64_to_c128(y, ym, M, s))) return rc;
+  if ((rc = do_y2k(p, 1, ym, bv, part, nullptr, s))) return rc;
+  if ((rc = pnh_fill_c128(xk, K, 0.0, 0.0, s))) return rc;
+  PNH_CHECK_HIP(hipMemcpyAsync(r, bv, (size_t)K * 16, hipMemcpyDeviceToDevice, s));
+  if ((rc = pnh_krylov_vdot2(bv, bv, nullptr, nullptr, K, red, done, s))) return rc;
+  if ((rc = pnh_gmres_scalar(st, red, 0, 0, 0, s))) return rc;
+  for (int outer = 0; outer < maxiter; ++outer) {
+    // augmentation vectors, oldest first (one stored per outer iteration)
+    const int n_aug = outer < outer_k ? outer : outer_k;
+    const int first_it = outer - n_aug;
+    LgmresZ zo;
+    memset(&zo, 0, sizeof(zo));
+    for (int t = 0; t < n_aug; ++t) zo.z[t] = OZ + (size_t)((first_it + t) % outer_k) * kv;
+    const int m = inner_m + n_aug;
+    if (outer > 0) {
+      if ((rc = normal_op(xk, tK, done))) return rc;
+      if ((rc = pnh_gmres_residual(r, bv, tK, st, K, red, s))) return rc;
+    } else {
+      if ((rc = pnh_krylov_vdot2(r, r, nullptr, nullptr, K, red, done, s))) return rc;
+    }
+    if ((rc = pnh_gmres_scalar(st, red, 7, 0, 0, s))) return rc;
+    if ((rc = pnh_gmres_normalize(vec(0), r, st, K, -1, s))) return rc;
+    for (int col = 0; col < m; ++col) {
+      if (col < inner_m) {
+        if ((rc = normal_op(vec(col), wv, skip))) return rc;
+      } else {
+        const int t = col - inner_m;
+        const void* az = OAZ + (size_t)((first_it + t) % outer_k) * kv;
+        PNH_CHECK_HIP(hipMemcpyAsync(wv, az, (size_t)K * 16, hipMemcpyDeviceToDevice, s));
+      }
+      if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
+      if ((rc = pnh_gmres_scalar(st, red, 2, col, 0, s))) return rc;
+      for (int k = 0; k <= col; ++k) {
+        if ((rc = pnh_krylov_vdot2(vec(k), wv, nullptr, nullptr, K, red, skip, s))) return rc;
+        if ((rc = pnh_gmres_scalar(st, red, 3, col, k, s))) return rc;
+        if ((rc = pnh_gmres_mgs(wv, vec(k), st, col, k, K, s))) return rc;
+      }
+      if ((rc = pnh_krylov_vdot2(wv, wv, nullptr, nullptr, K, red, skip, s))) return rc;
+      if ((rc = pnh_gmres_scalar(st, red, 4, col, 0, s))) return rc;
+      if ((rc = pnh_gmres_normalize(vec(col + 1), wv, st, K, col, s))) return rc;
+    }
+    if ((rc = pnh_gmres_solve(st, s))) return rc;
+    if ((rc = pnh_lgmres_dx(dx, V, ldv, inner_m, zo, st, K, red, s))) return rc;
+    if ((rc = pnh_gmres_scalar(st, red, 8, 0, 0, s))) return rc;
+    if (outer_k > 0) {
+      const size_t slot = (size_t)(outer % outer_k) * kv;
+      if ((rc = pnh_lgmres_store(xk, dx, V, ldv, OZ + slot, OAZ + slot, st, K, s))) return rc;
+    } else if ((rc = pnh_lgmres_store(xk, dx, V, ldv, nullptr, nullptr, st, K, s))) {
+      return rc;
+    }
+    if ((rc = pnh_gmres_scalar(st, red, 9, 0, 0, s))) return rc;
     if (outer + 1 < maxiter) {
       int flag = 0;
       PNH_CHECK_HIP(hipMemcpyAsync(&flag, done, sizeof(int), hipMemcpyDeviceToHost, s));

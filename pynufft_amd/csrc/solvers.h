@@ -108,16 +108,20 @@ int pnh_krylov_x(void* x, const void* p, void* r, const void* t, void* rt, const
 
 struct GmresState {
   double2 S[PNH_GMRES_MAXR + 1];
-  double2 H[PNH_GMRES_MAXR][PNH_GMRES_MAXR + 1];   // H[col][row]
+  double2 H[PNH_GMRES_MAXR][PNH_GMRES_MAXR + 1];   // H[col][row], rotated
+  double2 H0[PNH_GMRES_MAXR][PNH_GMRES_MAXR + 1];  // as built (LGMRES: A dx)
   double gc[PNH_GMRES_MAXR];
   double2 gs[PNH_GMRES_MAXR];
   double2 y[PNH_GMRES_MAXR];
-  double rtol, atol, tol, bnrm2, h0, vnorm, presid, ptol, ptol_max_factor, rnorm;
+  double2 q[PNH_GMRES_MAXR + 1];                   // LGMRES: H0 y
+  double rtol, atol, tol, bnrm2, h0, vnorm, presid, ptol, ptol_max_factor, rnorm, nx;
   int restart, maxiter, outer, inner_iter, col_end, breakdown, skip, done, info;
+  int flex;        // LGMRES inner cycle (scipy _fgmres): no H zeroing, strict res < ptol
 };
 
+// flex = 1: LGMRES inner cycles (scipy's _fgmres rules)
 int pnh_gmres_init(GmresState* st, int restart, double rtol, double atol, int maxiter,
-                   hipStream_t s);
+                   int flex, hipStream_t s);
 // scalar step `phase` for column col / basis vector k (see solvers.hip)
 int pnh_gmres_scalar(GmresState* st, const void* part, int phase, int col, int k,
                      hipStream_t s);
@@ -126,8 +130,9 @@ int pnh_gmres_solve(GmresState* st, hipStream_t s);
 // w -= H[col][k] v_k
 int pnh_gmres_mgs(void* w, const void* vk, const GmresState* st, int col, int k, int64_t n,
                   hipStream_t s);
-// out = in / vnorm (skipped once the cycle has ended)
-int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n,
+// out = in / vnorm (skipped once the cycle has ended, except, for LGMRES,
+// the basis vector of the column col that ended it; col < 0: a cycle start)
+int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n, int col,
                         hipStream_t s);
 // x += sum_{k <= col_end} y_k V[k]
 int pnh_gmres_x(void* x, const void* V, int64_t ldv, const GmresState* st, int64_t n,
@@ -135,3 +140,15 @@ int pnh_gmres_x(void* x, const void* V, int64_t ldv, const GmresState* st, int64
 // r = b - t with partials of |r|^2
 int pnh_gmres_residual(void* r, const void* b, const void* t, const GmresState* st, int64_t n,
                        void* part, hipStream_t s);
+
+// ---- LGMRES (scipy semantics, augmentation vectors stored with A z)
+struct LgmresZ {
+  const void* z[8];   // augmentation vectors of the cycle's last columns, oldest first
+};
+// dx = sum_{c <= col_end} y_c z_c (z_c = V[c] for c < inner_m, else zo.z[c - inner_m]);
+// partials of |dx|^2
+int pnh_lgmres_dx(void* dx, const void* V, int64_t ldv, int inner_m, const LgmresZ& zo,
+                  const GmresState* st, int64_t n, void* part, hipStream_t s);
+// ax = sum_{c <= col_end + 1} q_c V[c]; z_new = dx / nx; az_new = ax / nx; x += dx
+int pnh_lgmres_store(void* x, const void* dx, const void* V, int64_t ldv, void* z_new,
+                     void* az_new, const GmresState* st, int64_t n, hipStream_t s);

@@ -972,8 +972,9 @@ __device__ void zlartg(double2 f, double2 g, double& c, double2& s, double2& r) 
 }
 
 __global__ void gmres_init_kernel(GmresState* __restrict__ st, int restart, double rtol,
-                                  double atol, int maxiter) {
+                                  double atol, int maxiter, int flex) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->flex = flex;
   st->restart = restart;
   st->rtol = rtol;
   st->atol = atol;
@@ -1015,14 +1016,17 @@ __global__ void gmres_scalar_kernel(GmresState* __restrict__ st,
     st->skip = 0;
     return;
   }
-  if (st->skip) return;
+  if (st->skip && phase >= 2 && phase <= 4) return;
   if (phase == 2) { st->h0 = sqrt(S0.x); return; }
-  if (phase == 3) { st->H[col][k] = S0; return; }
+  if (phase == 3) { st->H[col][k] = S0; st->H0[col][k] = S0; return; }
   if (phase == 4) {
     const double h1 = sqrt(S0.x);
     st->H[col][col + 1] = make_double2(h1, 0.0);
+    st->H0[col][col + 1] = make_double2(h1, 0.0);
     st->vnorm = h1;
-    if (h1 <= eps * st->h0) {
+    if (st->flex) {
+      if (!(h1 > eps * st->h0)) st->breakdown = 1;
+    } else if (h1 <= eps * st->h0) {
       st->H[col][col + 1] = make_double2(0.0, 0.0);
       st->breakdown = 1;
     }
@@ -1048,7 +1052,50 @@ __global__ void gmres_scalar_kernel(GmresState* __restrict__ st,
     st->presid = hypot(tmp.x, tmp.y);
     st->inner_iter += 1;
     st->col_end = col;
-    if (st->presid <= st->ptol || st->breakdown) st->skip = 1;
+    if (st->flex ? (st->presid < st->ptol || st->breakdown)
+                 : (st->presid <= st->ptol || st->breakdown))
+      st->skip = 1;
+    return;
+  }
+  if (phase == 7) {
+    // LGMRES outer start: S0 = |b - A x|^2 (the cycle's first vector)
+    const double rn = sqrt(S0.x);
+    st->rnorm = rn;
+    if (rn <= st->tol) { st->done = 1; st->skip = 1; st->info = 0; return; }
+    st->vnorm = rn;
+    st->ptol = fmin(st->ptol_max_factor, st->tol / rn);
+    // the inner least-squares problem starts from the unit vector (its
+    // residual estimate is relative, as ptol); y is scaled by |r| after
+    for (int i = 0; i <= st->restart; ++i) st->S[i] = make_double2(0.0, 0.0);
+    st->S[0] = make_double2(1.0, 0.0);
+    st->breakdown = 0;
+    st->col_end = 0;
+    st->skip = 0;
+    return;
+  }
+  if (phase == 8) {
+    // LGMRES: |dx| and q = H0 y (H0 as built, before the rotations)
+    st->nx = sqrt(S0.x);
+    const int c = st->col_end;
+    for (int i = 0; i <= c + 1; ++i) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int j = 0; j <= c; ++j) {
+        if (i > j + 1) continue;
+        const double2 t = zmul(st->H0[j][i], st->y[j]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      st->q[i] = acc;
+    }
+    return;
+  }
+  if (phase == 9) {
+    // LGMRES end of an outer iteration: inner tolerance schedule, limit
+    if (st->presid > st->ptol) st->ptol_max_factor = fmin(1.0, 1.5 * st->ptol_max_factor);
+    else st->ptol_max_factor = fmax(1e-16, 0.25 * st->ptol_max_factor);
+    st->outer += 1;
+    if (st->outer >= st->maxiter) { st->done = 1; st->info = st->maxiter; }
+    st->skip = st->done;
     return;
   }
   // phase 6: rnorm after the cycle
@@ -1087,6 +1134,8 @@ __global__ void gmres_solve_kernel(GmresState* __restrict__ st) {
     }
   }
   if (st->y[0].x != 0.0 || st->y[0].y != 0.0) st->y[0] = cdivd(st->y[0], st->H[0][0]);
+  if (st->flex)
+    for (int i = 0; i <= col; ++i) st->y[i] = make_double2(st->y[i].x * st->rnorm, st->y[i].y * st->rnorm);
 }
 
 // w -= H[col][k] v_k
@@ -1104,8 +1153,10 @@ __global__ void gmres_mgs_kernel(double2* __restrict__ w, const double2* __restr
 // out = in / vnorm: the next basis vector (not needed, so skipped, once the
 // cycle or the solve has ended)
 __global__ void gmres_normalize_kernel(double2* __restrict__ out, const double2* __restrict__ in,
-                                       const GmresState* __restrict__ st, int64_t n) {
-  if (st->done || st->skip) return;
+                                       const GmresState* __restrict__ st, int64_t n, int col) {
+  // LGMRES keeps the basis vector of the column that ended the cycle (its
+  // A dx uses it); GMRES does not need it
+  if (st->done || (st->skip && !(st->flex && col >= 0 && col == st->col_end))) return;
   const double nv = st->vnorm;
   const double inv = nv > 0.0 ? 1.0 / nv : 1.0;
   GRID_STRIDE(i, n) {
@@ -1148,10 +1199,73 @@ gmres_residual_kernel(double2* __restrict__ r, const double2* __restrict__ b,
 
 }  // namespace
 
+namespace {
+
+__global__ void __launch_bounds__(256)
+lgmres_dx_kernel(double2* __restrict__ dx, const double2* __restrict__ V, int64_t ldv,
+                 int inner_m, LgmresZ zo, const GmresState* __restrict__ st, int64_t n,
+                 double2* __restrict__ part) {
+  if (st->done) return;
+  const int col = st->col_end;
+  double acc2 = 0.0;
+  GRID_STRIDE(i, n) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int c = 0; c <= col; ++c) {
+      const double2* z = c < inner_m ? V + c * ldv : (const double2*)zo.z[c - inner_m];
+      const double2 t = zmul(st->y[c], z[i]);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    dx[i] = acc;
+    acc2 += acc.x * acc.x + acc.y * acc.y;
+  }
+  block_sum2_store(make_double2(acc2, 0.0), part);
+}
+
+__global__ void lgmres_store_kernel(double2* __restrict__ x, const double2* __restrict__ dx,
+                                    const double2* __restrict__ V, int64_t ldv,
+                                    double2* __restrict__ z_new, double2* __restrict__ az_new,
+                                    const GmresState* __restrict__ st, int64_t n) {
+  if (st->done) return;
+  const int col = st->col_end;
+  const double nx = st->nx;
+  GRID_STRIDE(i, n) {
+    const double2 d = dx[i];
+    if (nx > 0.0 && z_new) {      // no augmentation vectors kept when outer_k == 0
+      double2 ax = make_double2(0.0, 0.0);
+      for (int c = 0; c <= col + 1; ++c) {
+        const double2 t = zmul(st->q[c], V[c * ldv + i]);
+        ax.x += t.x;
+        ax.y += t.y;
+      }
+      z_new[i] = make_double2(d.x / nx, d.y / nx);
+      az_new[i] = make_double2(ax.x / nx, ax.y / nx);
+    }
+    x[i] = make_double2(x[i].x + d.x, x[i].y + d.y);
+  }
+}
+
+}  // namespace
+
+int pnh_lgmres_dx(void* dx, const void* V, int64_t ldv, int inner_m, const LgmresZ& zo,
+                  const GmresState* st, int64_t n, void* part, hipStream_t s) {
+  hipLaunchKernelGGL(lgmres_dx_kernel, dim3(PNH_LSQR_BLOCKS), dim3(256), 0, s, (double2*)dx,
+                     (const double2*)V, ldv, inner_m, zo, st, n, (double2*)part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_lgmres_store(void* x, const void* dx, const void* V, int64_t ldv, void* z_new,
+                     void* az_new, const GmresState* st, int64_t n, hipStream_t s) {
+  LAUNCH(lgmres_store_kernel, n, (double2*)x, (const double2*)dx, (const double2*)V, ldv,
+         (double2*)z_new, (double2*)az_new, st, n);
+  return 0;
+}
+
 int pnh_gmres_init(GmresState* st, int restart, double rtol, double atol, int maxiter,
-                   hipStream_t s) {
+                   int flex, hipStream_t s) {
   hipLaunchKernelGGL(gmres_init_kernel, dim3(1), dim3(64), 0, s, st, restart, rtol, atol,
-                     maxiter);
+                     maxiter, flex);
   PNH_LAUNCH_CHECK();
   return 0;
 }
@@ -1176,9 +1290,9 @@ int pnh_gmres_mgs(void* w, const void* vk, const GmresState* st, int col, int k,
   return 0;
 }
 
-int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n,
+int pnh_gmres_normalize(void* out, const void* in, const GmresState* st, int64_t n, int col,
                         hipStream_t s) {
-  LAUNCH(gmres_normalize_kernel, n, (double2*)out, (const double2*)in, st, n);
+  LAUNCH(gmres_normalize_kernel, n, (double2*)out, (const double2*)in, st, n, col);
   return 0;
 }
 

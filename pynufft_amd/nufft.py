@@ -498,6 +498,10 @@ class NUFFT:
           in FP64 (``restart`` <= 64, ``maxiter`` cycles, ``rtol``,
           ``atol``; plans without ``batch``); ``self.bicg_info`` then holds
           (scipy's info, inner iterations).
+        * ``'lgmres'``: scipy's LGMRES on the same normal equations in FP64
+          (``maxiter`` outer iterations, ``inner_m``, ``outer_k``, ``rtol``,
+          ``atol``; plans without ``batch``); ``self.bicg_info`` then holds
+          (scipy's info, inner iterations).
         * ``'dc'``: density-compensated adjoint, ``maxiter`` pipe-density
           iterations (``solve_cpu.py:165-183``).
         * ``'L1TVOLS'``: L1 total-variation least squares by split Bregman,
@@ -509,8 +513,8 @@ class NUFFT:
           ``batch``).  ``self.lsqr_info`` then holds (istop, iterations).
         """
         if solver is None:
-            raise KeyError('solver must be one of: cg, bicgstab, bicg, gmres, dc, L1TVOLS, lsqr, '
-                           'lsmr')
+            raise KeyError('solver must be one of: cg, bicgstab, bicg, gmres, lgmres, dc, '
+                           'L1TVOLS, lsqr, lsmr')
         if solver in ('cgs', 'qmr', 'minres'):
             raise TypeError(solver + ' requires real symmetric matrix')
         if solver == 'dc':
@@ -525,10 +529,12 @@ class NUFFT:
             return self._solve_bicg(y, solver, *args, **kwargs)
         if solver == 'gmres':
             return self._solve_gmres(y, *args, **kwargs)
+        if solver == 'lgmres':
+            return self._solve_lgmres(y, *args, **kwargs)
         if solver != 'cg':
-            raise NotImplementedError("solver %r is not provided; use 'cg', 'bicgstab', "
-                                      "'bicg', 'gmres', 'dc', 'L1TVOLS', 'lsqr' or 'lsmr'"
-                                      % (solver,))
+            # the reference looks the name up in its table of scipy solvers
+            raise KeyError("unknown solver %r; use 'cg', 'bicgstab', 'bicg', 'gmres', "
+                           "'lgmres', 'dc', 'L1TVOLS', 'lsqr' or 'lsmr'" % (solver,))
         if args:
             raise TypeError('pass solver options as keywords (maxiter=..., rtol=...)')
         maxiter = kwargs.pop('maxiter', None)
@@ -672,6 +678,33 @@ class NUFFT:
         self._call(self._libh.pnh_solve_gmres, _ptr(yt), _ptr(out), restart,
                    int(maxiter) if maxiter is not None else 0, float(rtol), float(atol),
                    _ptr(ws), _ptr(info), self._stream())
+        self._bicg_info = info
+        return self._out(out, is_np)
+
+    def _solve_lgmres(self, y, maxiter=1000, inner_m=30, outer_k=3, rtol=None, atol=0.0,
+                      tol=None, x0=None, M=None, callback=None, outer_v=None,
+                      store_outer_Av=True, prepend_outer_v=False):
+        """scipy.sparse.linalg.lgmres semantics on P^H P, x0 = 0."""
+        self._need_plan()
+        if self.batch is not None:
+            raise NotImplementedError('lgmres solves plans without batch')
+        if (x0 is not None or M is not None or callback is not None or outer_v is not None
+                or not store_outer_Av or prepend_outer_v):
+            raise NotImplementedError('lgmres: x0, M, callback, outer_v, store_outer_Av=False '
+                                      'and prepend_outer_v are not supported')
+        rtol = rtol if rtol is not None else (tol if tol is not None else 1e-5)
+        self._ensure_fp64()
+        yt, is_np = self._in(y, 'y')
+        out = self._empty('x')
+        n = ctypes.c_size_t()
+        with torch.cuda.device(self.device):
+            _lib.check(self._libh.pnh_lgmres_workspace_bytes(self._plan, int(inner_m),
+                                                             int(outer_k), ctypes.byref(n)))
+        ws = torch.empty(max(int(n.value), 256), dtype=torch.uint8, device=self.device)
+        info = torch.zeros(2, dtype=torch.int32, device=self.device)
+        self._call(self._libh.pnh_solve_lgmres, _ptr(yt), _ptr(out), int(maxiter),
+                   int(inner_m), int(outer_k), float(rtol), float(atol), _ptr(ws), _ptr(info),
+                   self._stream())
         self._bicg_info = info
         return self._out(out, is_np)
 

@@ -198,8 +198,8 @@ def test_solver_argument_errors(hip):
         A.solve(y)
     with pytest.raises(TypeError):
         A.solve(y, 'cgs')
-    with pytest.raises(NotImplementedError):
-        A.solve(y, 'lgmres')
+    with pytest.raises(KeyError):
+        A.solve(y, 'no-such-solver')
     with pytest.raises(ValueError):
         A.forward(np.zeros((3, 3), np.complex64))
 

@@ -248,3 +248,34 @@ def test_cg_graph_replays_match_eager(hip, name, maxiter, rtol):
         assert np.array_equal(x, eager), rep
         assert A.cg_iters == it_eager
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('name', SOLVER_CASES)
+def test_solve_lgmres(hip, name):
+    """scipy's lgmres (2 outer iterations, default inner_m 30 and outer_k 3)
+    on P^H P in FP64 against the reference."""
+    A, g = make(name)
+    s = golden('solvers')
+    x = A.solve(g['adjoint_in'], 'lgmres', maxiter=2)
+    assert rel(x, s[name + '__lgmres']) < TOL
+
+
+@pytest.mark.parametrize('inner_m,outer_k', [(30, 3), (4, 2), (3, 0)])
+def test_lgmres_stops_like_scipy(hip, inner_m, outer_k):
+    """Augmented restarts to scipy's stopping test: same info and image as
+    scipy's lgmres on the oracle's operator."""
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make('tiny1d')
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y = g['adjoint_in']
+    K = B.Kdprod
+
+    def mv(v):
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    k, info = sla.lgmres(op, B.y2k(y).ravel(), maxiter=25, inner_m=inner_m, outer_k=outer_k)
+    x = A.solve(y, 'lgmres', maxiter=25, inner_m=inner_m, outer_k=outer_k)
+    assert A.bicg_info[0] == info
+    assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
