@@ -279,3 +279,25 @@ def test_lgmres_stops_like_scipy(hip, inner_m, outer_k):
     x = A.solve(y, 'lgmres', maxiter=25, inner_m=inner_m, outer_k=outer_k)
     assert A.bicg_info[0] == info
     assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
+
+
+@pytest.mark.parametrize('name,restart,rtol', [('tiny2d', 8, 1e-9), ('tiny3d', 6, 1e-10),
+                                               ('plain2d', 10, 1e-7)])
+def test_gmres_restarts_like_scipy(hip, name, restart, rtol):
+    """Several restart cycles, some ending early on scipy's inner tolerance:
+    the device follows scipy's restart schedule (same info, same image)."""
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make(name)
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y = g['adjoint_in']
+    K = B.Kdprod
+
+    def mv(v):
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    k, info = sla.gmres(op, B.y2k(y).ravel(), restart=restart, maxiter=30, rtol=rtol)
+    x = A.solve(y, 'gmres', restart=restart, maxiter=30, rtol=rtol)
+    assert A.bicg_info[0] == info
+    assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-5
