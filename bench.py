@@ -250,6 +250,21 @@ def bench_other_workloads(torch, W, steps=20):
         out[name] = {'pairs_per_s': 1.0 / dt, 'ms_per_pair': 1000.0 * dt,
                      'Nd': list(w['Nd']), 'M': int(w['om'].shape[0]),
                      'coils': int(w['S'].shape[-1]) if name == 'multicoil2d' else 1}
+        if name != 'multicoil2d':
+            # the same pair replayed from a torch.cuda graph (no per-kernel
+            # launch cost; the library's calls only launch, so they capture)
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, capture_error_mode='thread_local'):
+                fn()
+            graph.replay()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(steps):
+                graph.replay()
+            torch.cuda.synchronize()
+            dg = (time.perf_counter() - t) / steps
+            out[name]['graph_pairs_per_s'] = 1.0 / dg
+            del graph
         A.release()
     return out
 

@@ -166,6 +166,53 @@ def test_torch_in_torch_out(hip):
     assert rel(y.cpu().numpy(), A.forward(g['x'])) == 0.0
 
 
+@pytest.mark.parametrize('name', ['plain2d', 'small3d'])
+def test_forward_adjoint_replayed_from_torch_graph(hip, name):
+    """forward/adjoint only launch kernels on the caller's stream (no
+    allocation, copy, memset or synchronization inside the library), so
+    torch.cuda.graph can capture them.  Every replay, with eager calls and
+    synchronizations between replays, equals the eager result bit for bit
+    (the gridding is deterministic), and the first equals the reference."""
+    import torch
+    g = golden(name)
+    if name == 'small3d':
+        Nd, Kd, Jd = geometry(g)
+        A = make(dict(om=W.uniform_om(20000, 3, 303), Nd=Nd, Kd=Kd, Jd=Jd))
+        x0 = W.gaussian_image(Nd, 304)
+    else:
+        A = make(g)
+        x0 = g['x']
+    dev = A.device
+    rng = np.random.default_rng(11)
+    shape = A.Nd
+    inputs = [x0.astype(np.complex64)] + [
+        (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+        for _ in range(2)]
+    inputs = [torch.from_numpy(v).to(dev) for v in inputs]
+    xs = torch.zeros(shape, dtype=torch.complex64, device=dev)
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):       # workspaces are allocated before the capture
+        A.adjoint(A.forward(xs))
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize(dev)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, capture_error_mode='thread_local'):
+        ys = A.forward(xs)
+        xo = A.adjoint(ys)
+    for i, xin in enumerate(inputs):
+        xs.copy_(xin)
+        graph.replay()
+        y_e = A.forward(xin)
+        x_e = A.adjoint(y_e)
+        torch.cuda.synchronize(dev)
+        assert torch.equal(ys, y_e), (name, i)
+        assert torch.equal(xo, x_e), (name, i)
+        if i == 0:
+            assert rel(ys.cpu().numpy(), g['forward']) < TOL
+    del graph
+
+
 def test_edge_samples_and_single_sample(hip):
     Nd, Kd, Jd = (8, 8), (16, 16), (6, 6)
     om = np.array([[-np.pi, -np.pi], [np.pi - 1e-12, 0.0], [0.0, 0.0],
