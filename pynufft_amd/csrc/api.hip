@@ -434,7 +434,7 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
       const size_t y2 = align_up((size_t)p->M * p->nb * 16);
       const size_t part2 = align_up(partial_bytes(p, 1));
       const size_t red = align_up((size_t)PNH_RED_BLOCKS * p->nb * 16);
-      *bytes = 4 * g2 + y2 + part2 + red + align_up(sizeof(CgScalars));
+      *bytes = 4 * g2 + y2 + part2 + 2 * red + align_up(sizeof(CgScalars));
       break;
     }
     default: pnh_set_error("unknown op %d", op); return -1;
@@ -1120,6 +1120,7 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   void* yt = w; w += y2;
   void* part = w; w += part2;
   void* redp = w; w += red;
+  void* redpq = w; w += red;
   CgScalars* sc = (CgScalars*)w;
   const int* all_done = &sc->all_done;
   int rc;
@@ -1133,9 +1134,8 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   for (int it = 0; it < maxiter; ++it) {
     if ((rc = pnh_cg_p_update(vp, vr, K, nb, sc, it == 0, s))) return rc;
     if ((rc = do_k2y2k(p, 1, vp, yt, vq, part, all_done, s))) return rc;
-    if ((rc = pnh_cg_dot(0, vp, vq, K, nb, redp, all_done, s))) return rc;
-    if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_PQ, it, maxiter, rtol, atol, s))) return rc;
-    if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redp, s))) return rc;
+    if ((rc = pnh_cg_dot(0, vp, vq, K, nb, redpq, all_done, s))) return rc;
+    if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redpq, redp, s))) return rc;
     if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_RR, it, maxiter, rtol, atol, s))) return rc;
     if (host_checks && (it & 63) == 63 && it + 1 < maxiter) {
       // long runs: stop launching once every coil has converged

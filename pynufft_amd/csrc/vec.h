@@ -37,5 +37,8 @@ int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter
                     int maxiter, double rtol, double atol, hipStream_t s);
 int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
                     const CgScalars* sc, int first, hipStream_t s);
+// alpha = rr / (p.q) with p.q summed from the block partials pq_part (every
+// workgroup sums them itself); x += alpha p; r -= alpha q; |r|^2 partials
 int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
-                     int nb, const CgScalars* sc, void* part, hipStream_t s);
+                     int nb, const CgScalars* sc, const void* pq_part, void* part,
+                     hipStream_t s);

@@ -119,15 +119,11 @@ dot_partials_kernel(const double2* __restrict__ a, const double2* __restrict__ b
   }
 }
 
-// Sum the block partials of coil `c` (one block per coil) in block order
-// using a fixed tree, then apply the CG bookkeeping of `phase`.
-__global__ void __launch_bounds__(256)
-cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
-                   CgScalars* __restrict__ s, int phase, int iter, int maxiter,
-                   double rtol, double atol) {
-  __shared__ double2 red[256];
-  const int c = blockIdx.x;
-  if (phase != CG_PHASE_INIT && s->all_done) return;
+// Sum of the block partials of coil `c` (nblk blocks) in block order with a
+// fixed tree over 256 threads; every thread of the block gets the total.
+// Every caller uses this one routine, so each gets the same bits.
+__device__ double2 sum_partials(const double2* __restrict__ part, int nblk, int nb, int c,
+                                double2* red) {
   double sx = 0.0, sy = 0.0;
   for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
     const double2 v = part[(int64_t)i * nb + c];
@@ -142,8 +138,22 @@ cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
     }
     __syncthreads();
   }
-  if (threadIdx.x != 0) return;
   const double2 tot = red[0];
+  __syncthreads();   // red may be reused right away
+  return tot;
+}
+
+// Sum the block partials of coil `c` (one block per coil), then apply the
+// CG bookkeeping of `phase`.
+__global__ void __launch_bounds__(256)
+cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
+                   CgScalars* __restrict__ s, int phase, int iter, int maxiter,
+                   double rtol, double atol) {
+  __shared__ double2 red[256];
+  const int c = blockIdx.x;
+  if (phase != CG_PHASE_INIT && s->all_done) return;
+  const double2 tot = sum_partials(part, nblk, nb, c, red);
+  if (threadIdx.x != 0) return;
   if (phase == CG_PHASE_INIT) {
     const double bn = sqrt(tot.x);
     const double tol = fmax(atol, rtol * bn);
@@ -197,18 +207,21 @@ cg_p_update_kernel(double2* __restrict__ p, const double2* __restrict__ r,
   }
 }
 
-// alpha = rr / pq; x += alpha p; r -= alpha q; block partial of |r|^2
+// alpha = rr / pq; x += alpha p; r -= alpha q; block partial of |r|^2.
+// Every block sums the p.q block partials itself (same order and tree as the
+// finalize kernel), which saves a finalize launch per iteration.
 __global__ void __launch_bounds__(256)
 cg_xr_update_kernel(double2* __restrict__ x, double2* __restrict__ r,
                     const double2* __restrict__ p, const double2* __restrict__ q,
                     int64_t n_cells, int nb, const CgScalars* __restrict__ s,
+                    const double2* __restrict__ pq_part, int pq_blocks,
                     double2* __restrict__ part) {
   if (s->all_done) return;
   __shared__ double2 red[256];
   for (int c = 0; c < nb; ++c) {
     double acc = 0.0;
     if (!s->done[c]) {
-      const double2 pq = s->pq[c];
+      const double2 pq = sum_partials(pq_part, pq_blocks, nb, c, red);
       const double rho = s->rr[c];
       const double den = pq.x * pq.x + pq.y * pq.y;
       const double ax = rho * pq.x / den, ay = -rho * pq.y / den;
@@ -337,10 +350,12 @@ int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
 }
 
 int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
-                     int nb, const CgScalars* sc, void* part, hipStream_t s) {
+                     int nb, const CgScalars* sc, const void* pq_part, void* part,
+                     hipStream_t s) {
   hipLaunchKernelGGL(cg_xr_update_kernel, dim3(PNH_RED_BLOCKS), dim3(256), 0, s,
                      (double2*)x, (double2*)r, (const double2*)p, (const double2*)q,
-                     n_cells, nb, sc, (double2*)part);
+                     n_cells, nb, sc, (const double2*)pq_part, PNH_RED_BLOCKS,
+                     (double2*)part);
   PNH_LAUNCH_CHECK();
   return 0;
 }
