@@ -1122,7 +1122,6 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   void* redp = w; w += red;
   void* redpq = w; w += red;
   CgScalars* sc = (CgScalars*)w;
-  const int* all_done = &sc->all_done;
   int rc;
   const int64_t K = p->Ktot;
   // b = P^H y in FP64 (y is complex64; upcast exactly as scipy does)
@@ -1130,15 +1129,17 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   if ((rc = do_y2k(p, 1, yt, vr, part, nullptr, s))) return rc;
   if ((rc = pnh_zero(1, vx, K * nb, s))) return rc;
   if ((rc = pnh_cg_dot(1, vr, vr, K, nb, redp, nullptr, s))) return rc;
-  if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_INIT, 0, maxiter, rtol, atol, s))) return rc;
+  if ((rc = pnh_cg_init(redp, nb, sc, maxiter, rtol, atol, s))) return rc;
   for (int it = 0; it < maxiter; ++it) {
-    if ((rc = pnh_cg_p_update(vp, vr, K, nb, sc, it == 0, s))) return rc;
+    // the p update finishes the previous iteration's |r|^2 and stopping test
+    const int* all_done = &sc->all_done[it & 1];
+    if ((rc = pnh_cg_p_update(vp, vr, K, nb, sc, redp, it, s))) return rc;
     if ((rc = do_k2y2k(p, 1, vp, yt, vq, part, all_done, s))) return rc;
     if ((rc = pnh_cg_dot(0, vp, vq, K, nb, redpq, all_done, s))) return rc;
-    if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, redpq, redp, s))) return rc;
-    if ((rc = pnh_cg_finalize(redp, nb, sc, CG_PHASE_RR, it, maxiter, rtol, atol, s))) return rc;
+    if ((rc = pnh_cg_xr_update(vx, vr, vp, vq, K, nb, sc, it, redpq, redp, s))) return rc;
     if (host_checks && (it & 63) == 63 && it + 1 < maxiter) {
-      // long runs: stop launching once every coil has converged
+      // long runs: stop launching once every coil has converged (as seen by
+      // this iteration's p update)
       int flag = 0;
       PNH_CHECK_HIP(hipMemcpyAsync(&flag, all_done, sizeof(int), hipMemcpyDeviceToHost, s));
       PNH_CHECK_HIP(hipStreamSynchronize(s));

@@ -8,15 +8,16 @@
 
 enum { CG_PHASE_INIT = 0, CG_PHASE_PQ = 1, CG_PHASE_RR = 2 };
 
-// Device-resident CG state, one entry per coil (independent systems).
+// Device-resident CG state, one entry per coil (independent systems).  rr,
+// done and all_done have two parity buffers: iteration `it` reads buffer
+// it&1, which its p update writes from buffer (it+1)&1 (the init phase fills
+// buffer 1).
 struct CgScalars {
-  double rr[PNH_CG_MAXNB];
-  double rr_prev[PNH_CG_MAXNB];
   double tol[PNH_CG_MAXNB];
-  double2 pq[PNH_CG_MAXNB];
-  int done[PNH_CG_MAXNB];
+  double rr[2][PNH_CG_MAXNB];
+  int done[2][PNH_CG_MAXNB];
   int iters[PNH_CG_MAXNB];
-  int all_done;
+  int all_done[2];
 };
 
 int pnh_scale_sn(int fp64, const void* in, void* out, const void* sn,
@@ -33,12 +34,15 @@ int pnh_coil_combine(const void* xc, const void* sense, void* out, int64_t n_img
 int pnh_zero(int fp64, void* p, int64_t n, hipStream_t s);
 int pnh_cg_dot(int mode, const void* a, const void* b, int64_t n_cells, int nb,
                void* part, const int* skip, hipStream_t s);
-int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter,
-                    int maxiter, double rtol, double atol, hipStream_t s);
-int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
-                    const CgScalars* sc, int first, hipStream_t s);
+// tolerance, first rr and stopping state from the |b|^2 block partials
+int pnh_cg_init(const void* part, int nb, CgScalars* sc, int maxiter, double rtol,
+                double atol, hipStream_t s);
+// iteration it: rr from the previous |r|^2 partials rr_part (it > 0), the
+// stopping test, then p = r or p = r + beta p
+int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb, CgScalars* sc,
+                    const void* rr_part, int it, hipStream_t s);
 // alpha = rr / (p.q) with p.q summed from the block partials pq_part (every
 // workgroup sums them itself); x += alpha p; r -= alpha q; |r|^2 partials
 int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
-                     int nb, const CgScalars* sc, const void* pq_part, void* part,
+                     int nb, const CgScalars* sc, int it, const void* pq_part, void* part,
                      hipStream_t s);

@@ -143,86 +143,109 @@ __device__ double2 sum_partials(const double2* __restrict__ part, int nblk, int 
   return tot;
 }
 
-// Sum the block partials of coil `c` (one block per coil), then apply the
-// CG bookkeeping of `phase`.
+// Init phase of the CG: |b|^2 from the block partials of coil `c` (one block
+// per coil) gives the tolerance and the first rr, in parity buffer 1 (the
+// one the first p update reads).
 __global__ void __launch_bounds__(256)
-cg_finalize_kernel(const double2* __restrict__ part, int nblk, int nb,
-                   CgScalars* __restrict__ s, int phase, int iter, int maxiter,
-                   double rtol, double atol) {
+cg_init_kernel(const double2* __restrict__ part, int nblk, int nb,
+               CgScalars* __restrict__ s, int maxiter, double rtol, double atol) {
   __shared__ double2 red[256];
   const int c = blockIdx.x;
-  if (phase != CG_PHASE_INIT && s->all_done) return;
   const double2 tot = sum_partials(part, nblk, nb, c, red);
   if (threadIdx.x != 0) return;
-  if (phase == CG_PHASE_INIT) {
-    const double bn = sqrt(tot.x);
-    const double tol = fmax(atol, rtol * bn);
-    s->tol[c] = tol;
-    s->rr[c] = tot.x;
-    s->rr_prev[c] = tot.x;
-    const int done = (bn == 0.0) || (bn < tol);
-    s->done[c] = done;
-    s->iters[c] = done ? 0 : maxiter;
-  } else if (phase == CG_PHASE_PQ) {
-    if (!s->done[c]) s->pq[c] = tot;
-  } else {  // CG_PHASE_RR
-    if (!s->done[c]) {
-      s->rr_prev[c] = s->rr[c];
-      s->rr[c] = tot.x;
-      if (sqrt(tot.x) < s->tol[c]) {
-        s->done[c] = 1;
-        s->iters[c] = iter + 1;
-      }
-    }
-  }
-  if (nb == 1) s->all_done = s->done[0];   // one coil: no separate AND pass
+  const double bn = sqrt(tot.x);
+  const double tol = fmax(atol, rtol * bn);
+  s->tol[c] = tol;
+  s->rr[1][c] = tot.x;
+  const int done = (bn == 0.0) || (bn < tol);
+  s->done[1][c] = done;
+  s->iters[c] = done ? 0 : maxiter;
+  if (nb == 1) s->all_done[1] = done;   // one coil: no separate AND pass
 }
 
-// all_done = AND over coils (single thread, runs after every finalize)
+// all_done[1] = AND over coils after the init phase (single thread)
 __global__ void cg_all_done_kernel(CgScalars* __restrict__ s, int nb) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int all = 1;
-  for (int c = 0; c < nb; ++c) all &= s->done[c];
-  s->all_done = all;
+  for (int c = 0; c < nb; ++c) all &= s->done[1][c];
+  s->all_done[1] = all;
 }
 
-// p = r (first iteration) or p = r + beta p, beta = rr / rr_prev
+// Iteration `it` of the CG, first step.  Reads the state of parity buffer
+// (it+1)&1 and writes buffer it&1, which the rest of the iteration reads, so
+// no workgroup reads what another writes.  For it > 0 every workgroup sums
+// the |r|^2 block partials of the previous x/r update itself (one fixed order
+// and tree): rr_new, the stopping test and beta = rr_new / rr_old; workgroup 0
+// stores the new state.  Then p = r (it == 0) or p = r + beta p.
 __global__ void __launch_bounds__(256)
 cg_p_update_kernel(double2* __restrict__ p, const double2* __restrict__ r,
-                   int64_t n_cells, int nb, const CgScalars* __restrict__ s,
-                   int first) {
-  if (s->all_done) return;
+                   int64_t n_cells, int nb, CgScalars* __restrict__ s,
+                   const double2* __restrict__ rr_part, int it) {
+  __shared__ double2 red[256];
+  __shared__ double beta[PNH_CG_MAXNB];
+  __shared__ int skip[PNH_CG_MAXNB];
+  const int ob = (it + 1) & 1, nbuf = it & 1;
+  const bool writer = blockIdx.x == 0 && threadIdx.x == 0;
+  if (s->all_done[ob]) {
+    if (writer) s->all_done[nbuf] = 1;
+    return;
+  }
+  for (int c = 0; c < nb; ++c) {
+    const int dold = s->done[ob][c];
+    const double rr_old = s->rr[ob][c];
+    double rr_new = rr_old;
+    int dnew = dold;
+    if (it > 0 && !dold) {
+      rr_new = sum_partials(rr_part, PNH_RED_BLOCKS, nb, c, red).x;
+      dnew = sqrt(rr_new) < s->tol[c];
+    }
+    if (threadIdx.x == 0) {
+      beta[c] = it > 0 && !dnew ? rr_new / rr_old : 0.0;
+      skip[c] = dnew;
+    }
+    if (writer) {
+      s->rr[nbuf][c] = rr_new;
+      s->done[nbuf][c] = dnew;
+      if (dnew && !dold) s->iters[c] = it;
+    }
+  }
+  __syncthreads();
+  if (writer) {
+    int all = 1;
+    for (int c = 0; c < nb; ++c) all &= skip[c];
+    s->all_done[nbuf] = all;
+  }
   const int64_t n = n_cells * nb;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int c = (int)(i % nb);
-    if (s->done[c]) continue;
+    if (skip[c]) continue;
     const double2 rv = r[i];
-    if (first) { p[i] = rv; continue; }
-    const double beta = s->rr[c] / s->rr_prev[c];
+    if (it == 0) { p[i] = rv; continue; }
+    const double bt = beta[c];
     double2 pv = p[i];
-    pv.x = rv.x + beta * pv.x;
-    pv.y = rv.y + beta * pv.y;
+    pv.x = rv.x + bt * pv.x;
+    pv.y = rv.y + bt * pv.y;
     p[i] = pv;
   }
 }
 
 // alpha = rr / pq; x += alpha p; r -= alpha q; block partial of |r|^2.
-// Every block sums the p.q block partials itself (same order and tree as the
-// finalize kernel), which saves a finalize launch per iteration.
+// Every block sums the p.q block partials itself (one fixed order and tree).
+// State from parity buffer `buf` (the iteration's).
 __global__ void __launch_bounds__(256)
 cg_xr_update_kernel(double2* __restrict__ x, double2* __restrict__ r,
                     const double2* __restrict__ p, const double2* __restrict__ q,
-                    int64_t n_cells, int nb, const CgScalars* __restrict__ s,
+                    int64_t n_cells, int nb, const CgScalars* __restrict__ s, int buf,
                     const double2* __restrict__ pq_part, int pq_blocks,
                     double2* __restrict__ part) {
-  if (s->all_done) return;
+  if (s->all_done[buf]) return;
   __shared__ double2 red[256];
   for (int c = 0; c < nb; ++c) {
     double acc = 0.0;
-    if (!s->done[c]) {
+    if (!s->done[buf][c]) {
       const double2 pq = sum_partials(pq_part, pq_blocks, nb, c, red);
-      const double rho = s->rr[c];
+      const double rho = s->rr[buf][c];
       const double den = pq.x * pq.x + pq.y * pq.y;
       const double ax = rho * pq.x / den, ay = -rho * pq.y / den;
       for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_cells;
@@ -327,34 +350,33 @@ int pnh_cg_dot(int mode, const void* a, const void* b, int64_t n_cells, int nb,
   return 0;
 }
 
-int pnh_cg_finalize(const void* part, int nb, CgScalars* sc, int phase, int iter,
-                    int maxiter, double rtol, double atol, hipStream_t s) {
-  hipLaunchKernelGGL(cg_finalize_kernel, dim3(nb), dim3(256), 0, s,
-                     (const double2*)part, PNH_RED_BLOCKS, nb, sc, phase, iter, maxiter,
-                     rtol, atol);
+int pnh_cg_init(const void* part, int nb, CgScalars* sc, int maxiter, double rtol,
+                double atol, hipStream_t s) {
+  hipLaunchKernelGGL(cg_init_kernel, dim3(nb), dim3(256), 0, s, (const double2*)part,
+                     PNH_RED_BLOCKS, nb, sc, maxiter, rtol, atol);
   PNH_LAUNCH_CHECK();
-  // the p.q phase changes no coil's state; one coil sets all_done itself
-  if (nb > 1 && phase != CG_PHASE_PQ) {
+  if (nb > 1) {
     hipLaunchKernelGGL(cg_all_done_kernel, dim3(1), dim3(64), 0, s, sc, nb);
     PNH_LAUNCH_CHECK();
   }
   return 0;
 }
 
-int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb,
-                    const CgScalars* sc, int first, hipStream_t s) {
+int pnh_cg_p_update(void* p, const void* r, int64_t n_cells, int nb, CgScalars* sc,
+                    const void* rr_part, int it, hipStream_t s) {
   hipLaunchKernelGGL(cg_p_update_kernel, dim3(grid_for(n_cells * nb)), dim3(256), 0, s,
-                     (double2*)p, (const double2*)r, n_cells, nb, sc, first);
+                     (double2*)p, (const double2*)r, n_cells, nb, sc,
+                     (const double2*)rr_part, it);
   PNH_LAUNCH_CHECK();
   return 0;
 }
 
 int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_cells,
-                     int nb, const CgScalars* sc, const void* pq_part, void* part,
+                     int nb, const CgScalars* sc, int it, const void* pq_part, void* part,
                      hipStream_t s) {
   hipLaunchKernelGGL(cg_xr_update_kernel, dim3(PNH_RED_BLOCKS), dim3(256), 0, s,
                      (double2*)x, (double2*)r, (const double2*)p, (const double2*)q,
-                     n_cells, nb, sc, (const double2*)pq_part, PNH_RED_BLOCKS,
+                     n_cells, nb, sc, it & 1, (const double2*)pq_part, PNH_RED_BLOCKS,
                      (double2*)part);
   PNH_LAUNCH_CHECK();
   return 0;
