@@ -115,7 +115,8 @@ def tune(om, Nd, Kd, Jd, batch=None, host_plan=None, device=None, reps=5,
 
     Returns ``{'tile': tuple, 'threads': int, 'chunk': int or None, 'ms':
     float}``.  Two stages: every (tile, threads) candidate with the default
-    rows per item, then the rows per item for the winner.
+    rows per item, then every (rows per item, threads) pair for the winning
+    tile.
     """
     from .nufft import NUFFT
     Nd, Kd, Jd = tuple(Nd), tuple(Kd), tuple(Jd)
@@ -152,10 +153,14 @@ def tune(om, Nd, Kd, Jd, batch=None, host_plan=None, device=None, reps=5,
         ms = measure(tile, threads, None)
         if best is None or ms < best['ms']:
             best = {'tile': tile, 'threads': threads, 'chunk': None, 'ms': ms}
+    # rows per item and workgroup size act together (rows per thread), so the
+    # second stage tries every pair of them for the winning tile
+    thr_opts = sorted({thr for t, thr in candidates(Kd, Jd) if t == best['tile']})
     for chunk in chunks:
-        ms = measure(best['tile'], best['threads'], chunk)
-        if ms < best['ms']:
-            best = dict(best, chunk=chunk, ms=ms)
+        for thr in thr_opts:
+            ms = measure(best['tile'], thr, chunk)
+            if ms < best['ms']:
+                best = dict(best, threads=thr, chunk=chunk, ms=ms)
     if cache:
         _store(key, {'tile': list(best['tile']), 'threads': best['threads'],
                      'chunk': best['chunk'], 'ms': best['ms']})
