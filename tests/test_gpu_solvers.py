@@ -199,6 +199,66 @@ def test_cg_default_maxiter_converges(hip):
     assert dt < 5.0
 
 
+@pytest.mark.parametrize('rtol', [3.9e-4, 7.8e-5])
+def test_cg_long_runs_stop_early(hip, rtol):
+    """Solves that run past the launch loop's 64-iteration convergence checks
+    (about 70 and 130 iterations on plain 2-D, a singular system: M < K).
+    The early stop returns exactly the image of a fixed-length solve of that
+    many iterations, and the count is within a few of scipy's on the
+    oracle's operator.  It is not compared exactly: on this system CG
+    amplifies rounding, and scipy's own count moves by a few iterations
+    between CPUs."""
+    import oracle
+    import scipy.sparse.linalg as sla
+    A, g = make('plain2d')
+    y = g['adjoint_in']
+    x = A.solve(y, 'cg', maxiter=400, rtol=rtol)
+    n = A.cg_iters
+    assert 64 < n < 400
+    x_fixed = A.solve(y, 'cg', maxiter=n, rtol=0.0)
+    assert A.cg_iters == n
+    assert np.array_equal(x, x_fixed)
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    its = [0]
+
+    def mv(v):
+        its[0] += 1
+        return B.k2y2k(v.reshape(B.Kd)).ravel()
+    K = B.Kdprod
+    op = sla.LinearOperator((K, K), dtype=np.complex128, matvec=mv, rmatvec=mv)
+    k, info = sla.cg(op, B.y2k(y).ravel(), rtol=rtol, maxiter=400)
+    assert info == 0 and abs(its[0] - n) <= 0.1 * n, (its[0], n)
+    assert rel(x, B.k2xx(k.reshape(B.Kd)) / B.sn) < 1e-2
+
+
+def test_cg_coils_stop_independently(hip):
+    """batch=3 (coils share the trajectory, independent systems): each coil
+    stops at its own iteration, past the 64-iteration checks, near the count
+    of the same system solved alone and with its image.  Not bit for bit:
+    coils gridded in one pass share a fixed-point exponent (the largest
+    sample of the pass), which changes rounding at the 1e-15 level, and CG
+    on this singular system (M < K) amplifies that by a few iterations."""
+    import oracle
+    from pynufft_amd import workloads as W
+    Ab, g = make('plain2d', batch=3)
+    A1, _ = make('plain2d')
+    B = oracle.OracleNUFFT()
+    B.plan(g['om'], *geometry(g))
+    y0 = g['adjoint_in']
+    y2 = B.forward(W.gaussian_image(B.Nd, 7))
+    ys = np.stack([y0, np.conj(y0) * 0.5, y2], axis=-1).astype(np.complex64)
+    xb = Ab.solve(ys, 'cg', maxiter=300, rtol=2.1e-4)
+    itb = np.asarray(Ab.cg_iters)
+    it1 = []
+    for c in range(3):
+        x1 = A1.solve(ys[:, c], 'cg', maxiter=300, rtol=2.1e-4)
+        it1.append(A1.cg_iters)
+        assert abs(int(itb[c]) - it1[-1]) <= 3, (c, itb, it1)
+        assert rel(xb[..., c], x1) < 1e-2, c     # 1.1e-3 measured
+    assert itb.max() > 64 and itb.max() - itb.min() >= 5, itb
+
+
 @pytest.mark.parametrize('name', SOLVER_CASES)
 def test_solve_gmres(hip, name):
     """scipy's gmres (restart 4, 2 cycles) on P^H P in FP64 against the
