@@ -8,10 +8,17 @@ parts (seed 302).  A step is one ``forward`` followed by one ``adjoint`` with
 the image, the samples and the plan resident in HBM.  Prints one JSON line.
 
 ``roofline`` times the dominant kernel (k2y or y2k, whichever takes longer)
-with HIP events on the stream it is launched on and divides its algorithmic
-bytes per launch by the mean launch time.  ``cpu_baseline`` times the CPU
-oracle (a restatement of pynufft's NUFFT() path) on a bounded sample.
+with HIP events on the stream it is launched on and divides the bytes the
+running format must move per launch (``physical_bytes``) by the mean launch
+time.  The coefficient-streaming (CSR) formula's bytes, which count
+prod(Jd) coefficients per sample that the factor format never reads, are
+reported separately as ``csr_equivalent``; y2k also carries its LDS-atomic
+ceiling.  ``traffic`` is the measured HBM traffic per launch from
+``profiles/traffic.json``, reported only when that file was measured on the
+same kernel sources.  ``cpu_baseline`` times the CPU oracle (a restatement
+of pynufft's NUFFT() path) on a bounded sample.
 """
+import hashlib
 import argparse
 import json
 import os
@@ -26,6 +33,12 @@ sys.path.insert(0, HERE)
 METRIC = ("3D 128³ NUFFT fwd+adj pairs/s (% HBM peak); "
           "batched vs plain CG solves/s")
 HBM_PEAK_GBS = 8000.0
+# LDS atomic throughput on gfx950: one wave-wide ds_add_u64 issue every
+# LDS_ADD_U64_CYCLES cycles per CU (microbenchmark, profiles/lds_atomic_rate.txt),
+# 256 CUs at the 2.4 GHz peak engine clock
+LDS_ADD_U64_CYCLES = 8.0
+N_CU = 256
+CLOCK_HZ = 2.4e9
 
 
 def progress(msg):
@@ -54,6 +67,25 @@ def physical_bytes(fmt, M, Jd, Kd, nb=1):
     P = int(np.prod(Jd))
     coef = P * 12 if fmt == 'CSR' else int(np.sum(Jd)) * 8 + d * 4
     return M * (coef + 4 + 8 * nb) + K * 8 * nb
+
+
+def lds_atomic_floor_ms(M, Jd, nb=1):
+    """Lower bound of y2k's gridding time from LDS atomics alone: two
+    ds_add_u64 (real and imaginary fixed-point parts) per coefficient,
+    64 lanes per wave-wide instruction."""
+    waves = 2.0 * M * int(np.prod(Jd)) * nb / 64.0
+    return 1e3 * waves * LDS_ADD_U64_CYCLES / (N_CU * CLOCK_HZ)
+
+
+def source_hash():
+    """SHA-1 (16 hex digits) of the kernel sources; profiles record the hash
+    of the build they were measured on."""
+    h = hashlib.sha1()
+    csrc = os.path.join(HERE, 'pynufft_amd', 'csrc')
+    for name in sorted(os.listdir(csrc)):
+        with open(os.path.join(csrc, name), 'rb') as f:
+            h.update(name.encode() + b'\0' + f.read())
+    return h.hexdigest()[:16]
 
 
 def fft_bytes(Nd, Kd, nb=1):
@@ -149,7 +181,9 @@ def bench_batched(torch, W, maxiter=20, reps=3):
     t_plain, plain = timed(lambda: [op.solve(ys[i], 'cg', maxiter=maxiter, rtol=0.0)
                                     for i, op in enumerate(plain_ops)])
     t_batch, xb = timed(lambda: B.solve(y, maxiter=maxiter, rtol=0.0))
+    res_plain = np.array([op.cg_resid for op in plain_ops])
     t_graph, xg = timed(lambda: B.solve_graph(y, maxiter=maxiter, rtol=0.0))
+    res_batch = B.cg_resid.cpu().numpy()
     xgl = B.unpack_images(xg)
     diff = max(float(torch.linalg.vector_norm(xgl[i] - plain[i]) /
                      torch.linalg.vector_norm(plain[i])) for i in range(B.n))
@@ -160,7 +194,10 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         'batched_solves_per_s': n / t_batch,
         'batched_graph_solves_per_s': n / t_graph,
         'speedup_graph_vs_plain': t_plain / t_graph,
-        'max_rel_diff_vs_plain': diff, 'plan_s': t_plan,
+        'max_rel_diff_vs_plain': diff,
+        'final_resid_max_rel_diff_vs_plain': float(np.max(np.abs(res_batch - res_plain) /
+                                                          res_plain)),
+        'final_resid_median': float(np.median(res_batch)), 'plan_s': t_plan,
         'tuned': {'tile': B.tuned['tile'], 'chunk': B.tuned['chunk'],
                   'threads': B.tuned['threads']},
     }
@@ -335,13 +372,19 @@ def main():
         kname, kms = 'k2y', t_k2y
     else:
         kname, kms = 'y2k', t_y2k
-    nbytes = algorithmic_bytes(kname, fmt, M, w['Jd'], w['Kd'])
-    achieved = nbytes / (kms * 1e-3) / 1e9
-    traffic = None
+    phys = physical_bytes(fmt, M, w['Jd'], w['Kd'])
+    csr_bytes = algorithmic_bytes(kname, fmt, M, w['Jd'], w['Kd'])
+    achieved = phys / (kms * 1e-3) / 1e9
+    traffic, traffic_note = None, 'no measurement for this workload'
     tf = os.path.join(HERE, 'profiles', 'traffic.json')
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get('%s_%s_%s' % (args.workload, fmt, kname))
+            rec = json.load(f)
+        key = '%s_%s_%s' % (args.workload, fmt, kname)
+        if rec.get('source_sha16') != source_hash():
+            traffic_note = 'profiles/traffic.json was measured on other kernel sources'
+        elif key in rec:
+            traffic, traffic_note = rec[key], 'rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated'
 
     if args.copy_probe_mib:
         # a plain elementwise copy (reads and writes copy_probe_mib MiB each)
@@ -375,8 +418,30 @@ def main():
         progress('CPU oracle baseline')
         cpu = cpu_baseline(w, min(args.cpu_sample, M))
 
-    pair_bytes = (2 * algorithmic_bytes('k2y', fmt, M, w['Jd'], w['Kd']) +
+    pair_bytes = (2 * physical_bytes(fmt, M, w['Jd'], w['Kd']) +
                   2 * fft_bytes(w['Nd'], w['Kd']))
+    pair_csr_bytes = (2 * algorithmic_bytes('k2y', fmt, M, w['Jd'], w['Kd']) +
+                      2 * fft_bytes(w['Nd'], w['Kd']))
+    roofline = {'bound': 'hbm', 'kernel': kname, 'achieved': achieved,
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                'traffic': traffic, 'traffic_note': traffic_note,
+                'algorithmic_bytes': phys, 'launch_ms': kms,
+                'bytes_note': ('bytes the %s format must move per launch: per sample '
+                               'sum(Jd) complex64 factors, ndim int32 first cells, the '
+                               'int32 row index and the sample; the grid once' % fmt),
+                'csr_equivalent': {
+                    'bytes': csr_bytes, 'achieved': csr_bytes / (kms * 1e-3) / 1e9,
+                    'note': 'M*prod(Jd)*8 coefficient bytes of a coefficient-streaming '
+                            'kernel, which the factor format does not read'}}
+    if kname == 'y2k':
+        floor = lds_atomic_floor_ms(M, w['Jd'])
+        roofline['lds_atomic'] = {
+            'floor_ms': floor, 'frac': floor / kms,
+            'note': '2 ds_add_u64 per coefficient, %.0f cycles per wave-wide issue per '
+                    'CU, %d CUs at %.1f GHz' % (LDS_ADD_U64_CYCLES, N_CU, CLOCK_HZ / 1e9)}
+    kern_frac = {n: physical_bytes(fmt, M, w['Jd'], w['Kd']) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
+                 for n, t in (('k2y', t_k2y), ('y2k', t_y2k))}
+    kern_frac['xx2k'] = fft_bytes(w['Nd'], w['Kd']) / (t_fft * 1e-3) / 1e9 / HBM_PEAK_GBS
     out = {
         'metric': METRIC, 'value': value, 'unit': 'pairs/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
@@ -385,14 +450,13 @@ def main():
         'config': {'workload': args.workload, 'Nd': list(w['Nd']),
                    'Kd': list(w['Kd']), 'Jd': list(w['Jd']), 'M': M,
                    'format': fmt, 'step': 'forward+adjoint'},
-        'roofline': {'bound': 'hbm', 'kernel': kname, 'achieved': achieved,
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                     'algorithmic_bytes': nbytes, 'launch_ms': kms,
-                     'physical_bytes': physical_bytes(fmt, M, w['Jd'], w['Kd'])},
+        'roofline': roofline,
         'kernels_ms': {'k2y': t_k2y, 'y2k': t_y2k, 'xx2k': t_fft},
+        'kernels_hbm_frac': kern_frac,
         'pair_hbm_frac': pair_bytes * value / 1e9 / HBM_PEAK_GBS,
         'pair_algorithmic_bytes': pair_bytes,
+        'pair_csr_equivalent_bytes': pair_csr_bytes,
+        'source_sha16': source_hash(),
         'plan_s': t_plan,
         'batched_cg': batched,
         'cg_single': cg_single,

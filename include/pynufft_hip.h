@@ -105,6 +105,15 @@ typedef struct pnh_plan_desc {
   int32_t max_tile_slots;    /* largest slot count of any tile (sizes the gather) */
   int32_t threads;           /* gridding workgroup size: multiple of 64 in
                                 64..1024, or 0 to let the library choose */
+  /* 1: pnh_forward, pnh_adjoint and pnh_selfadjoint (and the operators
+   * built on them) keep complex64 inputs and outputs but run the
+   * apodization, FFTs and interpolation in FP64 (needs u_c128, tw_c128 and
+   * sn_f64).  For plans whose apodization spans a large range
+   * (max|sn| / min|sn|, e.g. Kd == Nd), where the interpolation cancels
+   * large terms and FP32 arithmetic loses digits that the reference's
+   * complex128 CSR products (nufft/_nufft_class_methods_cpu.py:269-314) and
+   * complex128 ifftn (:316-329) keep.  0: FP32 arithmetic. */
+  int32_t fp64_ops;
 } pnh_plan_desc;
 
 typedef struct pnh_plan pnh_plan;
@@ -249,12 +258,15 @@ int pnh_solve_lgmres(const pnh_plan* plan, const void* y, void* x, int maxiter, 
  * FP64 (complex128), x0 = 0, stop when ||r|| < max(rtol ||b||, atol) at the
  * start of an iteration or after maxiter iterations; result
  * x = crop(ifftn(k)) / sn cast to complex64.  The loop runs on the device with
- * no host synchronisation; iters_out (device int32[batch]) receives the
- * iteration count per coil.
+ * no host synchronisation; iters_out (device int32[batch], may be NULL)
+ * receives the iteration count per coil and resid_out (device double[batch],
+ * may be NULL) the final ||r|| per coil: the value of the stopping test
+ * when it held, else the norm after the last iteration (||b|| for
+ * maxiter = 0).
  * Reference linalg/solve_cpu.py:256-288 with scipy.sparse.linalg.cg. */
 int pnh_solve_cg(const pnh_plan* plan, const void* y, void* x, int maxiter,
                  double rtol, double atol, void* ws, int32_t* iters_out,
-                 void* stream);
+                 double* resid_out, void* stream);
 
 /* ---- many independent problems (uneven Nd, Kd, M; same ndim and Jd) ----
  * The reference handles one problem per NUFFT object; a batch here runs every
@@ -279,21 +291,23 @@ int pnh_batch_workspace_bytes(const pnh_batch* batch, int op, int dtype, size_t*
 /* Batched forward / adjoint (complex64), one launch per stage for all problems. */
 int pnh_batch_forward(const pnh_batch* batch, const void* x, void* y, void* ws, void* stream);
 int pnh_batch_adjoint(const pnh_batch* batch, const void* y, void* x, void* ws, void* stream);
-/* Batched FP64 CG, semantics of pnh_solve_cg per problem (iters_out: int32[n]). */
+/* Batched FP64 CG, semantics of pnh_solve_cg per problem (iters_out: int32[n],
+ * resid_out: double[n], each may be NULL). */
 int pnh_batch_solve_cg(const pnh_batch* batch, const void* y, void* x, int maxiter,
-                       double rtol, double atol, void* ws, int32_t* iters_out, void* stream);
+                       double rtol, double atol, void* ws, int32_t* iters_out,
+                       double* resid_out, void* stream);
 /* Capture pnh_batch_solve_cg with these buffers into a HIP graph, and replay
  * it on any stream; the buffers must outlive the graph. */
 int pnh_batch_cg_graph_create(const pnh_batch* batch, const void* y, void* x, int maxiter,
                               double rtol, double atol, void* ws, int32_t* iters_out,
-                              pnh_graph** out);
+                              double* resid_out, pnh_graph** out);
 /* Capture pnh_solve_cg of one plan with these buffers into a HIP graph
  * (every iteration launched; converged coils skip their work on the device);
  * replay with pnh_graph_launch.  No counterpart in the reference (new): for
  * repeated solves on one trajectory without per-kernel launch cost. */
 int pnh_cg_graph_create(const pnh_plan* plan, const void* y, void* x, int maxiter,
                         double rtol, double atol, void* ws, int32_t* iters_out,
-                        pnh_graph** out);
+                        double* resid_out, pnh_graph** out);
 int pnh_graph_launch(pnh_graph* graph, void* stream);
 int pnh_graph_destroy(pnh_graph* graph);
 

@@ -52,6 +52,7 @@ class PlanDesc(ctypes.Structure):
         ('n_slots', ctypes.c_int64),
         ('max_tile_slots', ctypes.c_int32),
         ('threads', ctypes.c_int32),
+        ('fp64_ops', ctypes.c_int32),
     ]
 
 
@@ -105,7 +106,7 @@ EXPORTS = {
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       _vp, _vp, _vp]),
     'pnh_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                    ctypes.c_double, _vp, _vp, _vp]),
+                                    ctypes.c_double, _vp, _vp, _vp, _vp]),
     'pnh_batch_table_bytes': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_size_t)]),
     'pnh_batch_create': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int, _vp, _vp,
@@ -119,12 +120,13 @@ EXPORTS = {
     'pnh_batch_forward': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_batch_adjoint': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
     'pnh_batch_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                          ctypes.c_double, _vp, _vp, _vp]),
+                                          ctypes.c_double, _vp, _vp, _vp, _vp]),
     'pnh_batch_cg_graph_create': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                                 ctypes.c_double, _vp, _vp,
+                                                 ctypes.c_double, _vp, _vp, _vp,
                                                  ctypes.POINTER(_vp)]),
     'pnh_cg_graph_create': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                           ctypes.c_double, _vp, _vp, ctypes.POINTER(_vp)]),
+                                           ctypes.c_double, _vp, _vp, _vp,
+                                           ctypes.POINTER(_vp)]),
     'pnh_graph_launch': (ctypes.c_int, [_vp, _vp]),
     'pnh_graph_destroy': (ctypes.c_int, [_vp]),
 }

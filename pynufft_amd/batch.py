@@ -161,10 +161,13 @@ class BatchNUFFT:
             y.to(self.device, torch.complex64).contiguous()
         x = torch.empty(self.n_image, dtype=torch.complex64, device=self.device)
         iters = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+        resid = torch.zeros(self.n, dtype=torch.float64, device=self.device)
         ws = self._workspace(_lib.OP_SOLVE_CG)
         self._call(self._libh.pnh_batch_solve_cg, _ptr(yt), _ptr(x), int(maxiter),
-                   float(rtol), float(atol), _ptr(ws), _ptr(iters), self._stream())
+                   float(rtol), float(atol), _ptr(ws), _ptr(iters), _ptr(resid),
+                   self._stream())
         self.cg_iters = iters
+        self.cg_resid = resid
         return x
 
     def solve_graph(self, y, maxiter=20, rtol=1e-5, atol=0.0):
@@ -176,19 +179,21 @@ class BatchNUFFT:
             yb = torch.empty(self.n_samples, dtype=torch.complex64, device=self.device)
             xb = torch.empty(self.n_image, dtype=torch.complex64, device=self.device)
             it = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+            res = torch.zeros(self.n, dtype=torch.float64, device=self.device)
             ws = torch.empty_like(self._workspace(_lib.OP_SOLVE_CG))
             h = ctypes.c_void_p()
             torch.cuda.synchronize(self.device)
             with torch.cuda.device(self.device):
                 _lib.check(self._libh.pnh_batch_cg_graph_create(
                     self._h, _ptr(yb), _ptr(xb), key[0], key[1], key[2], _ptr(ws),
-                    _ptr(it), ctypes.byref(h)))
-            g = (h, yb, xb, it, ws)
+                    _ptr(it), _ptr(res), ctypes.byref(h)))
+            g = (h, yb, xb, it, res, ws)
             self._graphs[key] = g
-        h, yb, xb, it, ws = g
+        h, yb, xb, it, res, ws = g
         yt = self.pack_samples(y) if isinstance(y, (list, tuple)) else y
         yb.copy_(yt.to(self.device, torch.complex64).reshape(-1))
         with torch.cuda.device(self.device):
             _lib.check(self._libh.pnh_graph_launch(h, self._stream()))
         self.cg_iters = it
+        self.cg_resid = res
         return xb.clone()

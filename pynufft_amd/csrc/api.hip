@@ -317,7 +317,90 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
     p->bins.threads = d.threads;
   }
   p->slot_nbc = p->nb >= 4 ? 4 : (p->nb >= 2 ? 2 : 1);
+  REQUIRE(d.fp64_ops == 0 || d.fp64_ops == 1, "fp64_ops must be 0 or 1");
+  if (d.fp64_ops) {
+    REQUIRE(d.u_c128 && d.sn_f64, "fp64_ops needs u_c128 and sn_f64");
+    for (int a = 0; a < p->D; ++a) REQUIRE(d.tw_c128[a], "fp64_ops needs tw_c128[%d]", a);
+  }
   return 0;
+}
+
+// Scratch of the complex64 operators.  FP32 plans: forward the grid; adjoint
+// the grid and the gridding partials; selfadjoint the grid, the samples and
+// the partials.  fp64_ops plans: complex128 image, grid and samples plus the
+// FP64 partials (forward needs no partials).
+size_t forward_ws_bytes(const pnh_plan* p) {
+  const size_t g = align_up((size_t)p->Ktot * p->nb * (p->d.fp64_ops ? 16 : 8));
+  if (!p->d.fp64_ops) return g;
+  return g + align_up((size_t)p->Ntot * p->nb * 16) + align_up((size_t)p->M * p->nb * 16);
+}
+
+size_t adjoint_ws_bytes(const pnh_plan* p) {
+  if (!p->d.fp64_ops)
+    return align_up((size_t)p->Ktot * p->nb * 8) + align_up(partial_bytes(p, 0));
+  return align_up((size_t)p->M * p->nb * 16) + align_up((size_t)p->Ktot * p->nb * 16) +
+         align_up((size_t)p->Ntot * p->nb * 16) + align_up(partial_bytes(p, 1));
+}
+
+size_t selfadjoint_ws_bytes(const pnh_plan* p) {
+  if (!p->d.fp64_ops)
+    return align_up((size_t)p->Ktot * p->nb * 8) + align_up((size_t)p->M * p->nb * 8) +
+           align_up(partial_bytes(p, 0));
+  return align_up((size_t)p->Ntot * p->nb * 16) + align_up((size_t)p->Ktot * p->nb * 16) +
+         align_up((size_t)p->M * p->nb * 16) + align_up(partial_bytes(p, 1));
+}
+
+// x (complex64) = xx2x(k2xx(y2k(v))) for complex128 samples v, with the
+// reference's precision: FP64 gridding and inverse FFT, rounded to complex64
+// before the apodization.
+int adjoint_c128(const pnh_plan* p, const void* v, void* x, void* g64, void* img64,
+                 void* part, hipStream_t s) {
+  int rc;
+  if ((rc = do_y2k(p, 1, v, g64, part, nullptr, s))) return rc;
+  if ((rc = run_passes_inverse(p, g64, img64, false, 1, nullptr, s))) return rc;
+  return pnh_cast_mul_sn(img64, x, p->d.sn_f32, p->Ntot, p->nb, s);
+}
+
+// The complex64 operators of an fp64_ops plan: the image is upcast, x * sn,
+// the FFTs and the interpolation run in FP64, the samples are rounded to
+// complex64, and the image is rounded to complex64 before the final
+// apodization (as the reference's k2xx / xx2x do).
+int forward_fp64(const pnh_plan* p, const void* x, void* y, void* ws, hipStream_t s) {
+  char* w = (char*)ws;
+  void* g64 = w; w += align_up((size_t)p->Ktot * p->nb * 16);
+  void* x64 = w; w += align_up((size_t)p->Ntot * p->nb * 16);
+  void* y64 = w;
+  int rc;
+  if ((rc = pnh_c64_to_c128(x, x64, p->Ntot * p->nb, s))) return rc;
+  if ((rc = run_passes_forward(p, x64, g64, true, 1, nullptr, s))) return rc;
+  if (p->M == 0) return 0;
+  if ((rc = do_k2y(p, 1, g64, y64, nullptr, s))) return rc;
+  return pnh_c128_to_c64(y64, y, p->M * p->nb, s);
+}
+
+int adjoint_fp64(const pnh_plan* p, const void* y, void* x, void* ws, hipStream_t s) {
+  char* w = (char*)ws;
+  void* y64 = w; w += align_up((size_t)p->M * p->nb * 16);
+  void* g64 = w; w += align_up((size_t)p->Ktot * p->nb * 16);
+  void* img64 = w; w += align_up((size_t)p->Ntot * p->nb * 16);
+  void* part = w;
+  int rc;
+  if (p->M > 0 && (rc = pnh_c64_to_c128(y, y64, p->M * p->nb, s))) return rc;
+  return adjoint_c128(p, y64, x, g64, img64, part, s);
+}
+
+int selfadjoint_fp64(const pnh_plan* p, const void* x, void* x_out, void* ws, hipStream_t s) {
+  char* w = (char*)ws;
+  void* img64 = w; w += align_up((size_t)p->Ntot * p->nb * 16);
+  void* g64 = w; w += align_up((size_t)p->Ktot * p->nb * 16);
+  void* y64 = w; w += align_up((size_t)p->M * p->nb * 16);
+  void* part = w;
+  int rc;
+  if ((rc = pnh_c64_to_c128(x, img64, p->Ntot * p->nb, s))) return rc;
+  if ((rc = run_passes_forward(p, img64, g64, true, 1, nullptr, s))) return rc;
+  if ((rc = do_k2y2k(p, 1, g64, y64, g64, part, nullptr, s))) return rc;
+  if ((rc = run_passes_inverse(p, g64, img64, false, 1, nullptr, s))) return rc;
+  return pnh_cast_mul_sn(img64, x_out, p->d.sn_f32, p->Ntot, p->nb, s);
 }
 
 }  // namespace
@@ -354,22 +437,26 @@ int pnh_plan_destroy(pnh_plan* plan) {
   return 0;
 }
 
-// Scratch of density(): ones [M nb], image [N nb], adjoint scratch (grid, partials).
+// Scratch of density(): ones [M nb], image [N nb], then the adjoint's
+// scratch, which afterwards holds the complex64 grid of the forward FFT.
 static size_t density_ws_bytes(const pnh_plan* p) {
+  const size_t adj = adjoint_ws_bytes(p);
+  const size_t grid = align_up((size_t)p->Ktot * p->nb * 8);
   return align_up((size_t)p->M * p->nb * 8) + align_up((size_t)p->Ntot * p->nb * 8) +
-         align_up((size_t)p->Ktot * p->nb * 8) + align_up(partial_bytes(p, 0));
+         (adj > grid ? adj : grid);
 }
 
 // Scratch of the L1TVOLS solve: uker and |density| (float [K]), images AHy,
-// AHyk, rhs, q and dd/bb per axis, then self-adjoint scratch (grid, samples,
-// partials), which also serves density() and the k-space solve.
+// AHyk, rhs, q and dd/bb per axis, then the self-adjoint scratch, which also
+// serves density(), the adjoint and the k-space solve's complex64 grid.
 static size_t l1tv_ws_bytes(const pnh_plan* p) {
   const size_t img = align_up((size_t)p->Ntot * 8);
-  const size_t sa = align_up((size_t)p->Ktot * 8) + align_up((size_t)p->M * 8) +
-                    align_up(partial_bytes(p, 0));
+  size_t sa = selfadjoint_ws_bytes(p);
   const size_t dens = density_ws_bytes(p);
-  return 2 * align_up((size_t)p->Ktot * 4) + (4 + 2 * (size_t)p->D) * img +
-         (sa > dens ? sa : dens);
+  const size_t adj = adjoint_ws_bytes(p);
+  sa = sa > dens ? sa : dens;
+  sa = sa > adj ? sa : adj;
+  return 2 * align_up((size_t)p->Ktot * 4) + (4 + 2 * (size_t)p->D) * img + sa;
 }
 
 // Scratch of the 'dc' solve: W, E and the weighted samples (complex128
@@ -399,16 +486,16 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
   const size_t ys = align_up((size_t)p->M * p->nb * esz);
   const size_t part = align_up(partial_bytes(p, fp64));
   switch (op) {
-    case PNH_OP_FORWARD: *bytes = grid; break;
-    case PNH_OP_ADJOINT: *bytes = grid + part; break;
-    case PNH_OP_SELFADJOINT: *bytes = grid + ys + part; break;
+    case PNH_OP_FORWARD: *bytes = forward_ws_bytes(p); break;
+    case PNH_OP_ADJOINT: *bytes = adjoint_ws_bytes(p); break;
+    case PNH_OP_SELFADJOINT: *bytes = selfadjoint_ws_bytes(p); break;
     case PNH_OP_Y2K: *bytes = part; break;
     case PNH_OP_K2Y2K: *bytes = ys + part; break;
     case PNH_OP_FORWARD_ONE2MANY:
-      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + grid;
+      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + forward_ws_bytes(p);
       break;
     case PNH_OP_ADJOINT_MANY2ONE:
-      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + grid + part;
+      *bytes = align_up((size_t)p->Ntot * p->nb * 8) + adjoint_ws_bytes(p);
       break;
     case PNH_OP_PRECOMPUTE_SP:
       *bytes = density_ws_bytes(p);
@@ -496,6 +583,7 @@ int pnh_forward(const pnh_plan* p, const void* x, void* y, void* ws, void* strea
   int rc = check_dtype_ready(p, PNH_C64);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (p->d.fp64_ops) return forward_fp64(p, x, y, ws, s);
   if ((rc = run_passes_forward(p, x, ws, true, 0, nullptr, s))) return rc;
   if (p->M == 0) return 0;
   return do_k2y(p, 0, ws, y, nullptr, s);
@@ -506,6 +594,7 @@ int pnh_adjoint(const pnh_plan* p, const void* y, void* x, void* ws, void* strea
   int rc = check_dtype_ready(p, PNH_C64);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (p->d.fp64_ops) return adjoint_fp64(p, y, x, ws, s);
   char* w = (char*)ws;
   void* grid = w;
   void* part = w + align_up((size_t)p->Ktot * p->nb * 8);
@@ -518,6 +607,7 @@ int pnh_selfadjoint(const pnh_plan* p, const void* x, void* x_out, void* ws, voi
   int rc = check_dtype_ready(p, PNH_C64);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (p->d.fp64_ops) return selfadjoint_fp64(p, x, x_out, ws, s);
   char* w = (char*)ws;
   void* grid = w;
   void* ytmp = w + align_up((size_t)p->Ktot * p->nb * 8);
@@ -553,7 +643,7 @@ int pnh_adjoint_many2one(const pnh_plan* p, const void* y, const void* sense, vo
 static int density(const pnh_plan* p, float* W, char* w, hipStream_t s) {
   void* ones = w;
   void* img = w + align_up((size_t)p->M * p->nb * 8);
-  char* aws = (char*)img + align_up((size_t)p->Ntot * p->nb * 8);
+  char* aws = (char*)img + align_up((size_t)p->Ntot * p->nb * 8);   // adjoint scratch, then grid
   int rc;
   if ((rc = pnh_fill_c64(ones, p->M * p->nb, 1.f, 0.f, s))) return rc;
   if ((rc = pnh_adjoint(p, ones, img, aws, s))) return rc;
@@ -576,17 +666,6 @@ int pnh_selfadjoint2(const pnh_plan* p, const void* x, const float* W, void* x_o
   if ((rc = run_passes_forward(p, x, ws, false, 0, nullptr, s))) return rc;
   if ((rc = pnh_scale_sn(0, ws, ws, W, p->Ktot, p->nb, s))) return rc;
   return run_passes_inverse(p, ws, x_out, false, 0, nullptr, s);
-}
-
-// x (complex64) = xx2x(k2xx(y2k(v))) for complex128 samples v, with the
-// reference's precision: FP64 gridding and inverse FFT, rounded to complex64
-// before the apodization.
-static int adjoint_c128(const pnh_plan* p, const void* v, void* x, void* g64, void* img64,
-                        void* part, hipStream_t s) {
-  int rc;
-  if ((rc = do_y2k(p, 1, v, g64, part, nullptr, s))) return rc;
-  if ((rc = run_passes_inverse(p, g64, img64, false, 1, nullptr, s))) return rc;
-  return pnh_cast_mul_sn(img64, x, p->d.sn_f32, p->Ntot, p->nb, s);
 }
 
 int pnh_solve_dc(const pnh_plan* p, const void* y, void* x, int maxiter, void* ws,
@@ -1097,8 +1176,8 @@ int pnh_forward_grad_om(const pnh_plan* p, const void* k, const void* g, float* 
 // The CG solve; host_checks: read the convergence flag back every 64
 // iterations and stop launching (off while a graph is being captured).
 static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
-                    double atol, void* ws, int32_t* iters_out, void* stream,
-                    bool host_checks) {
+                    double atol, void* ws, int32_t* iters_out, double* resid_out,
+                    void* stream, bool host_checks) {
   REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
   REQUIRE(maxiter >= 0, "maxiter must be >= 0");
   REQUIRE(p->nb <= PNH_CG_MAXNB, "solve supports at most %d coils", PNH_CG_MAXNB);
@@ -1153,12 +1232,14 @@ static int cg_solve(const pnh_plan* p, const void* y, void* x, int maxiter, doub
   if (iters_out)
     PNH_CHECK_HIP(hipMemcpyAsync(iters_out, sc->iters, sizeof(int) * nb,
                                  hipMemcpyDeviceToDevice, s));
+  if (resid_out && (rc = pnh_cg_resid(redp, nb, sc, resid_out, s))) return rc;
   return 0;
 }
 
 int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
-                 double atol, void* ws, int32_t* iters_out, void* stream) {
-  return cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, stream, true);
+                 double atol, void* ws, int32_t* iters_out, double* resid_out,
+                 void* stream) {
+  return cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, resid_out, stream, true);
 }
 
 }  // extern "C"
@@ -1234,6 +1315,8 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
     REQUIRE(p.nb == 1, "batched problems are single-coil (problem %d has batch %d)", i, p.nb);
     REQUIRE(p.d.u_c64 && p.d.u_c128 && p.d.tw_c128[0], "problem %d lacks c64/c128 factors", i);
     REQUIRE(!p.d.csr_c64, "batched problems use the per-axis factors");
+    REQUIRE(!p.d.fp64_ops, "problem %d needs FP64 operators (fp64_ops); batches run the "
+            "complex64 operators in FP32", i);
     if (i > 0) {
       REQUIRE(p.D == plans[0].D, "all problems must have the same ndim");
       for (int a = 0; a < p.D; ++a) {
@@ -1324,9 +1407,11 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   std::vector<std::pair<int, int2>> all;  // (count, (problem, item))
   for (int i = 0; i < n; ++i) {
     std::vector<int4> it(plans[i].d.n_items);
-    if (table) {
-      PNH_CHECK_HIP(hipMemcpy(it.data(), plans[i].d.items, it.size() * sizeof(int4),
-                              hipMemcpyDeviceToHost));
+    if (table && !it.empty()) {
+      // ordered after the caller's uploads of the work lists on `stream`
+      PNH_CHECK_HIP(hipMemcpyAsync(it.data(), plans[i].d.items, it.size() * sizeof(int4),
+                                   hipMemcpyDeviceToHost, stream));
+      PNH_CHECK_HIP(hipStreamSynchronize(stream));
     }
     for (int64_t k = 0; k < plans[i].d.n_items; ++k)
       all.push_back(std::make_pair(table ? it[k].z : 0, make_int2(i, (int)k)));
@@ -1423,12 +1508,19 @@ size_t batch_cg_layout(const pnh_batch* b, void* ws, BatchWs* w) {
   return off;
 }
 
-__global__ void copy_iters_kernel(const int* __restrict__ src, int* __restrict__ dst, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+// Iterations and final ||r|| per problem (rr stays at a problem's value
+// once its stopping test holds).
+__global__ void batch_cg_out_kernel(const int* __restrict__ iters, const double* __restrict__ rr,
+                                    int* __restrict__ iters_out, double* __restrict__ resid_out,
+                                    int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    if (iters_out) iters_out[i] = iters[i];
+    if (resid_out) resid_out[i] = sqrt(rr[i]);
+  }
 }
 
 int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double rtol,
-                double atol, void* ws, int32_t* iters_out, hipStream_t s) {
+                double atol, void* ws, int32_t* iters_out, double* resid_out, hipStream_t s) {
   BatchWs w;
   batch_cg_layout(b, ws, &w);
   const BatchCg& cg = w.cg;
@@ -1464,8 +1556,9 @@ int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double 
   if ((rc = batch_passes(b, BFFT_INV64, 1, nullptr, w.q, w.x, nullptr, nullptr, s))) return rc;
   if ((rc = pnh_batch_vec_launch(BVEC_CAST_DIV, b->probs, b->vec_t[2], ch, w.q, x, nullptr,
                                  nullptr, cg, 0, s))) return rc;
-  if (iters_out) {
-    hipLaunchKernelGGL(copy_iters_kernel, dim3(1), dim3(256), 0, s, cg.iters, iters_out, b->n);
+  if (iters_out || resid_out) {
+    hipLaunchKernelGGL(batch_cg_out_kernel, dim3(1), dim3(256), 0, s, cg.iters, cg.rr,
+                       iters_out, resid_out, b->n);
     PNH_LAUNCH_CHECK();
   }
   return 0;
@@ -1544,15 +1637,17 @@ int pnh_batch_adjoint(const pnh_batch* b, const void* y, void* x, void* ws, void
 }
 
 int pnh_batch_solve_cg(const pnh_batch* b, const void* y, void* x, int maxiter, double rtol,
-                       double atol, void* ws, int32_t* iters_out, void* stream) {
+                       double atol, void* ws, int32_t* iters_out, double* resid_out,
+                       void* stream) {
   REQUIRE(b && x && ws && (y || b->M_total == 0), "null argument");
   REQUIRE(maxiter >= 0, "maxiter must be >= 0");
-  return batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, (hipStream_t)stream);
+  return batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, resid_out,
+                     (hipStream_t)stream);
 }
 
 int pnh_batch_cg_graph_create(const pnh_batch* b, const void* y, void* x, int maxiter,
                               double rtol, double atol, void* ws, int32_t* iters_out,
-                              pnh_graph** out) {
+                              double* resid_out, pnh_graph** out) {
   REQUIRE(b && x && ws && out, "null argument");
   *out = nullptr;
   hipStream_t cs;
@@ -1563,7 +1658,7 @@ int pnh_batch_cg_graph_create(const pnh_batch* b, const void* y, void* x, int ma
     pnh_set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
     return -3;
   }
-  const int rc = batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, cs);
+  const int rc = batch_solve(b, y, x, maxiter, rtol, atol, ws, iters_out, resid_out, cs);
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);
@@ -1586,7 +1681,8 @@ int pnh_batch_cg_graph_create(const pnh_batch* b, const void* y, void* x, int ma
 }
 
 int pnh_cg_graph_create(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
-                        double atol, void* ws, int32_t* iters_out, pnh_graph** out) {
+                        double atol, void* ws, int32_t* iters_out, double* resid_out,
+                        pnh_graph** out) {
   REQUIRE(p && x && ws && out, "null argument");
   *out = nullptr;
   hipStream_t cs;
@@ -1597,7 +1693,7 @@ int pnh_cg_graph_create(const pnh_plan* p, const void* y, void* x, int maxiter, 
     pnh_set_error("hipStreamBeginCapture: %s", hipGetErrorString(e));
     return -3;
   }
-  const int rc = cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, cs, false);
+  const int rc = cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, resid_out, cs, false);
   hipGraph_t g = nullptr;
   e = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);

@@ -32,6 +32,15 @@ __global__ void c64_to_c128_kernel(const float2* __restrict__ in, double2* __res
   }
 }
 
+__global__ void c128_to_c64_kernel(const double2* __restrict__ in, float2* __restrict__ out,
+                                   int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = in[i];
+    out[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
 // x = complex64(xx) / sn (numpy complex64 division by a real-valued
 // complex64: both parts multiplied by 1/sn in single precision).
 __global__ void cast_div_sn_kernel(const double2* __restrict__ in, float2* __restrict__ out,
@@ -160,7 +169,23 @@ cg_init_kernel(const double2* __restrict__ part, int nblk, int nb,
   const int done = (bn == 0.0) || (bn < tol);
   s->done[1][c] = done;
   s->iters[c] = done ? 0 : maxiter;
+  s->stopped[c] = done;
+  s->res[c] = bn;
   if (nb == 1) s->all_done[1] = done;   // one coil: no separate AND pass
+}
+
+// Final residual norm per coil (one block per coil).
+__global__ void __launch_bounds__(256)
+cg_resid_kernel(const double2* __restrict__ part, int nblk, int nb,
+                const CgScalars* __restrict__ s, double* __restrict__ out) {
+  __shared__ double2 red[256];
+  const int c = blockIdx.x;
+  if (s->stopped[c]) {
+    if (threadIdx.x == 0) out[c] = s->res[c];
+    return;
+  }
+  const double2 tot = sum_partials(part, nblk, nb, c, red);
+  if (threadIdx.x == 0) out[c] = sqrt(tot.x);
 }
 
 // all_done[1] = AND over coils after the init phase (single thread)
@@ -206,7 +231,11 @@ cg_p_update_kernel(double2* __restrict__ p, const double2* __restrict__ r,
     if (writer) {
       s->rr[nbuf][c] = rr_new;
       s->done[nbuf][c] = dnew;
-      if (dnew && !dold) s->iters[c] = it;
+      if (dnew && !dold) {
+        s->iters[c] = it;
+        s->res[c] = sqrt(rr_new);
+        s->stopped[c] = 1;
+      }
     }
   }
   __syncthreads();
@@ -301,6 +330,14 @@ int pnh_c64_to_c128(const void* in, void* out, int64_t n, hipStream_t s) {
   return 0;
 }
 
+int pnh_c128_to_c64(const void* in, void* out, int64_t n, hipStream_t s) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(c128_to_c64_kernel, dim3(grid_for(n)), dim3(256), 0, s,
+                     (const double2*)in, (float2*)out, n);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
 int pnh_cast_div_sn(const void* in, void* out, const float* sn, int64_t n_img,
                     int nb, hipStream_t s) {
   hipLaunchKernelGGL(cast_div_sn_kernel, dim3(grid_for(n_img * nb)), dim3(256), 0, s,
@@ -378,6 +415,14 @@ int pnh_cg_xr_update(void* x, void* r, const void* p, const void* q, int64_t n_c
                      (double2*)x, (double2*)r, (const double2*)p, (const double2*)q,
                      n_cells, nb, sc, it & 1, (const double2*)pq_part, PNH_RED_BLOCKS,
                      (double2*)part);
+  PNH_LAUNCH_CHECK();
+  return 0;
+}
+
+int pnh_cg_resid(const void* rr_part, int nb, const CgScalars* sc, double* out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(cg_resid_kernel, dim3(nb), dim3(256), 0, s, (const double2*)rr_part,
+                     PNH_RED_BLOCKS, nb, sc, out);
   PNH_LAUNCH_CHECK();
   return 0;
 }

@@ -11,6 +11,10 @@ Precision: forward/adjoint/selfadjoint and the stages run on complex64 data
 with fp32 arithmetic (as the reference device path); ``k2y``/``y2k``/
 ``k2y2k`` given complex128 input run in FP64, and ``solve(..., 'cg')`` runs
 the whole CG in FP64 as the reference CPU path does (scipy on complex128).
+Plans whose apodization spans more than ``helper.FP64_SPREAD`` (max|sn| /
+min|sn|, e.g. Kd == Nd) set ``fp64_ops``: their complex64 operators and
+interpolation stages then run in FP64 inside (complex64 in and out), because
+the interpolation cancels terms up to that spread larger than its result.
 """
 import ctypes
 
@@ -57,11 +61,15 @@ class NUFFT:
         lib = _lib.load()
         if not torch.cuda.is_available():
             raise RuntimeError('pynufft_amd needs a HIP device (none visible)')
+        stream = None
         if device_indx is None:
             dev = torch.device('cuda', torch.cuda.current_device())
         elif isinstance(device_indx, tuple):
             dev = torch.device(device_indx[4]) if isinstance(device_indx[4], torch.device) \
                 else torch.device('cuda', int(device_indx[2]))
+            # the tuple's queue (helper.device_list): launch on that stream
+            if len(device_indx) > 5 and isinstance(device_indx[5], torch.cuda.Stream):
+                stream = device_indx[5]
         elif isinstance(device_indx, torch.device):
             dev = device_indx
         else:
@@ -71,6 +79,8 @@ class NUFFT:
         self._libh = lib
         self.device = dev
         self._dev_index = dev.index
+        self.thr = stream
+        self._stream_handle = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
         self.processor = 'hip'
         self.legacy = bool(legacy)
         self.dtype = np.complex64
@@ -81,6 +91,7 @@ class NUFFT:
         self.ndims = 0
         self.ft_axes = ()
         self.batch = None
+        self.fp64_ops = False
         self._plan = None
         self._ws = {}
         self._sense = None
@@ -163,6 +174,11 @@ class NUFFT:
             np.zeros(1, np.complex128)
         t['u_c64'] = torch.from_numpy(u128.astype(np.complex64)).to(dev)
         self._u128_host = u128
+        # apodization spread too large for FP32 operators (Kd close to Nd):
+        # forward/adjoint/selfadjoint and the complex64 stages run in FP64
+        self.fp64_ops = bool(st.get('fp64_ops', False))
+        if self.fp64_ops:
+            t['u_c128'] = torch.from_numpy(u128).to(dev)
         t['sn_f32'] = torch.from_numpy(st['sn'].astype(np.float32).ravel()).to(dev)
         t['sn_f64'] = torch.from_numpy(st['sn'].astype(np.float64).ravel()).to(dev)
         for a in range(dd):
@@ -265,6 +281,7 @@ class NUFFT:
         d.n_slots = int(st['items'].shape[0])
         d.max_tile_slots = int(st['tile_slots'][:, 1].max()) if st['tile_slots'].size else 0
         d.threads = self._threads
+        d.fp64_ops = 1 if getattr(self, 'fp64_ops', False) else 0
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self._libh.pnh_plan_create(ctypes.byref(d), ctypes.byref(h)))
@@ -296,6 +313,8 @@ class NUFFT:
             raise RuntimeError('call plan() first')
 
     def _stream(self):
+        if self._stream_handle is not None:
+            return self._stream_handle
         if _raw_stream is not None:
             return ctypes.c_void_p(_raw_stream(self._dev_index))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -348,10 +367,22 @@ class NUFFT:
     def _call(self, fn, *args):
         # the library launches on the current device: switch only when needed
         if torch.cuda.current_device() == self._dev_index:
-            _lib.check(fn(self._plan, *args))
+            self._launch(fn, self._plan, *args)
         else:
             with torch.cuda.device(self.device):
-                _lib.check(fn(self._plan, *args))
+                self._launch(fn, self._plan, *args)
+
+    def _launch(self, fn, *args):
+        """Call into the library.  A plan bound to the device tuple's stream
+        launches there, ordered after torch's current stream (inputs) and
+        before it (outputs, and the reuse of freed memory)."""
+        if self.thr is None:
+            _lib.check(fn(*args))
+            return
+        cur = torch.cuda.current_stream(self.device)
+        self.thr.wait_stream(cur)
+        _lib.check(fn(*args))
+        cur.wait_stream(self.thr)
 
     # --------------------------------------------------------- stages --
     def x2xx(self, x):
@@ -394,45 +425,48 @@ class NUFFT:
         """Reference ``_k2xx_one2one_cpu`` (``:331-344``)."""
         return self.k2xx(k)
 
+    def _interp_input(self, a, kind):
+        """Device input of an interpolation stage and whether it runs in FP64:
+        complex128 input, or any input on a plan with ``fp64_ops`` (then
+        upcast here and the result rounded back to complex64)."""
+        t, is_np = self._in(a, kind, allow128=True)
+        fp64 = t.dtype == _c128 or self.fp64_ops
+        out_dtype = t.dtype
+        if fp64:
+            self._ensure_fp64()
+            t = t.to(_c128)
+        return t, is_np, fp64, out_dtype
+
     def k2y(self, k):
         """y = P k (reference ``_k2y_cpu``, ``:282-288``); complex128 in -> FP64."""
         self._need_plan()
-        kt, is_np = self._in(k, 'k', allow128=True)
-        fp64 = kt.dtype == _c128
-        if fp64:
-            self._ensure_fp64()
+        kt, is_np, fp64, odt = self._interp_input(k, 'k')
         out = self._empty('y', kt.dtype)
         self._call(self._libh.pnh_k2y, _lib.C128 if fp64 else _lib.C64,
                    _ptr(kt), _ptr(out), self._stream())
-        return self._out(out, is_np)
+        return self._out(out.to(odt), is_np)
 
     def y2k(self, y):
         """k = P^H y (reference ``_y2k_cpu``, ``:308-314``); complex128 in -> FP64."""
         self._need_plan()
-        yt, is_np = self._in(y, 'y', allow128=True)
-        fp64 = yt.dtype == _c128
-        if fp64:
-            self._ensure_fp64()
+        yt, is_np, fp64, odt = self._interp_input(y, 'y')
         dt = _lib.C128 if fp64 else _lib.C64
         out = self._empty('k', yt.dtype)
         ws = self._workspace(_lib.OP_Y2K, dt)
         self._call(self._libh.pnh_y2k, dt, _ptr(yt), _ptr(out), _ptr(ws),
                    self._stream())
-        return self._out(out, is_np)
+        return self._out(out.to(odt), is_np)
 
     def k2y2k(self, k):
         """P^H P k (reference ``_k2y2k_cpu``, ``:353-362``); complex128 in -> FP64."""
         self._need_plan()
-        kt, is_np = self._in(k, 'k', allow128=True)
-        fp64 = kt.dtype == _c128
-        if fp64:
-            self._ensure_fp64()
+        kt, is_np, fp64, odt = self._interp_input(k, 'k')
         dt = _lib.C128 if fp64 else _lib.C64
         out = self._empty('k', kt.dtype)
         ws = self._workspace(_lib.OP_K2Y2K, dt)
         self._call(self._libh.pnh_k2y2k, dt, _ptr(kt), _ptr(out), _ptr(ws),
                    self._stream())
-        return self._out(out, is_np)
+        return self._out(out.to(odt), is_np)
 
     def forward_grad_om(self, k, g):
         """d(loss)/d(om) for y = k2y(k): Re(conj(g) dy/dom) summed over coils.
@@ -550,9 +584,12 @@ class NUFFT:
         out = self._empty('x')
         ws = self._workspace(_lib.OP_SOLVE_CG, _lib.C128)
         iters = torch.zeros(self._nb, dtype=torch.int32, device=self.device)
+        resid = torch.zeros(self._nb, dtype=torch.float64, device=self.device)
         self._call(self._libh.pnh_solve_cg, _ptr(yt), _ptr(out), int(maxiter),
-                   float(rtol), float(atol), _ptr(ws), _ptr(iters), self._stream())
+                   float(rtol), float(atol), _ptr(ws), _ptr(iters), _ptr(resid),
+                   self._stream())
         self._cg_iters_dev = iters
+        self._cg_resid_dev = resid
         return self._out(out, is_np)
 
     def solve_graph(self, y, maxiter=30, rtol=1e-5, atol=0.0):
@@ -569,21 +606,23 @@ class NUFFT:
             yb = self._empty('y')
             xb = self._empty('x')
             it = torch.zeros(self._nb, dtype=torch.int32, device=self.device)
+            res = torch.zeros(self._nb, dtype=torch.float64, device=self.device)
             ws = torch.empty_like(self._workspace(_lib.OP_SOLVE_CG, _lib.C128))
             h = ctypes.c_void_p()
             torch.cuda.synchronize(self.device)
             with torch.cuda.device(self.device):
                 _lib.check(self._libh.pnh_cg_graph_create(
                     self._plan, _ptr(yb), _ptr(xb), key[0], key[1], key[2], _ptr(ws),
-                    _ptr(it), ctypes.byref(h)))
-            g = (h, yb, xb, it, ws)
+                    _ptr(it), _ptr(res), ctypes.byref(h)))
+            g = (h, yb, xb, it, res, ws)
             self._graphs[key] = g
-        h, yb, xb, it, ws = g
+        h, yb, xb, it, res, ws = g
         yt, is_np = self._in(y, 'y')
         yb.copy_(yt)
         with torch.cuda.device(self.device):
-            _lib.check(self._libh.pnh_graph_launch(h, self._stream()))
+            self._launch(self._libh.pnh_graph_launch, h, self._stream())
         self._cg_iters_dev = it
+        self._cg_resid_dev = res
         return self._out(xb.clone(), is_np)
 
     def _solve_dc(self, y, maxiter):
@@ -767,6 +806,14 @@ class NUFFT:
     def cg_iters(self):
         it = self._cg_iters_dev.cpu().numpy()
         return int(it[0]) if self.batch is None else it
+
+    @property
+    def cg_resid(self):
+        """||r|| of the k-space normal equations at the end of the last CG
+        solve (per coil with ``batch``): the value of the stopping test when
+        it held, else the residual after the last iteration."""
+        r = self._cg_resid_dev.cpu().numpy()
+        return float(r[0]) if self.batch is None else r
 
     # ------------------------------------------------------ multi-coil --
     def set_sense(self, coil_profile):

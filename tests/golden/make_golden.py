@@ -202,6 +202,65 @@ def make_uniform3d(ref):
          **{'rows64_' + k: v for k, v in rec.items()})
 
 
+def make_uniform3d_adjoint(ref):
+    """3-D 128^3 / 256^3 adjoint of 4096 nonzero samples on the full grid.
+
+    The adjoint is linear and each CSR row depends only on its own sample,
+    so a plan of the selected samples gives the full plan's adjoint of a y
+    that is zero elsewhere.  The 128^3 image is kept as 16384 voxels, its
+    norm and its inner products with four seeded Gaussian images."""
+    w = W.uniform3d()
+    om, Nd, Kd, Jd = w['om'], w['Nd'], w['Kd'], w['Jd']
+    M = om.shape[0]
+    rng = np.random.default_rng(3031)
+    sel = np.sort(rng.choice(M, 4096, replace=False))
+    yv = (rng.standard_normal(4096) + 1j * rng.standard_normal(4096)).astype(np.complex64)
+    A = ref.NUFFT()
+    A.plan(om[sel], Nd, Kd, Jd)
+    adj = A.adjoint(yv)
+    xsel = np.sort(rng.choice(int(np.prod(Nd)), 16384, replace=False))
+    proj = np.array([np.vdot(W.gaussian_image(Nd, 3100 + i).astype(np.complex128).ravel(),
+                             adj.astype(np.complex128).ravel()) for i in range(4)])
+    save('uniform3d_adjoint', om_fp=np.array(W.fingerprint(om)), sel=sel, y=yv,
+         xsel=xsel, adjoint_sel=adj.ravel()[xsel], adjoint_norm=np.linalg.norm(adj),
+         adjoint_proj=proj, proj_seeds=np.arange(3100, 3104))
+
+
+# Plans without oversampling on some axis: the apodization spans a large
+# range and the interpolation cancels large terms (name: Nd, Kd, Jd, M, seed)
+NOOVERSAMPLE = {
+    'nos3d': ((18, 14, 7), (18, 14, 7), (6, 8, 7), 1313, 122),
+    'nos2d': ((32, 32), (32, 32), (6, 6), 3000, 123),
+    'k125': ((32, 32), (40, 40), (6, 6), 3000, 124),
+}
+
+
+def make_nooversample(ref):
+    """forward/adjoint/selfadjoint/y2k/CG of the reference on plans with
+    Kd == Nd (and one at Kd = 1.25 Nd), two images each (the two coils of a
+    batch=2 plan are checked against them one by one)."""
+    out = {}
+    for name, (Nd, Kd, Jd, M, seed) in NOOVERSAMPLE.items():
+        om = W.uniform_om(M, len(Nd), seed)
+        A = ref.NUFFT()
+        A.plan(om, Nd, Kd, Jd)
+        out[name + '__om'] = om
+        out[name + '__Nd'], out[name + '__Kd'], out[name + '__Jd'] = \
+            np.array(Nd), np.array(Kd), np.array(Jd)
+        for c in range(2):
+            x = W.gaussian_image(Nd, seed * 10 + c)
+            y = A.forward(x)
+            yin = y.astype(np.complex64)
+            p = '%s__%d_' % (name, c)
+            out[p + 'x'] = x
+            out[p + 'forward'] = y
+            out[p + 'adjoint'] = A.adjoint(yin)
+            out[p + 'selfadjoint'] = A.selfadjoint(x)
+            out[p + 'y2k'] = A.y2k(yin)
+            out[p + 'solve_cg10'] = A.solve(yin, 'cg', maxiter=10)
+    save('nooversample', **out)
+
+
 def make_small3d(ref):
     """3-D 32^3 / 64^3, M = 20000: full operator outputs (subset + norms)."""
     Nd, Kd, Jd = (32, 32, 32), (64, 64, 64), (6, 6, 6)
@@ -252,7 +311,8 @@ def make_solvers(ref):
 
 CASES = {'tiny': make_tiny, 'grid_j1': make_grid_j1, 'plain2d': make_plain2d,
          'radial2d': make_radial2d, 'small3d': make_small3d,
-         'uniform3d': make_uniform3d, 'solvers': make_solvers}
+         'uniform3d': make_uniform3d, 'uniform3d_adjoint': make_uniform3d_adjoint,
+         'nooversample': make_nooversample, 'solvers': make_solvers}
 
 
 def main():

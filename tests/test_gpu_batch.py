@@ -4,6 +4,7 @@ eager batched solve on every replay."""
 import numpy as np
 import pytest
 
+import oracle
 from pynufft_amd import workloads as W
 from conftest import rel
 
@@ -104,3 +105,62 @@ def test_autotuned_batch_matches_plain(hip, tmp_path, monkeypatch):
     for i, op in enumerate(B.ops):
         xp = op.solve(ys[i], 'cg', maxiter=8, rtol=0.0)
         assert float(torch.linalg.vector_norm(xb[i] - xp) / torch.linalg.vector_norm(xp)) < 1e-5
+
+
+def _pick(ps):
+    """One 192^2 / 384^2 radial problem, one spiral, one 128^2 and one 256^2
+    (distinct problems, first in order)."""
+    want = [lambda p: p['kind'] == 'radial' and p['Nd'][0] == 192,
+            lambda p: p['kind'] == 'spiral',
+            lambda p: p['Nd'][0] == 128,
+            lambda p: p['Nd'][0] == 256]
+    out = []
+    for f in want:
+        out.append(next(i for i, p in enumerate(ps) if f(p) and i not in out))
+    return out
+
+
+@pytest.mark.slow
+def test_batched2d_full_workload_vs_oracle(hip, tmp_path, monkeypatch):
+    """The batched benchmark workload as benchmarked: all 64 problems of
+    workloads.batched2d() planned with the autotuner, a graph-captured
+    20-iteration CG (rtol 0).  Four problems (a 384^2 radix-3 grid, a
+    spiral, a 128^2 and a 256^2 problem) match the oracle's forward, adjoint
+    and solve(..., 'cg', maxiter=20) (reference linalg/solve_cpu.py:256-288)
+    at 1e-5; all 64 match the per-problem NUFFT.solve."""
+    import torch
+    from pynufft_amd.batch import BatchNUFFT
+    from pynufft_amd.nufft import NUFFT
+    monkeypatch.setenv('PYNUFFT_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    ps = W.batched2d()
+    B = BatchNUFFT()
+    B.plan(ps, tune=True)
+    xs = [torch.from_numpy(p['x']).to(B.device) for p in ps]
+    y = B.forward(xs)
+    ys = B.unpack_samples(y)
+    xa = B.unpack_images(B.adjoint(y))
+    xg = B.unpack_images(B.solve_graph(y, maxiter=20, rtol=0.0))
+    assert bool((B.cg_iters == 20).all())
+    errs = {}
+    for i in _pick(ps):
+        p = ps[i]
+        O = oracle.OracleNUFFT()
+        O.plan(p['om'], p['Nd'], p['Kd'], p['Jd'])
+        yo = O.forward(p['x'])
+        yi = ys[i].cpu().numpy()
+        errs['forward%d' % i] = rel(yi, yo)
+        errs['adjoint%d' % i] = rel(xa[i].cpu().numpy(), O.adjoint(yi))
+        errs['cg%d' % i] = rel(xg[i].cpu().numpy(), O.solve(yi, 'cg', maxiter=20, rtol=0.0))
+    worst = max(errs, key=errs.get)
+    print('\nbatched vs oracle (problems %s): worst %s %.3e' % (_pick(ps), worst, errs[worst]))
+    assert errs[worst] < 1e-5, errs
+    diffs = []
+    for i, p in enumerate(ps):
+        op = NUFFT()
+        op.plan(p['om'], p['Nd'], p['Kd'], p['Jd'])
+        xp = op.solve(ys[i], 'cg', maxiter=20, rtol=0.0)
+        diffs.append(float(torch.linalg.vector_norm(xg[i] - xp) / torch.linalg.vector_norm(xp)))
+        op.release()
+    print('batched vs per-problem NUFFT.solve: max %.3e (problem %d)'
+          % (max(diffs), int(np.argmax(diffs))))
+    assert max(diffs) < 1e-5

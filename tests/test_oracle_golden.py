@@ -164,3 +164,43 @@ def test_laplacian_spectrum_closed_form():
         k = np.meshgrid(*[np.arange(K) for K in Kd], indexing='ij')
         ref = sum(2 * np.cos(2 * np.pi * k[d] / Kd[d]) - 2 for d in range(len(Kd)))
         assert np.abs(lap - ref).max() < 1e-5
+
+
+NOOVERSAMPLE = ['nos3d', 'nos2d', 'k125']
+
+
+@pytest.mark.parametrize('name', NOOVERSAMPLE)
+def test_nooversample_matches_reference(name):
+    """Plans with Kd == Nd (and Kd = 1.25 Nd): the oracle reproduces the
+    reference's operators there too (tests/golden/nooversample.npz)."""
+    g = golden('nooversample')
+    Nd, Kd, Jd = (tuple(int(v) for v in g[name + '__' + k]) for k in ('Nd', 'Kd', 'Jd'))
+    A = oracle.OracleNUFFT()
+    A.plan(g[name + '__om'], Nd, Kd, Jd)
+    for c in range(2):
+        p = '%s__%d_' % (name, c)
+        x = g[p + 'x']
+        yin = g[p + 'forward'].astype(np.complex64)
+        assert np.array_equal(A.forward(x), g[p + 'forward'])
+        assert np.array_equal(A.adjoint(yin), g[p + 'adjoint'])
+        assert np.array_equal(A.selfadjoint(x), g[p + 'selfadjoint'])
+        assert np.array_equal(A.y2k(yin), g[p + 'y2k'])
+        assert rel(A.solve(yin, 'cg', maxiter=10), g[p + 'solve_cg10']) < 1e-12
+
+
+def test_uniform3d_adjoint_matches_reference():
+    """The 128^3 / 256^3 adjoint of 4096 nonzero samples on the full grid:
+    the oracle's subset plan reproduces the reference's voxels, norm and
+    projections (tests/golden/uniform3d_adjoint.npz)."""
+    g = golden('uniform3d_adjoint')
+    w = W.uniform3d()
+    assert np.array_equal(np.array(W.fingerprint(w['om'])), g['om_fp'])
+    A = oracle.OracleNUFFT()
+    A.plan(w['om'][g['sel']], w['Nd'], w['Kd'], w['Jd'])
+    adj = A.adjoint(g['y'])
+    assert np.array_equal(adj.ravel()[g['xsel']], g['adjoint_sel'])
+    assert abs(np.linalg.norm(adj) / float(g['adjoint_norm']) - 1) < 1e-12
+    for i, seed in enumerate(g['proj_seeds']):
+        v = W.gaussian_image(w['Nd'], int(seed)).astype(np.complex128).ravel()
+        p = np.vdot(v, adj.astype(np.complex128).ravel())
+        assert abs(p - g['adjoint_proj'][i]) < 1e-12 * abs(g['adjoint_proj'][i])
