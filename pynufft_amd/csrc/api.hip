@@ -1495,73 +1495,56 @@ size_t batch_cg_layout(const pnh_batch* b, void* ws, BatchWs* w) {
   w->q = take(g2);
   w->yt = take((size_t)b->M_total * 16);
   w->part = take((size_t)b->part_elems * 16);
-  w->cg.rr = (double*)take(n * 8);
-  w->cg.rr_prev = (double*)take(n * 8);
   w->cg.tol = (double*)take(n * 8);
-  w->cg.pq = (double2*)take(n * 16);
-  w->cg.done = (int*)take(n * 4);
+  w->cg.rr = (double*)take(2 * n * 8);
+  w->cg.done = (int*)take(2 * n * 4);
   w->cg.iters = (int*)take(n * 4);
-  w->cg.all_done = (int*)take(4);
-  const int64_t nblk = std::max(b->vec_t[0].n_blocks,
-                                std::max(b->vec_t[1].n_blocks, b->vec_t[2].n_blocks));
-  w->cg.part = (double2*)take((size_t)nblk * 16);
+  w->cg.stopped = (int*)take(n * 4);
+  w->cg.res = (double*)take(n * 8);
+  w->cg.part_rr = (double2*)take((size_t)b->vec_t[0].n_blocks * 16);
+  w->cg.part_pq = (double2*)take((size_t)b->vec_t[0].n_blocks * 16);
+  w->cg.n = b->n;
   return off;
-}
-
-// Iterations and final ||r|| per problem (rr stays at a problem's value
-// once its stopping test holds).
-__global__ void batch_cg_out_kernel(const int* __restrict__ iters, const double* __restrict__ rr,
-                                    int* __restrict__ iters_out, double* __restrict__ resid_out,
-                                    int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    if (iters_out) iters_out[i] = iters[i];
-    if (resid_out) resid_out[i] = sqrt(rr[i]);
-  }
 }
 
 int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double rtol,
                 double atol, void* ws, int32_t* iters_out, double* resid_out, hipStream_t s) {
   BatchWs w;
   batch_cg_layout(b, ws, &w);
-  const BatchCg& cg = w.cg;
+  BatchCg cg = w.cg;
+  cg.maxiter = maxiter;
+  cg.rtol = rtol;
+  cg.atol = atol;
   const int64_t ch = b->vec_chunk;
+  const BatchTable& gt = b->vec_t[0];
   int rc;
-  // b = P^H y in FP64 (complex64 samples upcast exactly)
+  // b = P^H y in FP64 (complex64 samples upcast exactly) into r, x = 0, |b|^2
   if ((rc = pnh_batch_vec_launch(BVEC_CAST_Y, b->probs, b->vec_t[1], ch, (void*)y, w.yt,
                                  nullptr, nullptr, cg, 0, s))) return rc;
   if ((rc = batch_y2k(b, 1, w.yt, w.r, w.part, nullptr, nullptr, s))) return rc;
-  if ((rc = pnh_batch_vec_launch(BVEC_ZERO_X, b->probs, b->vec_t[0], ch, w.x, nullptr, nullptr,
+  if ((rc = pnh_batch_vec_launch(BVEC_ZERO_X, b->probs, gt, ch, w.x, nullptr, nullptr,
                                  nullptr, cg, 0, s))) return rc;
-  if ((rc = pnh_batch_vec_launch(BVEC_DOT_RR, b->probs, b->vec_t[0], ch, w.r, nullptr, nullptr,
+  if ((rc = pnh_batch_vec_launch(BVEC_DOT_RR, b->probs, gt, ch, w.r, nullptr, nullptr,
                                  nullptr, cg, 0, s))) return rc;
-  if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_INIT, 0, maxiter, rtol,
-                                      atol, s))) return rc;
   for (int it = 0; it < maxiter; ++it) {
-    if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, b->vec_t[0], ch, w.p, w.r, nullptr,
-                                   nullptr, cg, it == 0, s))) return rc;
+    // the p update finishes the previous iteration's |r|^2 and stopping test
+    const int* done = cg.done + (it & 1) * b->n;
+    if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, gt, ch, w.p, w.r, nullptr,
+                                   nullptr, cg, it, s))) return rc;
     if ((rc = pnh_batch_k2y2k_launch(b->probs, b->items, b->n_items, b->gather_t, b->D, b->J,
                                      1, b->pad_cells_max, b->grid_threads, w.p, w.yt, w.q,
-                                     w.part, cg.done, cg.all_done, s))) return rc;
-    if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, b->vec_t[0], ch, w.p, w.q, nullptr,
-                                   nullptr, cg, 0, s))) return rc;
-    if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_PQ, it, maxiter, rtol,
-                                        atol, s))) return rc;
-    if ((rc = pnh_batch_vec_launch(BVEC_XR_UPDATE, b->probs, b->vec_t[0], ch, w.x, w.r, w.p,
-                                   w.q, cg, 0, s))) return rc;
-    if ((rc = pnh_batch_finalize_launch(b->vec_t[0], b->n, cg, CG_PHASE_RR, it, maxiter, rtol,
-                                        atol, s))) return rc;
+                                     w.part, done, nullptr, s))) return rc;
+    if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, gt, ch, w.p, w.q, nullptr,
+                                   nullptr, cg, it, s))) return rc;
+    if ((rc = pnh_batch_vec_launch(BVEC_XR_UPDATE, b->probs, gt, ch, w.x, w.r, w.p,
+                                   w.q, cg, it, s))) return rc;
   }
   // image = crop(ifftn(k)) in FP64 into q (packed image layout), then
   // x = complex64(image) / sn
   if ((rc = batch_passes(b, BFFT_INV64, 1, nullptr, w.q, w.x, nullptr, nullptr, s))) return rc;
   if ((rc = pnh_batch_vec_launch(BVEC_CAST_DIV, b->probs, b->vec_t[2], ch, w.q, x, nullptr,
                                  nullptr, cg, 0, s))) return rc;
-  if (iters_out || resid_out) {
-    hipLaunchKernelGGL(batch_cg_out_kernel, dim3(1), dim3(256), 0, s, cg.iters, cg.rr,
-                       iters_out, resid_out, b->n);
-    PNH_LAUNCH_CHECK();
-  }
-  return 0;
+  return pnh_batch_cg_out_launch(gt, cg, iters_out, resid_out, s);
 }
 
 }  // namespace

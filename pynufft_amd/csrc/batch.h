@@ -29,16 +29,25 @@ struct BatchTable {
   int64_t n_blocks;
 };
 
-// Device-resident CG state of a batch (arrays of n_problems entries).
+// Device-resident CG state of a batch (arrays of n_problems entries).  rr
+// and done have two parity buffers ([2][n]): iteration `it` reads buffer
+// it&1, which its p update writes from buffer (it+1)&1, so no workgroup
+// reads what another writes in the same launch.  stopped / res / iters are
+// written once per problem, when its stopping test first holds.  The block
+// partials are per block of the grid table; every workgroup of a problem
+// sums its problem's partials itself, in one fixed order.
 struct BatchCg {
-  double* rr;
-  double* rr_prev;
   double* tol;
-  double2* pq;
-  int* done;
+  double* rr;                // [2][n]
+  int* done;                 // [2][n]
   int* iters;
-  int* all_done;
-  double2* part;             // per-block partial sums
+  int* stopped;
+  double* res;
+  double2* part_rr;          // |r|^2 (|b|^2 before the first iteration)
+  double2* part_pq;          // p . q
+  int n;
+  int maxiter;
+  double rtol, atol;
 };
 
 // Interpolates every problem's items (tile ownership, padded boxes in LDS).
@@ -61,12 +70,15 @@ int pnh_batch_fft_launch(const BatchProblem* probs, const BatchTable& t, int axi
                          int kind, int fp64, int pow2, size_t lds, int threads,
                          const void* in, int in_image, void* out, int out_image,
                          const int* done, const int* all_done, hipStream_t s);
+// One element-wise / CG step over every problem's packed vector (op
+// BVEC_*); `it` is the CG iteration of the CG steps.
 int pnh_batch_vec_launch(int op, const BatchProblem* probs, const BatchTable& t,
                          int64_t chunk, void* a, void* b, void* c, void* d,
-                         const BatchCg& cg, int first_iter, hipStream_t s);
-int pnh_batch_finalize_launch(const BatchTable& t, int n_problems, const BatchCg& cg,
-                              int phase, int iter, int maxiter, double rtol, double atol,
-                              hipStream_t s);
+                         const BatchCg& cg, int it, hipStream_t s);
+// Iterations run and final ||r|| per problem (either output may be null);
+// t is the grid table whose block partials the last CG step wrote.
+int pnh_batch_cg_out_launch(const BatchTable& t, const BatchCg& cg, int32_t* iters_out,
+                            double* resid_out, hipStream_t s);
 
 enum { BFFT_FWD = 0, BFFT_INV = 1, BFFT_INV64 = 2 };
 enum { BVEC_CAST_Y = 0, BVEC_ZERO_X = 1, BVEC_DOT_RR = 2, BVEC_P_UPDATE = 3,

@@ -195,22 +195,42 @@ k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
 constexpr int kMaxItemRows = 4096;
 constexpr int kFixBits = 50;
 
-// nearest integer to v * scale for |v * scale| < 2^51: adding 1.5 * 2^52
-// leaves that integer in the low mantissa bits of the double
-__device__ __forceinline__ long long to_fixed(double v, double scale) {
-  return __double_as_longlong(__builtin_fma(v, scale, 6755399441055744.0)) -
-         0x4338000000000000LL;
-}
+// Adding 1.5 * 2^52 to a double of magnitude below 2^51 rounds it to the
+// nearest integer and leaves that integer in the low mantissa bits; the low
+// word of the constant's bits is zero, so subtracting them is one 32-bit
+// subtract of the high word.
+constexpr double kMagic = 6755399441055744.0;
+constexpr long long kMagicBits = 0x4338000000000000LL;
 
 __device__ __forceinline__ void lds_add(long long* p, long long v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// real part at cell[0], imaginary part one plane further
-template <typename C>
-__device__ __forceinline__ void lds_add_c(long long* cell, int plane, C v, double scale) {
-  lds_add(cell, to_fixed((double)v.x, scale));
-  lds_add(cell + plane, to_fixed((double)v.y, scale));
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+
+// a * b in FP64 (fused multiply-adds)
+__device__ __forceinline__ double2 dmul(double2 a, double2 b) {
+  return make_double2(__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x));
+}
+
+// Fixed-point contribution u * a, with a already scaled by 2^(50 - e): each
+// part is two fused multiply-adds onto 1.5 * 2^52 (the inner one rounds its
+// product to an integer, the outer adds the other product and rounds again,
+// so the result is within one unit of the exact product), real part at
+// cell[0], imaginary part at cell[1] (one instruction offset apart).
+// |u * a| < 2^50 by the choice of e.
+__device__ __forceinline__ void lds_add_prod(long long* cell, double2 u, double2 a) {
+  const double re = __builtin_fma(u.x, a.x, __builtin_fma(-u.y, a.y, kMagic));
+  const double im = __builtin_fma(u.x, a.y, __builtin_fma(u.y, a.x, kMagic));
+  lds_add(cell, __double_as_longlong(re) - kMagicBits);
+  lds_add(cell + 1, __double_as_longlong(im) - kMagicBits);
+}
+
+// Fixed-point contribution of an already scaled value v.
+__device__ __forceinline__ void lds_add_fixed(long long* cell, double2 v) {
+  lds_add(cell, __double_as_longlong(v.x + kMagic) - kMagicBits);
+  lds_add(cell + 1, __double_as_longlong(v.y + kMagic) - kMagicBits);
 }
 
 // Complex product with contraction allowed (fused multiply-adds).
@@ -280,7 +300,7 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 }
 
 // One work item (tile id, first row, rows, slot).  LDS: the accumulator as
-// int64, a real plane [pad_cells][NBC] then an imaginary plane, then
+// int64, [pad_cells][NBC][real, imaginary], then
 // reduction scratch.  JN bounds the
 // unrolled support loops of the leading axes; JE > 0 states that they are
 // exactly JE long (no per-offset guards).
@@ -342,7 +362,7 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   typedef typename cplx<T>::t C;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int pc = (int)b.pad_cells;
-  const int plane = pc * NBC;            // real plane, then imaginary plane
+  const int plane = pc * NBC;            // (cell, coil) pairs of real, imaginary
   double* red = reinterpret_cast<double*>(acc + (int64_t)plane * 2);
   const C* __restrict__ y = (const C*)y_;
   const int nb = a.nb;
@@ -389,6 +409,16 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     // this lane's last-axis accumulator index
     const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
                               f.wrap[D - 1], a.K[D - 1]);
+    // The row's samples times the item's scale, in FP64 (exact: a float
+    // times a power of two); every product below is formed in FP64 and
+    // rounded once, onto the fixed-point grid.
+    double2 ys[NBC];
+#pragma unroll
+    for (int c = 0; c < NBC; ++c) {
+      ys[c] = to_d2(yv[c]);
+      ys[c].x *= scale;
+      ys[c].y *= scale;
+    }
     if (CSRV) {
       const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
       const C* __restrict__ val = (const C*)a.csr_val;
@@ -403,44 +433,46 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
         if (D == 3)
           idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
                   wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
-        const C w = cconj(val[i0 + g * gs]);
-        long long* cell = acc + idx * NBC;
+        const double2 w = to_d2(cconj(val[i0 + g * gs]));
+        long long* cell = acc + 2 * idx * NBC;
 #pragma unroll
-        for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
+        for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, w, ys[c]);
         if (D == 3 && ++j1 == J1) { j1 = 0; ++j0; }
       }
       continue;
     }
-    const C ul = cconj(u[a.u_off[D - 1] + (int64_t)r * JL + jl]);
-    if (D == 1) {
-      long long* cell = acc + xl * NBC;
+    // last-axis factor times the scaled samples
+    const double2 ul = to_d2(cconj(u[a.u_off[D - 1] + (int64_t)r * JL + jl]));
+    double2 wl[NBC];
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(ul, yv[c]), scale);
+    for (int c = 0; c < NBC; ++c) wl[c] = dmul(ul, ys[c]);
+    if (D == 1) {
+      long long* cell = acc + 2 * xl * NBC;
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) lds_add_fixed(cell + 2 * c, wl[c]);
       continue;
     }
-    // NBC == 1: fold the sample value into the last-axis factor
-    const C wl = NBC == 1 ? cmulf(ul, yv[0]) : ul;
     const int x0b = a.base[r] - f.T0[0];
     const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
-    C u0c[JN];
+    // accumulator offsets in bytes (a cell holds NBC (real, imaginary) pairs)
+    constexpr int CB = 16 * NBC;
+    char* const accb = reinterpret_cast<char*>(acc);
+    double2 u0c[JN];
     int row0[JN];
 #pragma unroll
     for (int j = 0; j < JN; ++j)
       if (JE > 0 || j < J0) {
-        u0c[j] = cconj(u0[j]);
-        row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+        u0c[j] = to_d2(cconj(u0[j]));
+        row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2 * CB;
       }
     if (D == 2) {
+      const int xlb = xl * CB;
 #pragma unroll
       for (int j0 = 0; j0 < JN; ++j0) {
         if (JE > 0 || j0 < J0) {
-          const C w = cmulf(u0c[j0], wl);
-          long long* cell = acc + (row0[j0] + xl) * NBC;
-          if (NBC == 1) lds_add_c(cell, plane, w, scale);
-          else {
+          long long* cell = reinterpret_cast<long long*>(accb + row0[j0] + xlb);
 #pragma unroll
-            for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
-          }
+          for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, u0c[j0], wl[c]);
         }
       }
       continue;
@@ -448,28 +480,26 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     // D == 3: factors of axes 0 and 1 in registers, both loops unrolled
     const int x1b = a.base[a.M + r] - f.T0[1];
     const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
-    C u1c[JN];
+    double2 u1c[JN];
     int off1[JN];
 #pragma unroll
     for (int j = 0; j < JN; ++j)
       if (JE > 0 || j < J1) {
-        u1c[j] = cconj(u1[j]);
-        off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+        u1c[j] = to_d2(cconj(u1[j]));
+        off1[j] = (wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl) * CB;
       }
 #pragma unroll
     for (int j0 = 0; j0 < JN; ++j0) {
       if (JE > 0 || j0 < J0) {
-        const C a0 = cmulf(u0c[j0], wl);
+        double2 a0[NBC];
+#pragma unroll
+        for (int c = 0; c < NBC; ++c) a0[c] = dmul(u0c[j0], wl[c]);
 #pragma unroll
         for (int j1 = 0; j1 < JN; ++j1) {
           if (JE > 0 || j1 < J1) {
-            const C w = cmulf(u1c[j1], a0);
-            long long* cell = acc + (row0[j0] + off1[j1]) * NBC;
-            if (NBC == 1) lds_add_c(cell, plane, w, scale);
-            else {
+            long long* cell = reinterpret_cast<long long*>(accb + row0[j0] + off1[j1]);
 #pragma unroll
-              for (int c = 0; c < NBC; ++c) lds_add_c(cell + c, plane, cmulf(w, yv[c]), scale);
-            }
+            for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, u1c[j1], a0[c]);
           }
         }
       }
@@ -482,7 +512,7 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   for (int i = tid; i < plane; i += nthr) {
     C v;
     if (poison) v = mk<T>(nan, nan);
-    else v = mk<T>((T)((double)acc[i] * unscale), (T)((double)acc[plane + i] * unscale));
+    else v = mk<T>((T)((double)acc[2 * i] * unscale), (T)((double)acc[2 * i + 1] * unscale));
     part[i] = v;
   }
 }

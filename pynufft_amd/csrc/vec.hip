@@ -461,151 +461,192 @@ __device__ __forceinline__ void block_sum2(double sx, double sy, double2* out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Sum of problem p's block partials (grid table blocks first[p] ..
+// first[p+1]) in one fixed order and tree; every thread gets the total, and
+// every workgroup of the problem the same bits.
+__device__ __forceinline__ double2 problem_sum(const double2* __restrict__ part,
+                                               const BatchTable& t, int p, double2* red) {
+  const int64_t b0 = t.first[p], b1 = t.first[p + 1];
+  double sx = 0.0, sy = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const double2 v = part[i];
+    sx += v.x; sy += v.y;
+  }
+  red[threadIdx.x] = make_double2(sx, sy);
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      red[threadIdx.x].x += red[threadIdx.x + st].x;
+      red[threadIdx.x].y += red[threadIdx.x + st].y;
+    }
+    __syncthreads();
+  }
+  const double2 tot = red[0];
+  __syncthreads();
+  return tot;
+}
+
+// blockDim.x == 256.  CG steps read the state of iteration `it` (see
+// BatchCg); the p update of iteration it also finishes the previous
+// iteration's |r|^2 (or, at it == 0, |b|^2: tolerance and first state),
+// applies the stopping test and writes buffer it&1 from its first block.
 __global__ void __launch_bounds__(256)
 batch_vec_kernel(int op, const BatchProblem* __restrict__ probs, BatchTable t,
-                 int64_t chunk, void* a_, void* b_, void* c_, void* d_, BatchCg cg,
-                 int first_iter) {
+                 int64_t chunk, void* a_, void* b_, void* c_, void* d_, BatchCg cg, int it) {
+  __shared__ double2 red[256];
   const int p = t.tab[blockIdx.x];
   const int64_t lb = (int64_t)blockIdx.x - t.first[p];
   const BatchProblem& pr = probs[p];
   int64_t off, lo, hi;
   seg_range(pr, op, lb, chunk, off, lo, hi);
-  const bool reduce = op == BVEC_DOT_RR || op == BVEC_DOT_PQ || op == BVEC_XR_UPDATE;
-  const bool skip = (op == BVEC_P_UPDATE || op == BVEC_DOT_PQ || op == BVEC_XR_UPDATE) &&
-                    (*cg.all_done || cg.done[p]);
+  const int cur = it & 1, prev = (it + 1) & 1;
+  const bool writer = lb == 0 && threadIdx.x == 0;
   double sx = 0.0, sy = 0.0;
-  if (!skip) {
+  if (op == BVEC_P_UPDATE) {
+    double beta = 0.0;
+    if (it == 0) {
+      const double bb = problem_sum(cg.part_rr, t, p, red).x;
+      const double bn = sqrt(bb);
+      const double tol = fmax(cg.atol, cg.rtol * bn);
+      const int dn = (bn == 0.0) || (bn < tol);
+      if (writer) {
+        cg.tol[p] = tol;
+        cg.rr[p] = bb;
+        cg.done[p] = dn;
+        cg.iters[p] = dn ? 0 : cg.maxiter;
+        cg.stopped[p] = dn;
+        cg.res[p] = bn;
+      }
+      if (dn) return;
+    } else {
+      const double rr_old = cg.rr[prev * cg.n + p];
+      if (cg.done[prev * cg.n + p]) {
+        if (writer) {
+          cg.rr[cur * cg.n + p] = rr_old;
+          cg.done[cur * cg.n + p] = 1;
+        }
+        return;
+      }
+      const double rr_new = problem_sum(cg.part_rr, t, p, red).x;
+      const int dn = sqrt(rr_new) < cg.tol[p];
+      if (writer) {
+        cg.rr[cur * cg.n + p] = rr_new;
+        cg.done[cur * cg.n + p] = dn;
+        if (dn) {
+          cg.iters[p] = it;
+          cg.stopped[p] = 1;
+          cg.res[p] = sqrt(rr_new);
+        }
+      }
+      if (dn) return;
+      beta = rr_new / rr_old;
+    }
+    double2* pv = (double2*)a_;
+    const double2* rv = (const double2*)b_;
     for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
       const int64_t e = off + i;
-      switch (op) {
-        case BVEC_CAST_Y: {
-          const float2 v = ((const float2*)a_)[e];
-          ((double2*)b_)[e] = make_double2(v.x, v.y);
-          break;
-        }
-        case BVEC_ZERO_X:
-          ((double2*)a_)[e] = make_double2(0.0, 0.0);
-          break;
-        case BVEC_DOT_RR: {
-          const double2 v = ((const double2*)a_)[e];
-          sx += v.x * v.x + v.y * v.y;
-          break;
-        }
-        case BVEC_P_UPDATE: {
-          double2* pv = (double2*)a_;
-          const double2 rv = ((const double2*)b_)[e];
-          if (first_iter) { pv[e] = rv; break; }
-          const double beta = cg.rr[p] / cg.rr_prev[p];
-          double2 q = pv[e];
-          q.x = rv.x + beta * q.x;
-          q.y = rv.y + beta * q.y;
-          pv[e] = q;
-          break;
-        }
-        case BVEC_DOT_PQ: {
-          const double2 u = ((const double2*)a_)[e];
-          const double2 v = ((const double2*)b_)[e];
-          sx += u.x * v.x + u.y * v.y;
-          sy += u.x * v.y - u.y * v.x;
-          break;
-        }
-        case BVEC_XR_UPDATE: {
-          const double2 pq = cg.pq[p];
-          const double rho = cg.rr[p];
-          const double den = pq.x * pq.x + pq.y * pq.y;
-          const double ax = rho * pq.x / den, ay = -rho * pq.y / den;
-          double2* x = (double2*)a_;
-          double2* r = (double2*)b_;
-          const double2 pv = ((const double2*)c_)[e], qv = ((const double2*)d_)[e];
-          double2 xv = x[e], rv = r[e];
-          xv.x += ax * pv.x - ay * pv.y;
-          xv.y += ax * pv.y + ay * pv.x;
-          rv.x -= ax * qv.x - ay * qv.y;
-          rv.y -= ax * qv.y + ay * qv.x;
-          x[e] = xv;
-          r[e] = rv;
-          sx += rv.x * rv.x + rv.y * rv.y;
-          break;
-        }
-        case BVEC_CAST_DIV: {
-          const double2 v = ((const double2*)a_)[e];
-          const float inv = 1.0f / pr.sn32[i];
-          ((float2*)b_)[e] = make_float2((float)v.x * inv, (float)v.y * inv);
-          break;
-        }
+      const double2 r = rv[e];
+      if (it == 0) { pv[e] = r; continue; }
+      double2 q = pv[e];
+      q.x = r.x + beta * q.x;
+      q.y = r.y + beta * q.y;
+      pv[e] = q;
+    }
+    return;
+  }
+  if ((op == BVEC_DOT_PQ || op == BVEC_XR_UPDATE) && cg.done[cur * cg.n + p]) return;
+  double ax = 0.0, ay = 0.0;
+  if (op == BVEC_XR_UPDATE) {
+    const double2 pq = problem_sum(cg.part_pq, t, p, red);
+    const double rho = cg.rr[cur * cg.n + p];
+    const double den = pq.x * pq.x + pq.y * pq.y;
+    ax = rho * pq.x / den;
+    ay = -rho * pq.y / den;
+  }
+  for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    const int64_t e = off + i;
+    switch (op) {
+      case BVEC_CAST_Y: {
+        const float2 v = ((const float2*)a_)[e];
+        ((double2*)b_)[e] = make_double2(v.x, v.y);
+        break;
+      }
+      case BVEC_ZERO_X:
+        ((double2*)a_)[e] = make_double2(0.0, 0.0);
+        break;
+      case BVEC_DOT_RR: {
+        const double2 v = ((const double2*)a_)[e];
+        sx += v.x * v.x + v.y * v.y;
+        break;
+      }
+      case BVEC_DOT_PQ: {
+        const double2 u = ((const double2*)a_)[e];
+        const double2 v = ((const double2*)b_)[e];
+        sx += u.x * v.x + u.y * v.y;
+        sy += u.x * v.y - u.y * v.x;
+        break;
+      }
+      case BVEC_XR_UPDATE: {
+        double2* x = (double2*)a_;
+        double2* r = (double2*)b_;
+        const double2 pv = ((const double2*)c_)[e], qv = ((const double2*)d_)[e];
+        double2 xv = x[e], rv = r[e];
+        xv.x += ax * pv.x - ay * pv.y;
+        xv.y += ax * pv.y + ay * pv.x;
+        rv.x -= ax * qv.x - ay * qv.y;
+        rv.y -= ax * qv.y + ay * qv.x;
+        x[e] = xv;
+        r[e] = rv;
+        sx += rv.x * rv.x + rv.y * rv.y;
+        break;
+      }
+      case BVEC_CAST_DIV: {
+        const double2 v = ((const double2*)a_)[e];
+        const float inv = 1.0f / pr.sn32[i];
+        ((float2*)b_)[e] = make_float2((float)v.x * inv, (float)v.y * inv);
+        break;
       }
     }
   }
-  if (reduce) block_sum2(sx, sy, &cg.part[blockIdx.x]);
+  if (op == BVEC_DOT_RR || op == BVEC_XR_UPDATE)
+    block_sum2(sx, sy, &cg.part_rr[blockIdx.x]);
+  else if (op == BVEC_DOT_PQ)
+    block_sum2(sx, sy, &cg.part_pq[blockIdx.x]);
 }
 
-// One block per problem: sum its block partials in order and apply the CG
-// bookkeeping (same semantics as the single-problem finalize).
+// One block per problem: iterations and final ||r|| (the stopping test's
+// value, else the norm from the last |r|^2 partials; ||b|| at maxiter 0).
 __global__ void __launch_bounds__(256)
-batch_finalize_kernel(BatchTable t, BatchCg cg, int phase, int iter, int maxiter,
-                      double rtol, double atol) {
+batch_cg_out_kernel(BatchTable t, BatchCg cg, int* __restrict__ iters_out,
+                    double* __restrict__ resid_out) {
+  __shared__ double2 red[256];
   const int p = blockIdx.x;
-  if (phase != CG_PHASE_INIT && (*cg.all_done || cg.done[p])) return;
-  const int64_t b0 = t.first[p], b1 = t.first[p + 1];
-  double sx = 0.0, sy = 0.0;
-  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-    sx += cg.part[i].x;
-    sy += cg.part[i].y;
-  }
-  __shared__ double2 tot;
-  block_sum2(sx, sy, &tot);
-  __syncthreads();
+  const bool stopped = cg.maxiter > 0 && cg.stopped[p];
+  double res = 0.0;
+  if (stopped) res = cg.res[p];
+  else res = sqrt(problem_sum(cg.part_rr, t, p, red).x);
   if (threadIdx.x != 0) return;
-  if (phase == CG_PHASE_INIT) {
-    const double bn = sqrt(tot.x);
-    const double tol = fmax(atol, rtol * bn);
-    cg.tol[p] = tol;
-    cg.rr[p] = tot.x;
-    cg.rr_prev[p] = tot.x;
-    const int dn = (bn == 0.0) || (bn < tol);
-    cg.done[p] = dn;
-    cg.iters[p] = dn ? 0 : maxiter;
-  } else if (phase == CG_PHASE_PQ) {
-    cg.pq[p] = tot;
-  } else {
-    cg.rr_prev[p] = cg.rr[p];
-    cg.rr[p] = tot.x;
-    if (sqrt(tot.x) < cg.tol[p]) {
-      cg.done[p] = 1;
-      cg.iters[p] = iter + 1;
-    }
-  }
-}
-
-__global__ void batch_all_done_kernel(BatchCg cg, int n) {
-  __shared__ int any_open;
-  if (threadIdx.x == 0) any_open = 0;
-  __syncthreads();
-  for (int p = threadIdx.x; p < n; p += blockDim.x)
-    if (!cg.done[p]) any_open = 1;
-  __syncthreads();
-  if (threadIdx.x == 0) *cg.all_done = any_open ? 0 : 1;
+  if (iters_out) iters_out[p] = cg.maxiter > 0 ? cg.iters[p] : 0;
+  if (resid_out) resid_out[p] = res;
 }
 
 }  // namespace
 
 int pnh_batch_vec_launch(int op, const BatchProblem* probs, const BatchTable& t,
                          int64_t chunk, void* a, void* b, void* c, void* d,
-                         const BatchCg& cg, int first_iter, hipStream_t s) {
+                         const BatchCg& cg, int it, hipStream_t s) {
   if (t.n_blocks == 0) return 0;
   hipLaunchKernelGGL(batch_vec_kernel, dim3((unsigned)t.n_blocks), dim3(256), 0, s, op, probs,
-                     t, chunk, a, b, c, d, cg, first_iter);
+                     t, chunk, a, b, c, d, cg, it);
   PNH_LAUNCH_CHECK();
   return 0;
 }
 
-int pnh_batch_finalize_launch(const BatchTable& t, int n_problems, const BatchCg& cg,
-                              int phase, int iter, int maxiter, double rtol, double atol,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(batch_finalize_kernel, dim3(n_problems), dim3(256), 0, s, t, cg, phase,
-                     iter, maxiter, rtol, atol);
-  PNH_LAUNCH_CHECK();
-  hipLaunchKernelGGL(batch_all_done_kernel, dim3(1), dim3(256), 0, s, cg, n_problems);
+int pnh_batch_cg_out_launch(const BatchTable& t, const BatchCg& cg, int32_t* iters_out,
+                            double* resid_out, hipStream_t s) {
+  if (!iters_out && !resid_out) return 0;
+  hipLaunchKernelGGL(batch_cg_out_kernel, dim3((unsigned)cg.n), dim3(256), 0, s, t, cg,
+                     iters_out, resid_out);
   PNH_LAUNCH_CHECK();
   return 0;
 }
