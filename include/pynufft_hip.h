@@ -114,6 +114,12 @@ typedef struct pnh_plan_desc {
    * complex128 CSR products (nufft/_nufft_class_methods_cpu.py:269-314) and
    * complex128 ifftn (:316-329) keep.  0: FP32 arithmetic. */
   int32_t fp64_ops;
+  /* [n_items] per-item bounds written by pnh_item_bounds for this work list
+   * (may be NULL).  With them the complex64 gridding keeps one 64-bit
+   * fixed-point word per cell (real and imaginary parts packed) instead of
+   * two: half the LDS atomics and accumulator.  No counterpart in the
+   * reference (an implementation choice of the gridding). */
+  const double* item_bound;
 } pnh_plan_desc;
 
 typedef struct pnh_plan pnh_plan;
@@ -133,6 +139,13 @@ int pnh_plan_create(const pnh_plan_desc* desc, pnh_plan** out);
 int pnh_plan_destroy(pnh_plan* plan);
 /* Bytes of `ws` needed by operator `op` in precision `dtype`. */
 int pnh_workspace_bytes(const pnh_plan* plan, int op, int dtype, size_t* bytes);
+/* bound[i] (device double[n_items]) = for work item i, the largest over the
+ * cells of its padded tile of the sum over its rows of the product of the
+ * per-axis complex64 factor magnitudes, rounded up.  A constant of the
+ * trajectory and work list: computed once after planning and passed back in
+ * pnh_plan_desc.item_bound of a new plan.  No counterpart in the reference
+ * (it sizes the fixed-point scale of the packed gridding accumulators). */
+int pnh_item_bounds(const pnh_plan* plan, double* bound, void* stream);
 
 /* xx = x * sn.  Reference nufft/_nufft_class_methods_cpu.py:224-230 (_x2xx_cpu). */
 int pnh_x2xx(const pnh_plan* plan, const void* x, void* xx, void* stream);
@@ -204,10 +217,14 @@ int pnh_solve_dc(const pnh_plan* plan, const void* y, void* x, int maxiter, void
                  void* stream);
 /* solve(y, 'L1TVOLS', maxiter, rho): L1 total-variation regularised least
  * squares by split Bregman with mu = 1, lambda = rho (complex64, batch 1).
+ * lap: device complex64 [prod Kd], the Laplacian spectrum of the reference's
+ * create_laplacian_kernel (the complex64 fftn of the periodic stencil; the
+ * solve is sensitive to its rounding), or NULL for the closed form
+ * sum_d 2 cos(2 pi k_d / K_d) - 2.
  * Reference linalg/solve_cpu.py:42-163 (L1TVOLS), src/_helper/helper.py:11-70
  * (Laplacian kernel, periodic difference maps). */
 int pnh_solve_l1tvols(const pnh_plan* plan, const void* y, void* x, int maxiter, double rho,
-                      void* ws, void* stream);
+                      const void* lap, void* ws, void* stream);
 /* solve(y, 'lsqr', ...): scipy.sparse.linalg.lsqr on A = P (k2y, adjoint
  * y2k) with k-space unknowns, x0 = 0, in FP64 on the device (batch 1), then
  * x = crop(ifftn(k)) / sn.  iter_lim <= 0 means scipy's default 2 prod(Kd).

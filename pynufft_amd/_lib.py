@@ -53,6 +53,7 @@ class PlanDesc(ctypes.Structure):
         ('max_tile_slots', ctypes.c_int32),
         ('threads', ctypes.c_int32),
         ('fp64_ops', ctypes.c_int32),
+        ('item_bound', _vp),
     ]
 
 
@@ -70,6 +71,7 @@ EXPORTS = {
     'pnh_plan_destroy': (ctypes.c_int, [_vp]),
     'pnh_workspace_bytes': (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_size_t)]),
+    'pnh_item_bounds': (ctypes.c_int, [_vp, _vp, _vp]),
     'pnh_x2xx': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     'pnh_xx2x': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     'pnh_xx2k': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -87,7 +89,7 @@ EXPORTS = {
     'pnh_selfadjoint2': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'pnh_solve_dc': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, _vp]),
     'pnh_solve_l1tvols': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
-                                         _vp, _vp]),
+                                         _vp, _vp, _vp]),
     'pnh_solve_lsqr': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       _vp, _vp, _vp]),

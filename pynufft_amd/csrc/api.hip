@@ -308,13 +308,21 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   p->bins.n_items = d.n_items;
   p->bins.items = d.items;
   p->bins.tile_slots = d.tile_slots;
+  p->bins.item_bound = d.item_bound;
   if (d.threads == 0) {
-    const int64_t bytes = pad_cells * 16;
+    // gridding: by the complex64 accumulator, one 64-bit word per cell when
+    // packed (two 512-thread workgroups share a CU at 16^3), else two;
+    // interpolation: one workgroup per CU on large boxes (3-D 16^3 k2y
+    // measured 0.20 ms at 1024 threads, 0.26 ms at 512)
+    const int64_t bytes = pad_cells * (d.item_bound ? 8 : 16);
     p->bins.threads = bytes > 81920 ? 1024 : (bytes > 40960 ? 512 : 256);
+    const int64_t kbytes = pad_cells * 16;
+    p->bins.k2y_threads = kbytes > 81920 ? 1024 : (kbytes > 40960 ? 512 : 256);
   } else {
     REQUIRE(d.threads >= 64 && d.threads <= 1024 && d.threads % 64 == 0,
             "threads must be a multiple of 64 in 64..1024 (got %d)", d.threads);
     p->bins.threads = d.threads;
+    p->bins.k2y_threads = d.threads;
   }
   p->slot_nbc = p->nb >= 4 ? 4 : (p->nb >= 2 ? 2 : 1);
   REQUIRE(d.fp64_ops == 0 || d.fp64_ops == 1, "fp64_ops must be 0 or 1");
@@ -456,7 +464,8 @@ static size_t l1tv_ws_bytes(const pnh_plan* p) {
   const size_t adj = adjoint_ws_bytes(p);
   sa = sa > dens ? sa : dens;
   sa = sa > adj ? sa : adj;
-  return 2 * align_up((size_t)p->Ktot * 4) + (4 + 2 * (size_t)p->D) * img + sa;
+  return align_up((size_t)p->Ktot * 8) + align_up((size_t)p->Ktot * 4) +
+         (4 + 2 * (size_t)p->D) * img + sa;
 }
 
 // Scratch of the 'dc' solve: W, E and the weighted samples (complex128
@@ -527,6 +536,12 @@ int pnh_workspace_bytes(const pnh_plan* p, int op, int dtype, size_t* bytes) {
     default: pnh_set_error("unknown op %d", op); return -1;
   }
   return 0;
+}
+
+int pnh_item_bounds(const pnh_plan* p, double* bound, void* stream) {
+  REQUIRE(p && (bound || p->bins.n_items == 0), "null argument");
+  REQUIRE(p->d.u_c64, "item bounds need the complex64 factors (u_c64)");
+  return pnh_item_bounds_launch(interp_args(p, 0), p->bins, bound, (hipStream_t)stream);
 }
 
 int pnh_x2xx(const pnh_plan* p, const void* x, void* xx, void* stream) {
@@ -700,7 +715,7 @@ int pnh_solve_dc(const pnh_plan* p, const void* y, void* x, int maxiter, void* w
 }
 
 int pnh_solve_l1tvols(const pnh_plan* p, const void* y, void* x, int maxiter, double rho,
-                      void* ws, void* stream) {
+                      const void* lap, void* ws, void* stream) {
   REQUIRE(p && x && ws && (y || p->M == 0), "null argument");
   REQUIRE(maxiter >= 0, "maxiter must be >= 0");
   REQUIRE(p->nb == 1, "L1TVOLS solves one coil (plan without batch)");
@@ -711,7 +726,7 @@ int pnh_solve_l1tvols(const pnh_plan* p, const void* y, void* x, int maxiter, do
   const int64_t N = p->Ntot, K = p->Ktot;
   const size_t img = align_up((size_t)N * 8);
   char* w = (char*)ws;
-  float* uker = (float*)w; w += align_up((size_t)K * 4);
+  void* uker = w; w += align_up((size_t)K * 8);
   float* wabs = (float*)w; w += align_up((size_t)K * 4);
   void* AHy = w; w += img;
   void* AHyk = w; w += img;
@@ -730,7 +745,7 @@ int pnh_solve_l1tvols(const pnh_plan* p, const void* y, void* x, int maxiter, do
   gk.n = K;
   gn.n = N;
   if ((rc = density(p, wabs, sws, s))) return rc;
-  if ((rc = pnh_l1tv_uker(wabs, uker, gk, mu, lam, s))) return rc;
+  if ((rc = pnh_l1tv_uker(wabs, lap, uker, gk, mu, lam, s))) return rc;
   if ((rc = pnh_adjoint(p, y, AHy, sws, s))) return rc;
   if ((rc = pnh_fill_c64(AHyk, N, 0.f, 0.f, s))) return rc;
   if ((rc = pnh_fill_c64(dd, p->D * N, 0.f, 0.f, s))) return rc;
@@ -1244,6 +1259,7 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 
 }  // extern "C"
 
+
 // ================================================================ batches ==
 #include <algorithm>
 #include <vector>
@@ -1252,6 +1268,7 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 
 struct pnh_batch {
   int n, D, J_last, J_max, grid_threads;
+  int packed;                          // every problem has per-item bounds
   int J[PNH_MAXD];
   int64_t N_total, M_total, K_total, part_elems, pad_cells_max;
   int fft_threads;
@@ -1333,9 +1350,11 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   for (int a = 0; a < D; ++a) b->J_max = std::max(b->J_max, plans[0].J[a]);
   b->pad_cells_max = 1;
   b->grid_threads = 64;
+  b->packed = 1;
   for (int i = 0; i < n; ++i) {
     b->pad_cells_max = std::max(b->pad_cells_max, plans[i].bins.pad_cells);
     b->grid_threads = std::max(b->grid_threads, plans[i].bins.threads);
+    if (!plans[i].bins.item_bound && plans[i].bins.n_items > 0) b->packed = 0;
   }
   b->fft_threads = 256;
   b->vec_chunk = kVecChunk;
@@ -1470,8 +1489,8 @@ int batch_passes(const pnh_batch* b, int kind, int fp64, const void* image_in,
 int batch_y2k(const pnh_batch* b, int fp64, const void* y, void* k, void* part,
               const int* done, const int* all_done, hipStream_t s) {
   return pnh_batch_y2k_launch(b->probs, b->items, b->n_items, b->gather_t,
-                              b->D, b->J, fp64, b->pad_cells_max, b->grid_threads, y, k,
-                              part, done, all_done, s);
+                              b->D, b->J, fp64, b->packed, b->pad_cells_max, b->grid_threads,
+                              y, k, part, done, all_done, s);
 }
 
 struct BatchWs {

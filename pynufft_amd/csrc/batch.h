@@ -62,8 +62,9 @@ int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t
                            const BatchTable& gather, int D, const int* Jd, int fp64,
                            int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
                            void* part, const int* done, const int* all_done, hipStream_t s);
+// packed: every problem has per-item bounds (complex64 packed accumulators).
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const BatchTable& gather, int D, const int* Jd, int fp64,
+                         const BatchTable& gather, int D, const int* Jd, int fp64, int packed,
                          int64_t pad_cells, int threads, const void* y, void* k, void* part,
                          const int* done, const int* all_done, hipStream_t s);
 int pnh_batch_fft_launch(const BatchProblem* probs, const BatchTable& t, int axis,

@@ -30,10 +30,16 @@ struct BinArgs {
   int64_t n_tiles;             // prod(tgrid)
   double u_bound;              // bound of |coefficient| over all rows and offsets
   int threads;                 // workgroup size of the gridding kernel
+  int k2y_threads;             // workgroup size of the interpolation (k2y) kernel
   int max_tile_slots;          // largest slot count of any tile
   int64_t n_items;
   const int* items;            // [n_items][4] tile id, first row, rows, slot
   const int* tile_slots;       // [n_tiles][2] first slot, slot count
+  // [n_items] or NULL: per work item, a bound of every cell's sum of
+  // |coefficient| over the item's rows (complex64 factors).  With it the
+  // complex64 gridding packs each cell's real and imaginary fixed-point
+  // parts into one 64-bit accumulator (see y2k_item).
+  const double* item_bound;
 };
 
 // Workgroups per tile in the gridding gather pass: one, or one per 256
@@ -80,7 +86,12 @@ int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
 // then y2k.  `part` as for pnh_y2k_launch.  grid may alias k.
 int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* k, void* y,
                      void* grid, void* part, const int* skip, hipStream_t s);
+// bound[i] = max over the cells of item i's padded box of the sum of
+// |u_0| |u_1| ... |u_last| (complex64 factors) over the item's rows, rounded
+// up; one workgroup per item.  a: complex64 factors.
+int pnh_item_bounds_launch(const InterpArgs& a, const BinArgs& b, double* bound, hipStream_t s);
 // LDS bytes available to one workgroup on the current device.
 size_t pnh_lds_limit();
-// Coils gridded per pass for this tile and LDS limit (0 if none fits).
-int pnh_y2k_coils_per_pass(const BinArgs& b, int nb);
+// Coils gridded per pass for this tile and LDS limit (0 if none fits);
+// packed: one accumulator word per cell and coil instead of two.
+int pnh_y2k_coils_per_pass(const BinArgs& b, int nb, bool packed);

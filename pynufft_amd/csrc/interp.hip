@@ -186,6 +186,17 @@ k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
 // complex64 resolution.  An item has at most 4096 rows, so a cell sums at
 // most 2^12 terms below 2^50 and cannot overflow.
 //
+// Packed accumulators (complex64 operators of plans with per-item bounds,
+// BinArgs::item_bound).  When 2^e bounds max(|Re y| + |Im y|) over the
+// item times the item's largest per-cell sum of |coefficient| (a plan-time
+// constant of the trajectory), every cell's real and imaginary sums stay
+// below 2^30 in units of 2^(e - 30).  Each contribution is then one 64-bit
+// integer re + im * 2^32 (two's complement, modulo 2^64), added with one
+// ds_add_u64: the low 32 bits of the total are the real sum exactly and the
+// rest, after removing it, the imaginary sum.  Half the atomics and half the
+// LDS of two separate accumulators (two 16^3 workgroups per CU in 3-D); the
+// rounding, 2^-30 of that bound, stays below complex64 resolution.
+//
 // The finished accumulator is written, scaled back, to the item's partial
 // slot.  The gather pass then forms every grid cell as the sum of the slots
 // that cover it -- its own tile's and those of the up to 2^D - 1 lower
@@ -194,6 +205,7 @@ k2y_dom_kernel(InterpArgs a, const void* __restrict__ grid_,
 
 constexpr int kMaxItemRows = 4096;
 constexpr int kFixBits = 50;
+constexpr int kPackBits = 30;
 
 // Adding 1.5 * 2^52 to a double of magnitude below 2^51 rounds it to the
 // nearest integer and leaves that integer in the low mantissa bits; the low
@@ -231,6 +243,33 @@ __device__ __forceinline__ void lds_add_prod(long long* cell, double2 u, double2
 __device__ __forceinline__ void lds_add_fixed(long long* cell, double2 v) {
   lds_add(cell, __double_as_longlong(v.x + kMagic) - kMagicBits);
   lds_add(cell + 1, __double_as_longlong(v.y + kMagic) - kMagicBits);
+}
+
+// Packed accumulator word of rounded parts re, im (each already offset by
+// 1.5 * 2^52, magnitude below 2^31): re + im * 2^32 modulo 2^64.  re's bits
+// minus the constant's are re as a 64-bit integer (sign included); im
+// enters through its low word only.
+__device__ __forceinline__ long long pack_fixed(double re, double im) {
+  const long long rb = __double_as_longlong(re) - kMagicBits;
+  const unsigned long long ib = (unsigned)__double_as_longlong(im);
+  return (long long)((unsigned long long)rb + (ib << 32));
+}
+
+// Packed contribution u * a (a scaled by 2^(30 - e)), rounding as lds_add_prod.
+__device__ __forceinline__ void lds_add_prod_packed(long long* cell, double2 u, double2 a) {
+  const double re = __builtin_fma(u.x, a.x, __builtin_fma(-u.y, a.y, kMagic));
+  const double im = __builtin_fma(u.x, a.y, __builtin_fma(u.y, a.x, kMagic));
+  lds_add(cell, pack_fixed(re, im));
+}
+
+__device__ __forceinline__ void lds_add_fixed_packed(long long* cell, double2 v) {
+  lds_add(cell, pack_fixed(v.x + kMagic, v.y + kMagic));
+}
+
+// Real and imaginary sums of a packed accumulator word.
+__device__ __forceinline__ void unpack_fixed(long long v, long long& re, long long& im) {
+  re = (long long)(int)(unsigned)(unsigned long long)v;
+  im = (long long)((unsigned long long)v - (unsigned long long)re) >> 32;
 }
 
 // Complex product with contraction allowed (fused multiply-adds).
@@ -305,16 +344,28 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 // unrolled support loops of the leading axes; JE > 0 states that they are
 // exactly JE long (no per-offset guards).
 // Fixed-point exponent of one work item: e with 2^e > (|Re y| + |Im y|) *
-// u_bound over the item's rows and coils c0 .. c0+NBC-1, or kPoisonExp when
-// one of those samples is not finite.  Block-wide (every thread calls it);
+// bound over the item's rows and coils c0 .. c0+NBC-1, or kPoisonExp when
+// one of those samples is not finite.  bound: u_bound (two accumulators per
+// cell) or the item's bound (packed).  Block-wide (every thread calls it);
 // red holds one double per wave.
 constexpr int kPoisonExp = 0x7fffffff;
 constexpr int kExponentPassItems = 2048;   // items from which the exponents are computed ahead
 
+// Whether the complex64 gridding of these work items packs its accumulators.
+template <typename T>
+__host__ __device__ inline bool packed_items(const BinArgs& b) {
+  return sizeof(T) == 4 && b.item_bound != nullptr;
+}
+
+template <typename T>
+__device__ __forceinline__ double exponent_bound(const BinArgs& b, int item) {
+  return packed_items<T>(b) ? b.item_bound[item] : b.u_bound;
+}
+
 template <typename T, int NBC>
 __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs& b,
                                              int r_begin, int r_end, const void* __restrict__ y_,
-                                             int c0, double* red) {
+                                             int c0, double bound, double* red) {
   typedef typename cplx<T>::t C;
   const C* __restrict__ y = (const C*)y_;
   double ymax = 0.0;
@@ -332,7 +383,7 @@ __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs&
   if (bad) ymax = __builtin_inf();
   ymax = block_max(ymax, red);
   if (!(ymax <= 1.0e300)) return kPoisonExp;
-  const double B = ymax * b.u_bound;
+  const double B = ymax * bound;
   int e = 0;
   if (B > 0.0) (void)frexp(B, &e);       // B < 2^e
   return e < -960 ? -960 : e;
@@ -347,30 +398,35 @@ y2k_exponent_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_, int c0
   if (skip && *skip) return;
   __shared__ double red[16];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  const int e = item_exponent<T, NBC>(a, b, it.y, it.y + it.z, y_, c0, red);
+  const int e = item_exponent<T, NBC>(a, b, it.y, it.y + it.z, y_, c0,
+                                      exponent_bound<T>(b, blockIdx.x), red);
   if (threadIdx.x == 0) item_e[blockIdx.x] = e;
 }
 
 // e_pre: the item's exponent computed ahead (y2k_exponent_kernel), or null
-// to compute it here.
-template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
+// to compute it here.  item: the item's index in the plan's work list.  PK:
+// packed accumulators (one 64-bit word per cell and coil, else two).
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool PK>
 __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
-                                         long long* __restrict__ acc,
+                                         int item, long long* __restrict__ acc,
                                          const void* __restrict__ y_,
                                          void* __restrict__ part_, int c0,
                                          const int* __restrict__ e_pre) {
   typedef typename cplx<T>::t C;
+  constexpr int CW = PK ? 1 : 2;         // accumulator words per (cell, coil)
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int pc = (int)b.pad_cells;
-  const int plane = pc * NBC;            // (cell, coil) pairs of real, imaginary
-  double* red = reinterpret_cast<double*>(acc + (int64_t)plane * 2);
+  const int plane = pc * NBC;            // (cell, coil) pairs
+  double* red = reinterpret_cast<double*>(acc + (int64_t)plane * CW);
   const C* __restrict__ y = (const C*)y_;
   const int nb = a.nb;
   const int r_begin = it.y, r_end = it.y + it.z;
 
-  {
+  if (PK) {
+    for (int i = tid; i < plane; i += nthr) acc[i] = 0;
+  } else {
     longlong2* acc2 = reinterpret_cast<longlong2*>(acc);
-    for (int i = tid; i < pc * NBC; i += nthr) acc2[i] = make_longlong2(0, 0);
+    for (int i = tid; i < plane; i += nthr) acc2[i] = make_longlong2(0, 0);
   }
   // scale: 2^e bounds (|Re y| + |Im y|) * |coef| for every row of the item
   int e;
@@ -378,13 +434,15 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     e = *e_pre;
     __syncthreads();                      // zeroing before the adds
   } else {
-    e = item_exponent<T, NBC>(a, b, r_begin, r_end, y_, c0, red);  // also a barrier
+    e = item_exponent<T, NBC>(a, b, r_begin, r_end, y_, c0,
+                              PK ? b.item_bound[item] : b.u_bound, red);  // also a barrier
   }
   // a non-finite sample poisons the item's whole slot
   const bool poison = e == kPoisonExp;
   if (poison) e = 0;
-  const double scale = ldexp(1.0, kFixBits - e);
-  const double unscale = ldexp(1.0, e - kFixBits);
+  constexpr int FB = PK ? kPackBits : kFixBits;
+  const double scale = ldexp(1.0, FB - e);
+  const double unscale = ldexp(1.0, e - FB);
 
   const int JL = a.J[D - 1];
   const int LPS = pnh_lanes_per_sample(JL);
@@ -434,9 +492,12 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
           idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
                   wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
         const double2 w = to_d2(cconj(val[i0 + g * gs]));
-        long long* cell = acc + 2 * idx * NBC;
+        long long* cell = acc + CW * idx * NBC;
 #pragma unroll
-        for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, w, ys[c]);
+        for (int c = 0; c < NBC; ++c) {
+          if (PK) lds_add_prod_packed(cell + c, w, ys[c]);
+          else lds_add_prod(cell + 2 * c, w, ys[c]);
+        }
         if (D == 3 && ++j1 == J1) { j1 = 0; ++j0; }
       }
       continue;
@@ -447,15 +508,18 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
 #pragma unroll
     for (int c = 0; c < NBC; ++c) wl[c] = dmul(ul, ys[c]);
     if (D == 1) {
-      long long* cell = acc + 2 * xl * NBC;
+      long long* cell = acc + CW * xl * NBC;
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) lds_add_fixed(cell + 2 * c, wl[c]);
+      for (int c = 0; c < NBC; ++c) {
+        if (PK) lds_add_fixed_packed(cell + c, wl[c]);
+        else lds_add_fixed(cell + 2 * c, wl[c]);
+      }
       continue;
     }
     const int x0b = a.base[r] - f.T0[0];
     const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
-    // accumulator offsets in bytes (a cell holds NBC (real, imaginary) pairs)
-    constexpr int CB = 16 * NBC;
+    // accumulator offsets in bytes (a cell holds NBC coils of CW words)
+    constexpr int CB = 8 * CW * NBC;
     char* const accb = reinterpret_cast<char*>(acc);
     double2 u0c[JN];
     int row0[JN];
@@ -472,7 +536,10 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
         if (JE > 0 || j0 < J0) {
           long long* cell = reinterpret_cast<long long*>(accb + row0[j0] + xlb);
 #pragma unroll
-          for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, u0c[j0], wl[c]);
+          for (int c = 0; c < NBC; ++c) {
+            if (PK) lds_add_prod_packed(cell + c, u0c[j0], wl[c]);
+            else lds_add_prod(cell + 2 * c, u0c[j0], wl[c]);
+          }
         }
       }
       continue;
@@ -499,7 +566,10 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
           if (JE > 0 || j1 < J1) {
             long long* cell = reinterpret_cast<long long*>(accb + row0[j0] + off1[j1]);
 #pragma unroll
-            for (int c = 0; c < NBC; ++c) lds_add_prod(cell + 2 * c, u1c[j1], a0[c]);
+            for (int c = 0; c < NBC; ++c) {
+              if (PK) lds_add_prod_packed(cell + c, u1c[j1], a0[c]);
+              else lds_add_prod(cell + 2 * c, u1c[j1], a0[c]);
+            }
           }
         }
       }
@@ -511,13 +581,20 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   const T nan = (T)__builtin_nan("");
   for (int i = tid; i < plane; i += nthr) {
     C v;
-    if (poison) v = mk<T>(nan, nan);
-    else v = mk<T>((T)((double)acc[2 * i] * unscale), (T)((double)acc[2 * i + 1] * unscale));
+    if (poison) {
+      v = mk<T>(nan, nan);
+    } else if (PK) {
+      long long re, im;
+      unpack_fixed(acc[i], re, im);
+      v = mk<T>((T)((double)re * unscale), (T)((double)im * unscale));
+    } else {
+      v = mk<T>((T)((double)acc[2 * i] * unscale), (T)((double)acc[2 * i + 1] * unscale));
+    }
     part[i] = v;
   }
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool PK>
 __global__ void __launch_bounds__(1024)
 y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
                 void* __restrict__ part_, int c0, const int* __restrict__ item_e,
@@ -525,8 +602,8 @@ y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
   if (skip && *skip) return;
   extern __shared__ long long smem_acc[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  y2k_item<T, D, CSRV, NBC, JN, JE>(a, b, it, smem_acc, y_, part_, c0,
-                                    item_e ? item_e + blockIdx.x : nullptr);
+  y2k_item<T, D, CSRV, NBC, JN, JE, PK>(a, b, it, blockIdx.x, smem_acc, y_, part_, c0,
+                                        item_e ? item_e + blockIdx.x : nullptr);
 }
 
 // ------------------------------------------------------------ k2y (tiles) --
@@ -866,8 +943,19 @@ y2k_gather_kernel(InterpArgs a, BinArgs b, int parts, const void* __restrict__ p
   gather_tile<T, D, NBC>(a, b, t, q * 256, span, (const C*)part_, (C*)grid_, c0);
 }
 
-size_t y2k_lds_bytes(int64_t pad_cells, int nbc) {
-  return (size_t)pad_cells * nbc * 16 + 256;  // accumulator + reduction scratch
+// accumulator (one or two 64-bit words per cell and coil) + reduction scratch
+size_t y2k_lds_bytes(int64_t pad_cells, int nbc, bool packed) {
+  return (size_t)pad_cells * nbc * (packed ? 8 : 16) + 256;
+}
+
+// Calls f(std::integral_constant<bool, PK>) with PK = packed (the packed
+// variant exists for complex64 only).
+template <typename T, typename F>
+int dispatch_pk(bool packed, F&& f) {
+  if constexpr (sizeof(T) == 4) {
+    if (packed) return f(std::integral_constant<bool, true>());
+  }
+  return f(std::integral_constant<bool, false>());
 }
 
 // Dynamic LDS above 64 KB must be allowed per kernel before the launch.
@@ -904,8 +992,8 @@ int launch_k2y_tile_j(const InterpArgs& a, const BinArgs& b, const void* grid, v
   return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
     auto kern = k2y_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value>;
     allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, grid, y,
-                       c0, skip);
+    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.k2y_threads), lds, s, a, b, grid,
+                       y, c0, skip);
     PNH_LAUNCH_CHECK();
     return 0;
   });
@@ -952,12 +1040,15 @@ int launch_tile_j(const InterpArgs& a, const BinArgs& b, const void* y, void* pa
     PNH_LAUNCH_CHECK();
   }
   return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
-    auto kern = y2k_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y, part,
-                       c0, item_e, skip);
-    PNH_LAUNCH_CHECK();
-    return 0;
+    return dispatch_pk<T>(packed_items<T>(b), [&](auto pk) {
+      auto kern = y2k_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value,
+                                  decltype(pk)::value>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y,
+                         part, c0, item_e, skip);
+      PNH_LAUNCH_CHECK();
+      return 0;
+    });
   });
 }
 
@@ -973,13 +1064,14 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
   if (part) part = (char*)part + pnh_item_exponent_bytes(b.n_items);
   int c0 = 0;
   while (c0 < a.nb) {
-    const int nbc = pnh_y2k_coils_per_pass(b, a.nb - c0);
+    const bool pk = packed_items<T>(b);
+    const int nbc = pnh_y2k_coils_per_pass(b, a.nb - c0, pk);
     if (!nbc) {
       pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)",
-                    y2k_lds_bytes(b.pad_cells, 1), pnh_lds_limit());
+                    y2k_lds_bytes(b.pad_cells, 1, pk), pnh_lds_limit());
       return -2;
     }
-    const size_t lds = y2k_lds_bytes(b.pad_cells, nbc);
+    const size_t lds = y2k_lds_bytes(b.pad_cells, nbc, pk);
     if (b.n_items > 0) {
       int rc;
       if (nbc == 4) rc = launch_tile_j<T, D, CSRV, 4>(a, b, y, part, c0, item_e, lds, skip, s);
@@ -1047,39 +1139,43 @@ namespace {
 // through the caller's sample buffer, which the barrier makes visible to the
 // whole workgroup, so the factors and samples of the second phase come from
 // L2 rather than HBM.  One coil, per-axis factors.
-template <typename T, int D, int JN, int JE>
+template <typename T, int D, int JN, int JE, bool PK>
 __device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
-                                           long long* smem, const void* __restrict__ k_,
-                                           void* __restrict__ y_, void* __restrict__ part_) {
+                                           int item, long long* smem,
+                                           const void* __restrict__ k_, void* __restrict__ y_,
+                                           void* __restrict__ part_) {
   typedef typename cplx<T>::t C;
   k2y_item<T, D, false, 1, JN, JE>(a, b, it, reinterpret_cast<C*>(smem), k_, y_, 0);
   __syncthreads();
-  y2k_item<T, D, false, 1, JN, JE>(a, b, it, smem, y_, part_, 0, nullptr);
+  y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, item, smem, y_, part_, 0, nullptr);
 }
 
-template <typename T, int D, int JN, int JE>
+template <typename T, int D, int JN, int JE, bool PK>
 __global__ void __launch_bounds__(1024)
 k2y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ k_, void* __restrict__ y_,
                   void* __restrict__ part_, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ long long smem_kk[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  k2y2k_item<T, D, JN, JE>(a, b, it, smem_kk, k_, y_, part_);
+  k2y2k_item<T, D, JN, JE, PK>(a, b, it, blockIdx.x, smem_kk, k_, y_, part_);
 }
 
 template <typename T, int D>
 int launch_k2y2k(const InterpArgs& a, const BinArgs& b, const void* k, void* y, void* grid,
                  void* part, const int* skip, hipStream_t s) {
   void* slots = (char*)part + pnh_item_exponent_bytes(b.n_items);
-  const size_t lds = y2k_lds_bytes(b.pad_cells, 1);
+  const size_t lds = y2k_lds_bytes(b.pad_cells, 1, packed_items<T>(b));
   if (b.n_items > 0) {
     int rc = dispatch_j<D, false>(a, [&](auto jn, auto je) {
-      auto kern = k2y2k_tile_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
-      allow_lds(kern, lds);
-      hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, k, y,
-                         slots, skip);
-      PNH_LAUNCH_CHECK();
-      return 0;
+      return dispatch_pk<T>(packed_items<T>(b), [&](auto pk) {
+        auto kern = k2y2k_tile_kernel<T, D, decltype(jn)::value, decltype(je)::value,
+                                      decltype(pk)::value>;
+        allow_lds(kern, lds);
+        hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, k,
+                           y, slots, skip);
+        PNH_LAUNCH_CHECK();
+        return 0;
+      });
     });
     if (rc) return rc;
   }
@@ -1095,10 +1191,11 @@ int launch_k2y2k(const InterpArgs& a, const BinArgs& b, const void* k, void* y, 
 int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* k, void* y,
                      void* grid, void* part, const int* skip, hipStream_t s) {
   const size_t esz = fp64 ? 16 : 8;
+  const bool pk = !fp64 && b.item_bound;
   const bool fused = a.nb == 1 && !a.csr_val && b.threads >= 64 && b.threads <= 1024 &&
                      b.threads % 64 == 0 &&
-                     y2k_lds_bytes(b.pad_cells, 1) <= pnh_lds_limit() &&
-                     (size_t)b.pad_cells * esz <= y2k_lds_bytes(b.pad_cells, 1);
+                     y2k_lds_bytes(b.pad_cells, 1, pk) <= pnh_lds_limit() &&
+                     (size_t)b.pad_cells * esz <= y2k_lds_bytes(b.pad_cells, 1, pk);
   if (!fused) {
     int rc = a.M > 0 ? pnh_k2y_launch(a, b, fp64, k, y, skip, s) : 0;
     return rc ? rc : pnh_y2k_launch(a, b, fp64, y, grid, part, skip, s);
@@ -1127,8 +1224,8 @@ int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void
 int pnh_k2y_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* grid, void* y,
                    const int* skip, hipStream_t s) {
   const bool csrv = a.csr_val != nullptr;
-  if (b.threads < 64 || b.threads > 1024 || b.threads % 64) {
-    pnh_set_error("workgroup size %d is not a multiple of 64 in 64..1024", b.threads);
+  if (b.k2y_threads < 64 || b.k2y_threads > 1024 || b.k2y_threads % 64) {
+    pnh_set_error("workgroup size %d is not a multiple of 64 in 64..1024", b.k2y_threads);
     return -1;
   }
 #define K2Y_CASE(T, D)                                                       \
@@ -1167,6 +1264,103 @@ int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
   return -2;
 }
 
+namespace {
+
+// Per-item bound of the packed accumulators (BinArgs::item_bound): the
+// largest over the cells of the item's padded box of the sum over the
+// item's rows of |u_0| ... |u_last| (complex64 factors, magnitudes in FP64),
+// summed in 64-bit fixed point (so the bound is reproducible), then rounded
+// up.  One workgroup per item, one thread per (row, last-axis offset).
+template <int D>
+__global__ void __launch_bounds__(256)
+item_bound_kernel(InterpArgs a, BinArgs b, double* __restrict__ bound) {
+  extern __shared__ long long smem_ib[];
+  __shared__ double red[16];
+  long long* acc = smem_ib;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int pc = (int)b.pad_cells;
+  const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
+  for (int i = tid; i < pc; i += nthr) acc[i] = 0;
+  // every product of factor magnitudes is at most u_bound < 2^eu: units of
+  // 2^(eu - 50), at most 4096 rows per cell, so no overflow
+  int eu = 0;
+  (void)frexp(b.u_bound, &eu);
+  const double scale = ldexp(1.0, kFixBits - eu);
+  __syncthreads();
+  const TileFrame f = tile_frame<D>(a, b, it.x);
+  const float2* __restrict__ u = (const float2*)a.u;
+  const int JL = a.J[D - 1];
+  const int J0 = D > 1 ? a.J[0] : 1, J1 = D > 2 ? a.J[1] : 1;
+  const int P1 = D > 1 ? b.pad[1] : 1, P2 = D > 2 ? b.pad[2] : 1;
+  const int n = it.z * JL;
+  for (int i = tid; i < n; i += nthr) {
+    const int q = i / JL;
+    const int r = it.y + q, jl = i - q * JL;
+    const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
+                              f.wrap[D - 1], a.K[D - 1]);
+    const float2 vl = u[a.u_off[D - 1] + (int64_t)r * JL + jl];
+    const double wl = hypot((double)vl.x, (double)vl.y) * scale;
+    const int x0b = D > 1 ? a.base[r] - f.T0[0] : 0;
+    const int x1b = D > 2 ? a.base[a.M + r] - f.T0[1] : 0;
+    for (int j0 = 0; j0 < J0; ++j0) {
+      double w0 = wl;
+      int c0 = xl;
+      if (D > 1) {
+        const float2 v0 = u[a.u_off[0] + (int64_t)r * J0 + j0];
+        w0 *= hypot((double)v0.x, (double)v0.y);
+        c0 += wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 * P2;
+      }
+      for (int j1 = 0; j1 < J1; ++j1) {
+        double w = w0;
+        int c = c0;
+        if (D > 2) {
+          const float2 v1 = u[a.u_off[1] + (int64_t)r * J1 + j1];
+          w *= hypot((double)v1.x, (double)v1.y);
+          c += wrap_local(x1b + j1, f.wrap[1], a.K[1]) * P2;
+        }
+        lds_add(acc + c, __double_as_longlong(w + kMagic) - kMagicBits);
+      }
+    }
+  }
+  __syncthreads();
+  double m = 0.0;
+  for (int i = tid; i < pc; i += nthr) m = fmax(m, (double)acc[i]);
+  m = block_max(m, red);
+  // a rounded term is at most half a unit below its value; margin for the
+  // FP64 products and the conversion
+  if (tid == 0)
+    bound[blockIdx.x] = (m + 0.5 * n + 1.0) * ldexp(1.0, eu - kFixBits) * (1.0 + 0x1p-20);
+}
+
+}  // namespace
+
+int pnh_item_bounds_launch(const InterpArgs& a, const BinArgs& b, double* bound, hipStream_t s) {
+  if (b.n_items == 0) return 0;
+  if (!a.u || !bound) {
+    pnh_set_error("item bounds need the complex64 factors and an output buffer");
+    return -1;
+  }
+  const size_t lds = (size_t)b.pad_cells * 8;
+  if (lds > pnh_lds_limit()) {
+    pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
+                  pnh_lds_limit());
+    return -2;
+  }
+  switch (a.D) {
+#define IB(DD)                                                                            \
+    case DD:                                                                              \
+      allow_lds(item_bound_kernel<DD>, lds);                                              \
+      hipLaunchKernelGGL(item_bound_kernel<DD>, dim3((unsigned)b.n_items), dim3(256), lds, s, \
+                         a, b, bound);                                                    \
+      PNH_LAUNCH_CHECK();                                                                 \
+      return 0;
+    IB(1) IB(2) IB(3)
+#undef IB
+  }
+  pnh_set_error("unsupported ndim %d", a.D);
+  return -2;
+}
+
 size_t pnh_lds_limit() {
   static int cached[64];
   int dev = 0;
@@ -1181,10 +1375,10 @@ size_t pnh_lds_limit() {
   return (size_t)cached[dev];
 }
 
-int pnh_y2k_coils_per_pass(const BinArgs& b, int nb) {
+int pnh_y2k_coils_per_pass(const BinArgs& b, int nb, bool packed) {
   const size_t lim = pnh_lds_limit();
   for (int nbc = nb >= 4 ? 4 : (nb >= 2 ? 2 : 1); nbc >= 1; nbc >>= 1)
-    if (y2k_lds_bytes(b.pad_cells, nbc) <= lim) return nbc;
+    if (y2k_lds_bytes(b.pad_cells, nbc, packed) <= lim) return nbc;
   return 0;
 }
 
@@ -1235,7 +1429,7 @@ k2y_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
                                (const C*)k_ + probs[p].k_off, (C*)y_ + probs[p].y_off, 0);
 }
 
-template <typename T, int D, int JN, int JE>
+template <typename T, int D, int JN, int JE, bool PK>
 __global__ void __launch_bounds__(1024)
 y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
                       const void* __restrict__ y_, void* __restrict__ part_,
@@ -1249,7 +1443,7 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
-  y2k_item<T, D, false, 1, JN, JE>(a, b, it, smem_acc, (const C*)y_ + probs[p].y_off,
+  y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, pi.y, smem_acc, (const C*)y_ + probs[p].y_off,
                                (C*)part_ + probs[p].part_off, 0, nullptr);
 }
 
@@ -1269,7 +1463,7 @@ k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __re
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
-  k2y2k_item<T, D, JN, JE>(a, b, it, smem_kkb, (const C*)k_ + probs[p].k_off,
+  k2y2k_item<T, D, JN, JE, false>(a, b, it, pi.y, smem_kkb, (const C*)k_ + probs[p].k_off,
                            (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
 }
 
@@ -1323,15 +1517,18 @@ int batch_k2y_t(const BatchProblem* probs, const int2* items, int64_t n_items, c
 
 template <typename T, int D>
 int batch_y2k_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
-                size_t lds, int threads, const void* y, void* part, const int* done,
-                const int* all_done, hipStream_t s) {
+                size_t lds, int threads, bool packed, const void* y, void* part,
+                const int* done, const int* all_done, hipStream_t s) {
   return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
-    auto kern = y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, y,
-                       part, done, all_done);
-    PNH_LAUNCH_CHECK();
-    return 0;
+    return dispatch_pk<T>(packed, [&](auto pk) {
+      auto kern = y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value,
+                                        decltype(pk)::value>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, y,
+                         part, done, all_done);
+      PNH_LAUNCH_CHECK();
+      return 0;
+    });
   });
 }
 
@@ -1375,7 +1572,7 @@ int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t
                            const BatchTable& gather, int D, const int* Jd, int fp64,
                            int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
                            void* part, const int* done, const int* all_done, hipStream_t s) {
-  const size_t lds = y2k_lds_bytes(pad_cells, 1);
+  const size_t lds = y2k_lds_bytes(pad_cells, 1, false);
   if (lds > pnh_lds_limit()) {
     pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
                   pnh_lds_limit());
@@ -1401,17 +1598,18 @@ int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t
 }
 
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
-                         const BatchTable& gather, int D, const int* Jd, int fp64,
+                         const BatchTable& gather, int D, const int* Jd, int fp64, int packed,
                          int64_t pad_cells, int threads, const void* y, void* k, void* part,
                          const int* done, const int* all_done, hipStream_t s) {
-  const size_t lds = y2k_lds_bytes(pad_cells, 1);
+  const bool pk = packed && !fp64;
+  const size_t lds = y2k_lds_bytes(pad_cells, 1, pk);
   if (lds > pnh_lds_limit()) {
     pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
                   pnh_lds_limit());
     return -2;
   }
   int rc = 0;
-#define YB(T, DD) rc = batch_y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, y, part, done, all_done, s)
+#define YB(T, DD) rc = batch_y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, pk, y, part, done, all_done, s)
   if (n_items > 0) {
     if (fp64) { if (D == 1) YB(double, 1); else if (D == 2) YB(double, 2); else YB(double, 3); }
     else { if (D == 1) YB(float, 1); else if (D == 2) YB(float, 2); else YB(float, 3); }

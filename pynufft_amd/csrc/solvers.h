@@ -26,11 +26,12 @@ int pnh_cast_mul_sn(const void* in, void* out, const float* sn, int64_t n_img, i
 int pnh_cmul_c64(const void* a, const void* b, void* out, int64_t n, hipStream_t s);
 // w[i] = |k[i * nb]| (first coil)
 int pnh_abs_first_coil(const void* k, float* w, int64_t n, int nb, hipStream_t s);
-// uker = mu w - lam * (sum_d 2 cos(2 pi k_d / K_d) - 2) on the grid sh
-int pnh_l1tv_uker(const float* w, float* uker, const ShapeArgs& sh, float mu, float lam,
-                  hipStream_t s);
-// k = (k + 1e-7) / (uker + 1e-7)
-int pnh_l1tv_kdiv(void* k, const float* uker, int64_t n, hipStream_t s);
+// uker (complex64) = mu w - lam * Lap on the grid sh; Lap = lap (complex64)
+// or, with lap null, sum_d 2 cos(2 pi k_d / K_d) - 2
+int pnh_l1tv_uker(const float* w, const void* lap, void* uker, const ShapeArgs& sh, float mu,
+                  float lam, hipStream_t s);
+// k = (k + 1e-7) / (uker + 1e-7), uker complex64
+int pnh_l1tv_kdiv(void* k, const void* uker, int64_t n, hipStream_t s);
 // rhs = mu AHyk + lam sum_p D+_p(dd_p - bb_p) (periodic forward differences)
 int pnh_l1tv_rhs(const void* AHyk, const void* dd, const void* bb, void* rhs,
                  const ShapeArgs& sh, float mu, float lam, hipStream_t s);

@@ -94,29 +94,44 @@ __global__ void abs_kernel(const float2* __restrict__ k, float* __restrict__ w, 
   }
 }
 
-// uker = mu w - lambda Lap, Lap = sum_d 2 cos(2 pi k_d / K_d) - 2 (the FFT
-// of the periodic Laplacian stencil), evaluated in FP64 and rounded once.
-__global__ void uker_kernel(const float* __restrict__ w, float* __restrict__ uker,
-                            ShapeArgs s, float mu, float lam) {
+// uker = mu w - lambda Lap in complex64, rounded step by step as numpy does
+// it for the reference's arrays (float32 w, complex64 Lap): lambda Lap, then
+// the difference.  Lap: the Laplacian spectrum passed in (the reference's
+// complex64 FFT of the periodic stencil, whose rounding the solve is
+// sensitive to), or, with lap null, its closed form
+// sum_d 2 cos(2 pi k_d / K_d) - 2 evaluated in FP64.
+__global__ void uker_kernel(const float* __restrict__ w, const float2* __restrict__ lap,
+                            float2* __restrict__ uker, ShapeArgs s, float mu, float lam) {
   GRID_STRIDE(i, s.n) {
-    int64_t rest = i;
-    double lap = 0.0;
-    for (int d = s.D - 1; d >= 0; --d) {
-      const int64_t c = rest % s.dim[d];
-      rest /= s.dim[d];
-      lap += 2.0 * cospi(2.0 * (double)c / (double)s.dim[d]) - 2.0;
+    float2 l;
+    if (lap) {
+      l = lap[i];
+    } else {
+      int64_t rest = i;
+      double acc = 0.0;
+      for (int d = s.D - 1; d >= 0; --d) {
+        const int64_t c = rest % s.dim[d];
+        rest /= s.dim[d];
+        acc += 2.0 * cospi(2.0 * (double)c / (double)s.dim[d]) - 2.0;
+      }
+      l = make_float2((float)acc, 0.0f);
     }
-    uker[i] = mu * w[i] - lam * (float)lap;
+    const float tr = __fmul_rn(lam, l.x), ti = __fmul_rn(lam, l.y);
+    uker[i] = make_float2(__fsub_rn(__fmul_rn(mu, w[i]), tr), -ti);
   }
 }
 
-// k = (k + 1e-7) / (uker + 1e-7): the k-space solve of (mu W - lambda Lap)
-__global__ void kdiv_kernel(float2* __restrict__ k, const float* __restrict__ uker,
+// k = (k + 1e-7) / (uker + 1e-7): the k-space solve of (mu W - lambda Lap);
+// the complex quotient is formed in FP64 and rounded once.
+__global__ void kdiv_kernel(float2* __restrict__ k, const float2* __restrict__ uker,
                             int64_t n) {
   GRID_STRIDE(i, n) {
-    const float d = uker[i] + 1e-7f;
+    const float2 u = uker[i];
     const float2 v = k[i];
-    k[i] = make_float2((v.x + 1e-7f) / d, v.y / d);
+    const double ar = (double)__fadd_rn(v.x, 1e-7f), ai = v.y;
+    const double br = (double)__fadd_rn(u.x, 1e-7f), bi = u.y;
+    const double den = br * br + bi * bi;
+    k[i] = make_float2((float)((ar * br + ai * bi) / den), (float)((ai * br - ar * bi) / den));
   }
 }
 
@@ -229,14 +244,14 @@ int pnh_abs_first_coil(const void* k, float* w, int64_t n, int nb, hipStream_t s
   return 0;
 }
 
-int pnh_l1tv_uker(const float* w, float* uker, const ShapeArgs& sh, float mu, float lam,
-                  hipStream_t s) {
-  LAUNCH(uker_kernel, sh.n, w, uker, sh, mu, lam);
+int pnh_l1tv_uker(const float* w, const void* lap, void* uker, const ShapeArgs& sh, float mu,
+                  float lam, hipStream_t s) {
+  LAUNCH(uker_kernel, sh.n, w, (const float2*)lap, (float2*)uker, sh, mu, lam);
   return 0;
 }
 
-int pnh_l1tv_kdiv(void* k, const float* uker, int64_t n, hipStream_t s) {
-  LAUNCH(kdiv_kernel, n, (float2*)k, uker, n);
+int pnh_l1tv_kdiv(void* k, const void* uker, int64_t n, hipStream_t s) {
+  LAUNCH(kdiv_kernel, n, (float2*)k, (const float2*)uker, n);
   return 0;
 }
 

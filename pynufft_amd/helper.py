@@ -486,6 +486,26 @@ def needs_fp64_operators(sn):
     return bool(a.max() / a.min() > FP64_SPREAD)
 
 
+def laplacian_spectrum(Kd):
+    """The Laplacian kernel of the L1-TV solve in k-space, computed as the
+    reference computes it (src/_helper/helper.py:11-45,
+    create_laplacian_kernel): the periodic stencil (-2 ndim at the origin, 1
+    at the two neighbours along each axis) in complex64, transformed by
+    numpy's fftn (complex64 out).  Its rounding (imaginary parts of order
+    1e-7 where the exact spectrum is real) moves the split-Bregman result by
+    up to 1e-5, so the solve uses these values rather than the closed form."""
+    d = len(Kd)
+    stencil = np.zeros(tuple(Kd), dtype=np.complex64)
+    origin = (0,) * d
+    stencil[origin] = -2.0 * d
+    for p in range(d):
+        for step in (1, -1):
+            at = list(origin)
+            at[p] = step
+            stencil[tuple(at)] = 1
+    return np.fft.fftn(stencil)
+
+
 def twiddles(n):
     """exp(-2 pi i t / n), t < n, complex128 (rounded per dtype by the caller)."""
     t = np.arange(n)

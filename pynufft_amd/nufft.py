@@ -191,7 +191,21 @@ class NUFFT:
         if self._format == 'CSR':
             self._build_csr()
         self._create()
+        if st['items'].shape[0]:
+            self._item_bounds()
         return 0
+
+    def _item_bounds(self):
+        """Per-work-item bounds of the gridding sums (a constant of the
+        trajectory and work list, computed on the device once); the plan is
+        re-created with them so that the complex64 gridding packs each cell's
+        fixed-point real and imaginary parts into one 64-bit word."""
+        n = int(self._host['items'].shape[0])
+        b = torch.empty(n, dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            self._launch(self._libh.pnh_item_bounds, self._plan, _ptr(b), self._stream())
+        self._t['item_bound'] = b
+        self._create()
 
     def _ensure_du(self):
         """Upload d u_d / d om_d (FP64 on the host, stored complex64)."""
@@ -282,6 +296,7 @@ class NUFFT:
         d.max_tile_slots = int(st['tile_slots'][:, 1].max()) if st['tile_slots'].size else 0
         d.threads = self._threads
         d.fp64_ops = 1 if getattr(self, 'fp64_ops', False) else 0
+        d.item_bound = t['item_bound'].data_ptr() if 'item_bound' in t else None
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             _lib.check(self._libh.pnh_plan_create(ctypes.byref(d), ctypes.byref(h)))
@@ -650,9 +665,14 @@ class NUFFT:
             raise ValueError('rho must be > 0')
         yt, is_np = self._in(y, 'y')
         out = self._empty('x')
+        lap = self._t.get('laplacian')
+        if lap is None:
+            # the reference's Laplacian spectrum (a constant of Kd)
+            lap = torch.from_numpy(helper.laplacian_spectrum(self.Kd)).to(self.device)
+            self._t['laplacian'] = lap
         ws = self._workspace(_lib.OP_SOLVE_L1TVOLS, _lib.C64)
         self._call(self._libh.pnh_solve_l1tvols, _ptr(yt), _ptr(out), int(maxiter),
-                   float(rho), _ptr(ws), self._stream())
+                   float(rho), _ptr(lap), _ptr(ws), self._stream())
         return self._out(out, is_np)
 
     def _solve_lsqr(self, y, damp=0.0, atol=1e-6, btol=1e-6, conlim=1e8, iter_lim=None,

@@ -187,11 +187,12 @@ def test_cg_residual_output(hip):
         assert abs(A.cg_resid - ref) < 1e-9 * bn
     A.solve(yin, 'cg', maxiter=200, rtol=1e-2)
     assert A.cg_iters < 200 and A.cg_resid < 1e-2 * bn
-    # per coil with batch: coil 1 is coil 0 scaled by 3
+    # per coil with batch: coil 1 is coil 0 scaled by 4 (a power of two, so
+    # the complex64 samples and the shared fixed-point grid scale exactly)
     C = _nufft(g['om'], Nd, Kd, Jd, batch=2)
-    C.solve(np.stack([yin, 3 * yin], axis=-1), 'cg', maxiter=4, rtol=0.0)
+    C.solve(np.stack([yin, 4 * yin], axis=-1), 'cg', maxiter=4, rtol=0.0)
     r = C.cg_resid
-    assert r.shape == (2,) and abs(r[1] / (3 * r[0]) - 1) < 1e-9
+    assert r.shape == (2,) and abs(r[1] / (4 * r[0]) - 1) < 1e-9
 
 
 def test_device_list_stream(hip):

@@ -96,6 +96,90 @@ def emulate_y2k(st, Kd, Jd, y):
     return out
 
 
+def item_bounds(st, Kd, Jd):
+    """The item-bound kernel's quantity: per work item, the largest over
+    the cells of its padded tile of the sum over its rows of the product of
+    the complex64 factor magnitudes."""
+    D = len(Kd)
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
+    pad = helper.tile_pad(tile, Kd, Jd)
+    perm = st['perm']
+    base = st['base'][:, perm]
+    mag = [np.abs(st['ud'][a][perm].astype(np.complex64)).astype(np.float64) for a in range(D)]
+    out = []
+    for tid, first, cnt, slot in st['items']:
+        T0, wrap = tile_frame(int(tid), tile, tgrid, Kd)
+        acc = np.zeros(int(np.prod(pad)))
+        for r in range(first, first + cnt):
+            for j in np.ndindex(*Jd):
+                x = [base[d][r] - T0[d] + j[d] for d in range(D)]
+                x = [x[d] - Kd[d] if x[d] >= wrap[d] else x[d] for d in range(D)]
+                w = 1.0
+                for d in range(D):
+                    w *= mag[d][r][j[d]]
+                acc[int(np.ravel_multi_index(x, pad))] += w
+        out.append(acc.max())
+    return np.array(out)
+
+
+def emulate_y2k_packed(st, Kd, Jd, y):
+    """The packed complex64 gridding in numpy: complex64 factors, scale
+    2^(30 - e) with 2^e > max(|Re y| + |Im y|) * item bound, each rounded
+    contribution added as the 64-bit word re + im * 2^32 (modulo 2^64), the
+    real and imaginary sums unpacked from the total.  Asserts that the sums
+    stay below 2^31 and that unpacking recovers them exactly."""
+    D = len(Kd)
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
+    pad = helper.tile_pad(tile, Kd, Jd)
+    perm = st['perm']
+    base = st['base'][:, perm]
+    u = [st['ud'][a][perm].astype(np.complex64).astype(np.complex128) for a in range(D)]
+    yrow = np.asarray(y).astype(np.complex64).astype(np.complex128)[perm]
+    bounds = item_bounds(st, Kd, Jd)
+    slots = {}
+    for n_item, (tid, first, cnt, slot) in enumerate(st['items']):
+        T0, wrap = tile_frame(int(tid), tile, tgrid, Kd)
+        rows = range(first, first + cnt)
+        ymax = max(abs(yrow[r].real) + abs(yrow[r].imag) for r in rows)
+        B = ymax * bounds[n_item]
+        e = max(np.frexp(B)[1], -960) if B > 0 else 0
+        scale = 2.0 ** (30 - e)
+        n = int(np.prod(pad))
+        packed = [0] * n
+        acc_re = np.zeros(n, dtype=np.int64)
+        acc_im = np.zeros(n, dtype=np.int64)
+        for r in rows:
+            for j in np.ndindex(*Jd):
+                x = [base[d][r] - T0[d] + j[d] for d in range(D)]
+                x = [x[d] - Kd[d] if x[d] >= wrap[d] else x[d] for d in range(D)]
+                w = 1.0
+                for d in range(D):
+                    w = w * u[d][r][j[d]]
+                v = np.conj(w) * yrow[r]
+                re, im = int(np.rint(v.real * scale)), int(np.rint(v.imag * scale))
+                i = int(np.ravel_multi_index(x, pad))
+                packed[i] = (packed[i] + re + (im << 32)) % (1 << 64)
+                acc_re[i] += re
+                acc_im[i] += im
+        assert np.abs(acc_re).max() < 2 ** 31 and np.abs(acc_im).max() < 2 ** 31
+        lo = np.array([(v & 0xffffffff) for v in packed], dtype=np.int64)
+        re = np.where(lo >= 2 ** 31, lo - 2 ** 32, lo)
+        im = np.array([((v - int(r)) % (1 << 64)) >> 32 for v, r in zip(packed, re)],
+                      dtype=np.int64)
+        im = np.where(im >= 2 ** 31, im - 2 ** 32, im)
+        assert np.array_equal(re, acc_re) and np.array_equal(im, acc_im)
+        slots[int(slot)] = (re + 1j * im) / scale
+    out = np.zeros(Kd, dtype=complex)
+    for c in np.ndindex(*Kd):
+        for tix, pidx in gather_sources(c, tile, tgrid, Kd, Jd):
+            s0, ns = st['tile_slots'][tix]
+            for k in range(ns):
+                out[c] += slots[int(s0 + k)][pidx]
+    return out
+
+
 def emulate_k2y(st, Kd, Jd, k):
     """The tile k2y kernel's algorithm in numpy: per work item, the padded
     box of the tile copied from the grid ((T0 + x) wrapped once), every row
@@ -208,6 +292,37 @@ def test_gridding_emulation(name, mode):
     st = helper.plan(g['om'], Nd, Kd, Jd, tile=tile, chunk_rows=chunk)
     k = emulate_y2k(st, Kd, Jd, g['adjoint_in'])
     assert rel(k, g['y2k']) < 1e-13
+
+
+@pytest.mark.parametrize('name', TINY_CASES)
+@pytest.mark.parametrize('mode', ['default', 'chunk3'])
+def test_packed_gridding_emulation(name, mode):
+    """Packed accumulators: no real or imaginary sum reaches 2^31, unpacking
+    the 64-bit totals recovers both sums exactly, and the result is as close
+    to the reference's complex128 y2k as the complex64 factors allow (the
+    same products summed exactly in FP64)."""
+    g = golden(name)
+    Nd, Kd, Jd = geometry(g)
+    st = helper.plan(g['om'], Nd, Kd, Jd, chunk_rows=3 if mode == 'chunk3' else None)
+    k = emulate_y2k_packed(st, Kd, Jd, g['adjoint_in'])
+    A = oracle.OracleNUFFT()
+    A.plan(g['om'], Nd, Kd, Jd)
+    # the same complex64 factors summed exactly (FP64 products, no rounding)
+    perm = st['perm']
+    D = len(Kd)
+    u = [st['ud'][a].astype(np.complex64).astype(np.complex128) for a in range(D)]
+    exact = np.zeros(Kd, dtype=complex)
+    y = np.asarray(g['adjoint_in']).astype(np.complex128)
+    for m in range(len(perm)):
+        for j in np.ndindex(*Jd):
+            w = 1.0
+            for d in range(D):
+                w = w * u[d][m][j[d]]
+            c = tuple((st['base'][d][m] + j[d]) % Kd[d] for d in range(D))
+            exact[c] += np.conj(w) * y[m]
+    e_fix = rel(k, exact)
+    assert e_fix < 2e-8, e_fix
+    assert rel(k, g['y2k']) < rel(exact, g['y2k']) + 2e-8
 
 
 def test_work_list_invariants():
