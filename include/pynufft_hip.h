@@ -284,6 +284,21 @@ int pnh_solve_lgmres(const pnh_plan* plan, const void* y, void* x, int maxiter, 
 int pnh_solve_cg(const pnh_plan* plan, const void* y, void* x, int maxiter,
                  double rtol, double atol, void* ws, int32_t* iters_out,
                  double* resid_out, void* stream);
+/* pnh_solve_cg without a workspace argument and without any allocation: the
+ * scratch is a static device array of PNH_NOALLOC_SCRATCH_BYTES in the
+ * library, one per device, held until the process ends.  A solve whose
+ * pnh_workspace_bytes(PNH_OP_SOLVE_CG) exceeds it fails with -1 (the radial
+ * 2-D 256^2 / 512^2 solve, M 205 824, needs 27 MiB).  Every call on a device
+ * shares that device's array, so calls that overlap on one device (other
+ * streams, other threads) race; calls ordered on one stream do not.  The
+ * plan's complex128 factors must exist before the call.  No counterpart in
+ * the reference (new): for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). */
+#define PNH_NOALLOC_SCRATCH_BYTES (32u << 20)
+int pnh_solve_cg_noalloc(const pnh_plan* plan, const void* y, void* x, int maxiter,
+                         double rtol, double atol, int32_t* iters_out,
+                         double* resid_out, void* stream);
 
 /* ---- many independent problems (uneven Nd, Kd, M; same ndim and Jd) ----
  * The reference handles one problem per NUFFT object; a batch here runs every

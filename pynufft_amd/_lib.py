@@ -18,6 +18,7 @@ C64, C128 = 0, 1
  OP_FORWARD_ONE2MANY, OP_ADJOINT_MANY2ONE, OP_PRECOMPUTE_SP, OP_SELFADJOINT2,
  OP_SOLVE_DC, OP_SOLVE_L1TVOLS, OP_SOLVE_LSQR, OP_SOLVE_LSMR, OP_SOLVE_BICG) = range(15)
 BICGSTAB, BICG = 0, 1
+NOALLOC_SCRATCH_BYTES = 32 << 20   # PNH_NOALLOC_SCRATCH_BYTES
 
 _vp = ctypes.c_void_p
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -109,6 +110,8 @@ EXPORTS = {
                                       _vp, _vp, _vp]),
     'pnh_solve_cg': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, _vp, _vp, _vp, _vp]),
+    'pnh_solve_cg_noalloc': (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, _vp, _vp, _vp]),
     'pnh_batch_table_bytes': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_size_t)]),
     'pnh_batch_create': (ctypes.c_int, [ctypes.POINTER(PlanDesc), ctypes.c_int, _vp, _vp,

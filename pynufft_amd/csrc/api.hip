@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
 
 #include "../../include/pynufft_hip.h"
@@ -1259,6 +1260,63 @@ int pnh_solve_cg(const pnh_plan* p, const void* y, void* x, int maxiter, double 
 
 }  // extern "C"
 
+// ------------------------------------------------ CG with static scratch --
+// Workspace of pnh_solve_cg_noalloc: a file-scope device array, one copy per
+// device, held until the process ends.  Host code never uses the array's
+// name as an address (on the host it is only a stand-in for the symbol); it
+// uses the per-device address from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[PNH_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices];
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int scratch_address(void** out) {
+  int dev = -1;
+  PNH_CHECK_HIP(hipGetDevice(&dev));
+  REQUIRE(dev >= 0 && dev < kScratchDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      pnh_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                    hipGetErrorString(e));
+      return -3;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pnh_solve_cg_noalloc(const pnh_plan* p, const void* y, void* x, int maxiter, double rtol,
+                         double atol, int32_t* iters_out, double* resid_out, void* stream) {
+  REQUIRE(p && x && (y || p->M == 0), "null argument");
+  size_t need = 0;
+  int rc = pnh_workspace_bytes(p, PNH_OP_SOLVE_CG, PNH_C128, &need);
+  if (rc) return rc;
+  REQUIRE(need <= (size_t)PNH_NOALLOC_SCRATCH_BYTES,
+          "this solve needs %zu bytes of scratch, above the %zu bytes of "
+          "pnh_solve_cg_noalloc's static scratch; call pnh_solve_cg with a workspace",
+          need, (size_t)PNH_NOALLOC_SCRATCH_BYTES);
+  void* ws = nullptr;
+  if ((rc = scratch_address(&ws))) return rc;
+  // Every CG buffer in the scratch is written before it is read (partial
+  // slots and item exponents by the gridding, x by a zero kernel, p by the
+  // first p update, the scalars by the init kernel), so nothing is cleared.
+  return cg_solve(p, y, x, maxiter, rtol, atol, ws, iters_out, resid_out, stream, true);
+}
+
+}  // extern "C"
 
 // ================================================================ batches ==
 #include <algorithm>

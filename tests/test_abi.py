@@ -139,3 +139,10 @@ def test_gridding_tile_validation_on_host(lib):
     d = _fake_desc(2, 6, 32)
     d.threads = 100
     assert lib.pnh_batch_table_bytes((_lib.PlanDesc * 1)(d), 1, ctypes.byref(n)) == -1
+
+
+def test_noalloc_scratch_size_matches_header():
+    """The ctypes mirror's static-scratch size is the header's."""
+    from pynufft_amd import _lib
+    m = re.search(r'#define PNH_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)', open(HEADER).read())
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.NOALLOC_SCRATCH_BYTES
