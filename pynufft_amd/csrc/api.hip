@@ -74,10 +74,12 @@ InterpArgs interp_args(const pnh_plan* p, int fp64) {
   return a;
 }
 
-// gridding scratch: item exponents, then the partial slots
+// gridding scratch: item exponents, the row-ordered samples of the exponent
+// pass (long work lists), then the partial slots
 size_t partial_bytes(const pnh_plan* p, int fp64) {
   const size_t esz = fp64 ? 16 : 8;
   return pnh_item_exponent_bytes(p->bins.n_items) +
+         pnh_row_samples_bytes(p->bins, p->M, p->nb, esz) +
          (size_t)p->d.n_slots * p->bins.pad_cells * p->slot_nbc * esz;
 }
 

@@ -74,10 +74,21 @@ int pnh_k2y_dom_launch(const InterpArgs& a, int fp64, const void* grid,
 inline size_t pnh_item_exponent_bytes(int64_t n_items) {
   return ((size_t)n_items * 4 + 255) & ~(size_t)255;
 }
+// Work items from which the gridding computes every item's fixed-point
+// exponent in a pass ahead, which also stores the samples in row order.
+constexpr int64_t PNH_EXPONENT_PASS_ITEMS = 2048;
+// Bytes after the exponents for those row-ordered samples (the coils of one
+// gridding pass, at most 4, of a complex type of esz bytes), or 0 when the
+// work list is too short for the exponent pass.
+inline size_t pnh_row_samples_bytes(const BinArgs& b, int64_t M, int nb, size_t esz) {
+  if (b.n_items < PNH_EXPONENT_PASS_ITEMS) return 0;
+  const int nbc = nb >= 4 ? 4 : (nb >= 2 ? 2 : 1);
+  return ((size_t)M * nbc * esz + 255) & ~(size_t)255;
+}
 // grid[cell*nb + c] = sum over rows r of conj(coef(r, j)) * y[perm[r]*nb + c]
-// for every support cell of r; `part` holds pnh_item_exponent_bytes(n_items)
-// then n_slots * pad_cells * min(nb, 4) elements of the operator's complex
-// type.
+// for every support cell of r; `part` holds pnh_item_exponent_bytes(n_items),
+// then pnh_row_samples_bytes(...), then n_slots * pad_cells * min(nb, 4)
+// elements of the operator's complex type.
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);

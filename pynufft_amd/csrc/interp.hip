@@ -349,7 +349,6 @@ __device__ __forceinline__ T block_max(T v, T* red) {
 // cell) or the item's bound (packed).  Block-wide (every thread calls it);
 // red holds one double per wave.
 constexpr int kPoisonExp = 0x7fffffff;
-constexpr int kExponentPassItems = 2048;   // items from which the exponents are computed ahead
 
 // Whether the complex64 gridding of these work items packs its accumulators.
 template <typename T>
@@ -362,12 +361,16 @@ __device__ __forceinline__ double exponent_bound(const BinArgs& b, int item) {
   return packed_items<T>(b) ? b.item_bound[item] : b.u_bound;
 }
 
+// ys: when non-null, also stores the item's samples in row order there
+// (ys[r * NBC + c]).
 template <typename T, int NBC>
 __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs& b,
                                              int r_begin, int r_end, const void* __restrict__ y_,
-                                             int c0, double bound, double* red) {
+                                             int c0, double bound, double* red,
+                                             void* __restrict__ ys_ = nullptr) {
   typedef typename cplx<T>::t C;
   const C* __restrict__ y = (const C*)y_;
+  C* __restrict__ ys = (C*)ys_;
   double ymax = 0.0;
   int bad = 0;
   for (int r = r_begin + (int)threadIdx.x; r < r_end; r += blockDim.x) {
@@ -375,6 +378,7 @@ __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs&
 #pragma unroll
     for (int c = 0; c < NBC; ++c) {
       const C v = y[m * a.nb + c0 + c];
+      if (ys) ys[(int64_t)r * NBC + c] = v;
       const double s = fabs((double)v.x) + fabs((double)v.y);
       if (s <= 1.0e300) ymax = fmax(ymax, s);
       else bad = 1;                       // NaN or infinity
@@ -390,28 +394,34 @@ __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs&
 }
 
 // Exponents of all work items for coils c0 .. c0+NBC-1, one workgroup per
-// item, ahead of the gridding kernel (so its workgroups start adding at once).
+// item, ahead of the gridding kernel (so its workgroups start adding at once);
+// the samples, read through the row permutation anyway, are stored in row
+// order (ys) for the gridding kernel to read contiguously.
 template <typename T, int NBC>
 __global__ void __launch_bounds__(256)
 y2k_exponent_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_, int c0,
-                    int* __restrict__ item_e, const int* __restrict__ skip) {
+                    int* __restrict__ item_e, const int* __restrict__ skip,
+                    void* __restrict__ ys) {
   if (skip && *skip) return;
   __shared__ double red[16];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
   const int e = item_exponent<T, NBC>(a, b, it.y, it.y + it.z, y_, c0,
-                                      exponent_bound<T>(b, blockIdx.x), red);
+                                      exponent_bound<T>(b, blockIdx.x), red, ys);
   if (threadIdx.x == 0) item_e[blockIdx.x] = e;
 }
 
 // e_pre: the item's exponent computed ahead (y2k_exponent_kernel), or null
 // to compute it here.  item: the item's index in the plan's work list.  PK:
-// packed accumulators (one 64-bit word per cell and coil, else two).
+// packed accumulators (one 64-bit word per cell and coil, else two).  ys_:
+// the samples in row order (ys_[r * NBC + c]) or null to read y through the
+// row permutation.
 template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool PK>
 __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          int item, long long* __restrict__ acc,
                                          const void* __restrict__ y_,
                                          void* __restrict__ part_, int c0,
-                                         const int* __restrict__ e_pre) {
+                                         const int* __restrict__ e_pre,
+                                         const void* __restrict__ ys_ = nullptr) {
   typedef typename cplx<T>::t C;
   constexpr int CW = PK ? 1 : 2;         // accumulator words per (cell, coil)
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -460,10 +470,16 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
   for (int r0 = r_begin + wave * SPW; !poison && r0 < r_end; r0 += nw * SPW) {
     const int r = r0 + sl;
     if (sl >= SPW || jl >= JL || r >= r_end) continue;
-    const int64_t m = a.perm[r];
     C yv[NBC];
+    if (ys_) {
+      // row-ordered samples of the exponent pass (contiguous)
 #pragma unroll
-    for (int c = 0; c < NBC; ++c) yv[c] = y[m * nb + c0 + c];
+      for (int c = 0; c < NBC; ++c) yv[c] = ((const C*)ys_)[(int64_t)r * NBC + c];
+    } else {
+      const int64_t m = a.perm[r];
+#pragma unroll
+      for (int c = 0; c < NBC; ++c) yv[c] = y[m * nb + c0 + c];
+    }
     // this lane's last-axis accumulator index
     const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
                               f.wrap[D - 1], a.K[D - 1]);
@@ -598,12 +614,12 @@ template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool PK>
 __global__ void __launch_bounds__(1024)
 y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
                 void* __restrict__ part_, int c0, const int* __restrict__ item_e,
-                const int* __restrict__ skip) {
+                const int* __restrict__ skip, const void* __restrict__ ys_) {
   if (skip && *skip) return;
   extern __shared__ long long smem_acc[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
   y2k_item<T, D, CSRV, NBC, JN, JE, PK>(a, b, it, blockIdx.x, smem_acc, y_, part_, c0,
-                                        item_e ? item_e + blockIdx.x : nullptr);
+                                        item_e ? item_e + blockIdx.x : nullptr, ys_);
 }
 
 // ------------------------------------------------------------ k2y (tiles) --
@@ -1028,15 +1044,16 @@ int launch_k2y_tiles(const InterpArgs& a, const BinArgs& b, const void* grid, vo
 
 template <typename T, int D, bool CSRV, int NBC>
 int launch_tile_j(const InterpArgs& a, const BinArgs& b, const void* y, void* part, int c0,
-                  int* item_e, size_t lds, const int* skip, hipStream_t s) {
+                  int* item_e, void* ys, size_t lds, const int* skip, hipStream_t s) {
   // With many items per CU, computing every item's exponent in one pass
   // ahead saves each gridding workgroup a dependent load round and a
   // barrier (3-D 128^3: 4 096 items, 11 us of 650); with few items the extra
   // launch costs more than it saves.
-  if (b.n_items < kExponentPassItems) item_e = nullptr;
+  if (b.n_items < PNH_EXPONENT_PASS_ITEMS) item_e = nullptr;
+  if (!item_e) ys = nullptr;
   if (item_e) {
     hipLaunchKernelGGL((y2k_exponent_kernel<T, NBC>), dim3((unsigned)b.n_items), dim3(256), 0,
-                       s, a, b, y, c0, item_e, skip);
+                       s, a, b, y, c0, item_e, skip, ys);
     PNH_LAUNCH_CHECK();
   }
   return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
@@ -1045,7 +1062,7 @@ int launch_tile_j(const InterpArgs& a, const BinArgs& b, const void* y, void* pa
                                   decltype(pk)::value>;
       allow_lds(kern, lds);
       hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds, s, a, b, y,
-                         part, c0, item_e, skip);
+                         part, c0, item_e, skip, ys);
       PNH_LAUNCH_CHECK();
       return 0;
     });
@@ -1059,9 +1076,18 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
     pnh_set_error("gridding workgroup size %d is not a multiple of 64 in 64..1024", b.threads);
     return -1;
   }
-  // part: the items' exponents, then the partial slots
+  // part: the items' exponents, the row-ordered samples, then the slots
+  typedef typename cplx<T>::t C;
   int* item_e = (int*)part;
-  if (part) part = (char*)part + pnh_item_exponent_bytes(b.n_items);
+  void* ys = nullptr;
+  if (part) {
+    part = (char*)part + pnh_item_exponent_bytes(b.n_items);
+    const size_t yb = pnh_row_samples_bytes(b, a.M, a.nb, sizeof(C));
+    if (yb) {
+      ys = part;
+      part = (char*)part + yb;
+    }
+  }
   int c0 = 0;
   while (c0 < a.nb) {
     const bool pk = packed_items<T>(b);
@@ -1074,9 +1100,9 @@ int launch_y2k_nb(const InterpArgs& a, const BinArgs& b, const void* y,
     const size_t lds = y2k_lds_bytes(b.pad_cells, nbc, pk);
     if (b.n_items > 0) {
       int rc;
-      if (nbc == 4) rc = launch_tile_j<T, D, CSRV, 4>(a, b, y, part, c0, item_e, lds, skip, s);
-      else if (nbc == 2) rc = launch_tile_j<T, D, CSRV, 2>(a, b, y, part, c0, item_e, lds, skip, s);
-      else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, item_e, lds, skip, s);
+      if (nbc == 4) rc = launch_tile_j<T, D, CSRV, 4>(a, b, y, part, c0, item_e, ys, lds, skip, s);
+      else if (nbc == 2) rc = launch_tile_j<T, D, CSRV, 2>(a, b, y, part, c0, item_e, ys, lds, skip, s);
+      else rc = launch_tile_j<T, D, CSRV, 1>(a, b, y, part, c0, item_e, ys, lds, skip, s);
       if (rc) return rc;
     }
     const int parts = pnh_gather_parts(b);
@@ -1444,7 +1470,7 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
   y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, pi.y, smem_acc, (const C*)y_ + probs[p].y_off,
-                               (C*)part_ + probs[p].part_off, 0, nullptr);
+                                   (C*)part_ + probs[p].part_off, 0, nullptr);
 }
 
 // P^H P of every problem's work items (k2y then y2k per item, as k2y2k_item).
@@ -1464,7 +1490,7 @@ k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __re
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
   k2y2k_item<T, D, JN, JE, false>(a, b, it, pi.y, smem_kkb, (const C*)k_ + probs[p].k_off,
-                           (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
+                                  (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
 }
 
 // Gather of every problem's grid: pnh_gather_parts blocks per tile, blocks by table.

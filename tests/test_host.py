@@ -87,6 +87,14 @@ def emulate_y2k(st, Kd, Jd, y):
                 acc_re[i] += int(np.rint(v.real * scale))
                 acc_im[i] += int(np.rint(v.imag * scale))
         slots[int(slot)] = (acc_re + 1j * acc_im) / scale
+    return gather(st, Kd, Jd, slots)
+
+
+def gather(st, Kd, Jd, slots):
+    """The gather pass: every grid cell sums, in the kernel's order, the
+    slots of the tiles whose padded box holds it."""
+    tile = st['tile']
+    tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(len(Kd))]
     out = np.zeros(Kd, dtype=complex)
     for c in np.ndindex(*Kd):
         for tix, pidx in gather_sources(c, tile, tgrid, Kd, Jd):
@@ -171,13 +179,7 @@ def emulate_y2k_packed(st, Kd, Jd, y):
         im = np.where(im >= 2 ** 31, im - 2 ** 32, im)
         assert np.array_equal(re, acc_re) and np.array_equal(im, acc_im)
         slots[int(slot)] = (re + 1j * im) / scale
-    out = np.zeros(Kd, dtype=complex)
-    for c in np.ndindex(*Kd):
-        for tix, pidx in gather_sources(c, tile, tgrid, Kd, Jd):
-            s0, ns = st['tile_slots'][tix]
-            for k in range(ns):
-                out[c] += slots[int(s0 + k)][pidx]
-    return out
+    return gather(st, Kd, Jd, slots)
 
 
 def emulate_k2y(st, Kd, Jd, k):
