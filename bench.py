@@ -33,10 +33,10 @@ sys.path.insert(0, HERE)
 METRIC = ("3D 128³ NUFFT fwd+adj pairs/s (% HBM peak); "
           "batched vs plain CG solves/s")
 HBM_PEAK_GBS = 8000.0
-# LDS atomic throughput on gfx950: one wave-wide ds_add_u64 issue every
-# LDS_ADD_U64_CYCLES cycles per CU (microbenchmark, profiles/lds_atomic_rate.txt),
-# 256 CUs at the 2.4 GHz peak engine clock
-LDS_ADD_U64_CYCLES = 8.0
+# LDS bandwidth per CU (bytes per clock): a wave-wide 64-bit atomic moves
+# 64 * 8 B, so it occupies the LDS for at least 4 clocks; 256 CUs at the
+# 2.4 GHz peak engine clock
+LDS_BYTES_PER_CLK = 128.0
 N_CU = 256
 CLOCK_HZ = 2.4e9
 
@@ -69,12 +69,12 @@ def physical_bytes(fmt, M, Jd, Kd, nb=1):
     return M * (coef + 4 + 8 * nb) + K * 8 * nb
 
 
-def lds_atomic_floor_ms(M, Jd, nb=1):
-    """Lower bound of y2k's gridding time from LDS atomics alone: two
-    ds_add_u64 (real and imaginary fixed-point parts) per coefficient,
-    64 lanes per wave-wide instruction."""
-    waves = 2.0 * M * int(np.prod(Jd)) * nb / 64.0
-    return 1e3 * waves * LDS_ADD_U64_CYCLES / (N_CU * CLOCK_HZ)
+def lds_atomic_floor_ms(M, Jd, nb=1, per_coef=1):
+    """Lower bound of y2k's gridding time from the LDS bandwidth its atomics
+    need, without bank conflicts: per_coef ds_add_u64 per coefficient (1
+    with packed accumulators, else 2), 8 bytes per lane."""
+    nbytes = per_coef * 8.0 * M * int(np.prod(Jd)) * nb
+    return 1e3 * nbytes / (LDS_BYTES_PER_CLK * N_CU * CLOCK_HZ)
 
 
 def source_hash():
@@ -368,6 +368,7 @@ def main():
     t_y2k = time_kernel(lambda: A.y2k(y), reps, torch)
     t_fft = time_kernel(lambda: A.xx2k(x), reps, torch)
     fmt = A._format
+    packed = 'item_bound' in A._t
     if t_k2y >= t_y2k:
         kname, kms = 'k2y', t_k2y
     else:
@@ -434,11 +435,15 @@ def main():
                     'note': 'M*prod(Jd)*8 coefficient bytes of a coefficient-streaming '
                             'kernel, which the factor format does not read'}}
     if kname == 'y2k':
-        floor = lds_atomic_floor_ms(M, w['Jd'])
+        per_coef = 1 if packed else 2
+        floor = lds_atomic_floor_ms(M, w['Jd'], per_coef=per_coef)
         roofline['lds_atomic'] = {
             'floor_ms': floor, 'frac': floor / kms,
-            'note': '2 ds_add_u64 per coefficient, %.0f cycles per wave-wide issue per '
-                    'CU, %d CUs at %.1f GHz' % (LDS_ADD_U64_CYCLES, N_CU, CLOCK_HZ / 1e9)}
+            'note': '%d ds_add_u64 per coefficient (%s accumulators), 8 B per lane, LDS at '
+                    '%.0f B/clock per CU, %d CUs at %.1f GHz, no bank conflicts; the op '
+                    'also holds the exponent pass and the gather'
+                    % (per_coef, 'packed' if per_coef == 1 else 'separate real/imaginary',
+                       LDS_BYTES_PER_CLK, N_CU, CLOCK_HZ / 1e9)}
     kern_frac = {n: physical_bytes(fmt, M, w['Jd'], w['Kd']) / (t * 1e-3) / 1e9 / HBM_PEAK_GBS
                  for n, t in (('k2y', t_k2y), ('y2k', t_y2k))}
     kern_frac['xx2k'] = fft_bytes(w['Nd'], w['Kd']) / (t_fft * 1e-3) / 1e9 / HBM_PEAK_GBS
