@@ -319,6 +319,10 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=131072)
     ap.add_argument('--no-batched', action='store_true',
                     help='skip the batched uneven-problem CG comparison')
+    ap.add_argument('--only', choices=['batched', 'cg'], default=None,
+                    help='run only the batched uneven-problem CG comparison or only the '
+                         'radial 2-D solver timings (for profiling them alone); prints '
+                         'that section as JSON')
     ap.add_argument('--copy-probe-mib', type=int, default=0,
                     help='after timing, copy this many MiB with an elementwise '
                          'kernel (known bytes, to calibrate HBM byte counters)')
@@ -331,6 +335,14 @@ def main():
     import torch
     from pynufft_amd import workloads as W
     from pynufft_amd.nufft import NUFFT
+
+    if args.only == 'batched':
+        print(json.dumps({'batched_cg': bench_batched(torch, W)}))
+        return
+    if args.only == 'cg':
+        cg, others = bench_radial_solvers(torch, W, cpu=False)
+        print(json.dumps({'cg_single': cg, 'solvers_radial2d': others}))
+        return
 
     progress('workload %s' % args.workload)
     w = getattr(W, args.workload)()
