@@ -57,7 +57,7 @@ int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n
 // Grids every problem's items into partial slots, then gathers the grids
 // (one block per tile, by table `gather`).
 // k_out = P^H P k for every problem (fused k2y + y2k per work item, then the
-// gather); y receives P k.
+// gather); y receives P k (in row order within each problem).
 int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
                            const BatchTable& gather, int D, const int* Jd, int fp64,
                            int64_t pad_cells, int threads, const void* k, void* y, void* k_out,

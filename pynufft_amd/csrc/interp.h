@@ -92,9 +92,10 @@ inline size_t pnh_row_samples_bytes(const BinArgs& b, int64_t M, int nb, size_t 
 int pnh_y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64,
                    const void* y, void* grid, void* part, const int* skip,
                    hipStream_t s);
-// grid = P^H P k (y2k of k2y): y (the sample buffer) receives P k; one fused
-// gridding pass per work item for one coil and per-axis factors, else k2y
-// then y2k.  `part` as for pnh_y2k_launch.  grid may alias k.
+// grid = P^H P k (y2k of k2y): y is scratch of M * nb elements that receives
+// P k (in row order when fused, else in sample order); one fused gridding
+// pass per work item for one coil and per-axis factors, else k2y then y2k.
+// `part` as for pnh_y2k_launch.  grid may alias k.
 int pnh_k2y2k_launch(const InterpArgs& a, const BinArgs& b, int fp64, const void* k, void* y,
                      void* grid, void* part, const int* skip, hipStream_t s);
 // bound[i] = max over the cells of item i's padded box of the sum of

@@ -362,8 +362,9 @@ __device__ __forceinline__ double exponent_bound(const BinArgs& b, int item) {
 }
 
 // ys: when non-null, also stores the item's samples in row order there
-// (ys[r * NBC + c]).
-template <typename T, int NBC>
+// (ys[r * NBC + c]).  YR: y_ itself holds the samples in row order (then c0
+// and the coil count are not used).
+template <typename T, int NBC, bool YR = false>
 __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs& b,
                                              int r_begin, int r_end, const void* __restrict__ y_,
                                              int c0, double bound, double* red,
@@ -374,10 +375,10 @@ __device__ __forceinline__ int item_exponent(const InterpArgs& a, const BinArgs&
   double ymax = 0.0;
   int bad = 0;
   for (int r = r_begin + (int)threadIdx.x; r < r_end; r += blockDim.x) {
-    const int64_t m = a.perm[r];
+    const int64_t m = YR ? 0 : a.perm[r];
 #pragma unroll
     for (int c = 0; c < NBC; ++c) {
-      const C v = y[m * a.nb + c0 + c];
+      const C v = YR ? y[(int64_t)r * NBC + c] : y[m * a.nb + c0 + c];
       if (ys) ys[(int64_t)r * NBC + c] = v;
       const double s = fabs((double)v.x) + fabs((double)v.y);
       if (s <= 1.0e300) ymax = fmax(ymax, s);
@@ -414,7 +415,7 @@ y2k_exponent_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_, int c0
 // to compute it here.  item: the item's index in the plan's work list.  PK:
 // packed accumulators (one 64-bit word per cell and coil, else two).  ys_:
 // the samples in row order (ys_[r * NBC + c]) or null to read y through the
-// row permutation.
+// row permutation; with e_pre null, the exponent is taken from ys_ too.
 template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool PK>
 __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          int item, long long* __restrict__ acc,
@@ -444,8 +445,9 @@ __device__ __forceinline__ void y2k_item(const InterpArgs& a, const BinArgs& b, 
     e = *e_pre;
     __syncthreads();                      // zeroing before the adds
   } else {
-    e = item_exponent<T, NBC>(a, b, r_begin, r_end, y_, c0,
-                              PK ? b.item_bound[item] : b.u_bound, red);  // also a barrier
+    const double bound = PK ? b.item_bound[item] : b.u_bound;
+    e = ys_ ? item_exponent<T, NBC, true>(a, b, r_begin, r_end, ys_, c0, bound, red)
+            : item_exponent<T, NBC>(a, b, r_begin, r_end, y_, c0, bound, red);  // also a barrier
   }
   // a non-finite sample poisons the item's whole slot
   const bool poison = e == kPoisonExp;
@@ -635,7 +637,8 @@ template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
 __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          typename cplx<T>::t* __restrict__ box,
                                          const void* __restrict__ grid_,
-                                         void* __restrict__ y_, int c0) {
+                                         void* __restrict__ y_, int c0,
+                                         bool y_rows = false) {
   typedef typename cplx<T>::t C;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nb = a.nb;
@@ -786,9 +789,14 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
       }
     }
     if (valid && jl == 0) {
-      const int64_t m = a.perm[r];
+      if (y_rows) {
 #pragma unroll
-      for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
+        for (int c = 0; c < NBC; ++c) y[(int64_t)r * NBC + c] = acc[c];
+      } else {
+        const int64_t m = a.perm[r];
+#pragma unroll
+        for (int c = 0; c < NBC; ++c) y[m * nb + c0 + c] = acc[c];
+      }
     }
   }
 }
@@ -1162,18 +1170,19 @@ namespace {
 // input grid, then y2k of those rows into the same-sized accumulator (the
 // rows' support boxes lie in the tile's padded box for both).  The box and the
 // accumulator share the LDS (the box is dead once y is formed); y goes
-// through the caller's sample buffer, which the barrier makes visible to the
-// whole workgroup, so the factors and samples of the second phase come from
-// L2 rather than HBM.  One coil, per-axis factors.
+// through the caller's sample buffer, in row order (y[r], contiguous over the
+// item's rows), which the barrier makes visible to the whole workgroup, so
+// the factors and samples of the second phase come from L2 rather than HBM.
+// One coil, per-axis factors.
 template <typename T, int D, int JN, int JE, bool PK>
 __device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                            int item, long long* smem,
                                            const void* __restrict__ k_, void* __restrict__ y_,
                                            void* __restrict__ part_) {
   typedef typename cplx<T>::t C;
-  k2y_item<T, D, false, 1, JN, JE>(a, b, it, reinterpret_cast<C*>(smem), k_, y_, 0);
+  k2y_item<T, D, false, 1, JN, JE>(a, b, it, reinterpret_cast<C*>(smem), k_, y_, 0, true);
   __syncthreads();
-  y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, item, smem, y_, part_, 0, nullptr);
+  y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, item, smem, y_, part_, 0, nullptr, y_);
 }
 
 template <typename T, int D, int JN, int JE, bool PK>
