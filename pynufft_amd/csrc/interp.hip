@@ -248,11 +248,17 @@ __device__ __forceinline__ void lds_add_fixed(long long* cell, double2 v) {
 // Packed accumulator word of rounded parts re, im (each already offset by
 // 1.5 * 2^52, magnitude below 2^31): re + im * 2^32 modulo 2^64.  re's bits
 // minus the constant's are re as a 64-bit integer (sign included); im
-// enters through its low word only.
+// enters through its low word only.  The constant's low word is zero, so the
+// low word is re's and the high word is one three-operand 32-bit add (written
+// out: the compiler otherwise forms a 64-bit add and a second 32-bit add).
 __device__ __forceinline__ long long pack_fixed(double re, double im) {
-  const long long rb = __double_as_longlong(re) - kMagicBits;
-  const unsigned long long ib = (unsigned)__double_as_longlong(im);
-  return (long long)((unsigned long long)rb + (ib << 32));
+  const unsigned long long rb = (unsigned long long)__double_as_longlong(re);
+  const unsigned neg_magic_hi = 0u - (unsigned)((unsigned long long)kMagicBits >> 32);
+  unsigned hi;
+  asm("v_add3_u32 %0, %1, %2, %3"
+      : "=v"(hi)
+      : "v"((unsigned)(rb >> 32)), "v"((unsigned)__double_as_longlong(im)), "s"(neg_magic_hi));
+  return (long long)(((unsigned long long)hi << 32) | (rb & 0xffffffffull));
 }
 
 // Packed contribution u * a (a scaled by 2^(30 - e)), rounding as lds_add_prod.
