@@ -120,6 +120,14 @@ typedef struct pnh_plan_desc {
    * two: half the LDS atomics and accumulator.  No counterpart in the
    * reference (an implementation choice of the gridding). */
   const double* item_bound;
+  /* >= max over rows of sum_j |coefficient(row, j)| over the row's support
+   * (for per-axis factors prod_d sum_j |u_d[row][j]|), or 0.  With it the
+   * FP64 P^H P (pnh_k2y2k in complex128, and the solvers) grids each work
+   * item in one pass over its rows, reading each row's factors once; the
+   * fixed-point exponent then comes from this bound and the largest input
+   * value of the item's box instead of the largest sample.  No counterpart
+   * in the reference (an implementation choice of the gridding). */
+  double row_bound;
 } pnh_plan_desc;
 
 typedef struct pnh_plan pnh_plan;

@@ -55,6 +55,7 @@ class PlanDesc(ctypes.Structure):
         ('threads', ctypes.c_int32),
         ('fp64_ops', ctypes.c_int32),
         ('item_bound', _vp),
+        ('row_bound', ctypes.c_double),
     ]
 
 

@@ -312,6 +312,8 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   p->bins.items = d.items;
   p->bins.tile_slots = d.tile_slots;
   p->bins.item_bound = d.item_bound;
+  REQUIRE(d.row_bound >= 0.0 && d.row_bound < 1e300, "row_bound must be >= 0");
+  p->bins.row_bound = d.row_bound;
   if (d.threads == 0) {
     // gridding: by the complex64 accumulator, one 64-bit word per cell when
     // packed (two 512-thread workgroups share a CU at 16^3), else two;

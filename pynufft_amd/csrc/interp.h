@@ -40,6 +40,10 @@ struct BinArgs {
   // complex64 gridding packs each cell's real and imaginary fixed-point
   // parts into one 64-bit accumulator (see y2k_item).
   const double* item_bound;
+  // >= max over rows of the sum of |coefficient| over the row's support, or
+  // 0.  With it the FP64 P^H P grids each work item in one pass over its
+  // rows (the fixed-point exponent from the row bound and the input box).
+  double row_bound;
 };
 
 // Workgroups per tile in the gridding gather pass: one, or one per 256

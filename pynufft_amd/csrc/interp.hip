@@ -908,6 +908,17 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
           for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], v[mask][cc]);
         }
     } else {
+      // The first slot of every source is loaded before any sum (one
+      // round trip when the sources have one slot each, the common case);
+      // a heavy source's further slots follow in NQ running sums (slots
+      // k = 1 + j mod NQ, in row order each), combined pairwise in a fixed
+      // tree, so the result stays reproducible.  Eight for one coil, four
+      // for several (register budget).
+      constexpr int NQ = NBC == 1 ? 8 : 4;
+      const int64_t stride = b.pad_cells * NBC;
+      C v0[1 << D][NBC];
+      const C* ps[1 << D];
+      int nsl[1 << D];
 #pragma unroll
       for (int mask = 0; mask < (1 << D); ++mask) {
         bool ok = src[mask].y > 0;
@@ -918,38 +929,49 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
           ok &= !bit || l[d] < below[d];
           pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
         }
-        if (!ok) continue;
-        // a heavy tile has many slots: NQ running sums (slots k = j mod NQ,
-        // in row order each) keep NQ loads in flight; they are combined
-        // pairwise in a fixed tree, so the result stays reproducible.  Eight
-        // for one coil, four for several (register budget).
-        constexpr int NQ = NBC == 1 ? 8 : 4;
-        const int ns = src[mask].y;
-        const int64_t stride = b.pad_cells * NBC;
-        const C* p = part + ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC;
-        C q[NQ][NBC];
+        nsl[mask] = ok ? src[mask].y : 0;
+        ps[mask] = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC : 0);
 #pragma unroll
-        for (int j = 0; j < NQ; ++j)
+        for (int cc = 0; cc < NBC; ++cc)
+          v0[mask][cc] = ok ? ps[mask][cc] : mk<T>((T)0, (T)0);
+      }
 #pragma unroll
-          for (int cc = 0; cc < NBC; ++cc) q[j][cc] = mk<T>((T)0, (T)0);
-        int k = 0;
-        for (; k + NQ <= ns; k += NQ) {
+      for (int mask = 0; mask < (1 << D); ++mask) {
+        const int ns = nsl[mask];
+        if (ns == 0) continue;
+        C t[NBC];
+#pragma unroll
+        for (int cc = 0; cc < NBC; ++cc) t[cc] = v0[mask][cc];
+        if (ns > 1) {
+          const C* p = ps[mask] + stride;
+          C q[NQ][NBC];
 #pragma unroll
           for (int j = 0; j < NQ; ++j)
 #pragma unroll
-            for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[j][cc], p[(k + j) * stride + cc]);
+            for (int cc = 0; cc < NBC; ++cc) q[j][cc] = mk<T>((T)0, (T)0);
+          const int nr = ns - 1;
+          int k = 0;
+          for (; k + NQ <= nr; k += NQ) {
+#pragma unroll
+            for (int j = 0; j < NQ; ++j)
+#pragma unroll
+              for (int cc = 0; cc < NBC; ++cc)
+                q[j][cc] = cadd(q[j][cc], p[(k + j) * stride + cc]);
+          }
+          for (; k < nr; ++k)
+#pragma unroll
+            for (int cc = 0; cc < NBC; ++cc) q[0][cc] = cadd(q[0][cc], p[k * stride + cc]);
+#pragma unroll
+          for (int w = NQ / 2; w >= 1; w /= 2)
+#pragma unroll
+            for (int j = 0; j < w; ++j)
+#pragma unroll
+              for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[2 * j][cc], q[2 * j + 1][cc]);
+#pragma unroll
+          for (int cc = 0; cc < NBC; ++cc) t[cc] = cadd(t[cc], q[0][cc]);
         }
-        for (; k < ns; ++k)
 #pragma unroll
-          for (int cc = 0; cc < NBC; ++cc) q[0][cc] = cadd(q[0][cc], p[k * stride + cc]);
-#pragma unroll
-        for (int w = NQ / 2; w >= 1; w /= 2)
-#pragma unroll
-          for (int j = 0; j < w; ++j)
-#pragma unroll
-            for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[2 * j][cc], q[2 * j + 1][cc]);
-#pragma unroll
-        for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], q[0][cc]);
+        for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], t[cc]);
       }
     }
     int64_t g = 0;
@@ -977,6 +999,9 @@ y2k_gather_kernel(InterpArgs a, BinArgs b, int parts, const void* __restrict__ p
 size_t y2k_lds_bytes(int64_t pad_cells, int nbc, bool packed) {
   return (size_t)pad_cells * nbc * (packed ? 8 : 16) + 256;
 }
+
+// LDS of the one-pass FP64 P^H P: the box and the two-word accumulator.
+size_t k2y2k_1p_lds_bytes(int64_t pad_cells) { return (size_t)pad_cells * 32 + 256; }
 
 // Calls f(std::integral_constant<bool, PK>) with PK = packed (the packed
 // variant exists for complex64 only).
@@ -1191,6 +1216,196 @@ __device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b
   y2k_item<T, D, false, 1, JN, JE, PK>(a, b, it, item, smem, y_, part_, 0, nullptr, y_);
 }
 
+// P^H P on one work item in one pass over its rows (FP64, one coil,
+// per-axis factors).  The item's padded box of the input and the accumulator
+// are both in LDS; each row's factors are loaded once, its sample
+// y_r = sum_j coef(r, j) box[cell(r, j)] is formed by the row's lanes (as
+// k2y_item) and at once gridded back as conj(coef(r, j)) y_r (as y2k_item),
+// so the samples never leave registers.  The fixed-point exponent is fixed
+// before the first add from a bound of every sample of the item:
+// |Re y_r| + |Im y_r| <= sqrt(2) |y_r| <= sqrt(2) row_bound m, m the largest
+// |Re k| + |Im k| of the box.  A non-finite sample poisons the item's slot,
+// as in the two-phase pass.  LDS: box [pad_cells] complex128, accumulator
+// [pad_cells][2] int64, then 16 doubles of reduction scratch and a flag.
+template <int D, int JN, int JE>
+__device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs& b,
+                                              const int4 it, long long* smem,
+                                              const void* __restrict__ k_,
+                                              void* __restrict__ part_) {
+  typedef double2 C;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int pc = (int)b.pad_cells;
+  C* __restrict__ box = reinterpret_cast<C*>(smem);
+  long long* __restrict__ acc = smem + 2 * (int64_t)pc;
+  double* red = reinterpret_cast<double*>(acc + 2 * (int64_t)pc);
+  int* poisoned = reinterpret_cast<int*>(red + 16);
+  const TileFrame f = tile_frame<D>(a, b, it.x);
+  const int P1 = D > 1 ? b.pad[1] : 1;
+  const int P2 = D > 2 ? b.pad[2] : 1;
+  const C* __restrict__ grid = (const C*)k_;
+  // padded box -> LDS (last axis fastest), accumulator cleared, box maximum
+  double kmax = 0.0;
+  {
+    const float r1 = 1.0f / (float)P1, r2 = 1.0f / (float)P2;
+    longlong2* acc2 = reinterpret_cast<longlong2*>(acc);
+    for (int i = tid; i < pc; i += nthr) {
+      int x2 = 0, x1 = 0, x0 = i;
+      if (D > 2) x0 = fdivmod(x0, P2, r2, x2);
+      if (D > 1) x0 = fdivmod(x0, P1, r1, x1);
+      int c = f.T0[0] + x0;
+      int g = (c >= a.K[0] ? c - a.K[0] : c) * (int)a.Kstride[0];
+      if (D > 1) {
+        c = f.T0[1] + x1;
+        g += (c >= a.K[1] ? c - a.K[1] : c) * (int)a.Kstride[1];
+      }
+      if (D > 2) {
+        c = f.T0[2] + x2;
+        g += c >= a.K[2] ? c - a.K[2] : c;
+      }
+      const C v = grid[g];
+      box[i] = v;
+      acc2[i] = make_longlong2(0, 0);
+      // non-finite values are caught in the samples they reach
+      const double m = fabs(v.x) + fabs(v.y);
+      if (m <= 1.0e300) kmax = fmax(kmax, m);
+    }
+  }
+  if (tid == 0) *poisoned = 0;
+  kmax = block_max(kmax, red);            // also the barrier after the box writes
+  // 1.5 > sqrt(2), with room for the rounding of the products
+  const double B = 1.5 * b.row_bound * b.u_bound * kmax;
+  int e = 0;
+  if (B > 1.0e300) e = 1000;
+  else if (B > 0.0) (void)frexp(B, &e);   // B < 2^e
+  if (e < -960) e = -960;
+  const double scale = ldexp(1.0, kFixBits - e);
+  const double unscale = ldexp(1.0, e - kFixBits);
+
+  const int JL = a.J[D - 1];
+  const int LPS = pnh_lanes_per_sample(JL);
+  const int SPW = 64 / LPS;
+  const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+  const int sl = lane / LPS;
+  const int jl = lane - sl * LPS;
+  const C* __restrict__ u = (const C*)a.u;
+  const int J0 = JE > 0 ? JE : a.J[0];
+  const int J1 = D > 2 ? (JE > 0 ? JE : a.J[1]) : 1;
+  const int r_end = it.y + it.z;
+  constexpr int CB = 16;                  // bytes of one accumulator cell
+  char* const accb = reinterpret_cast<char*>(acc);
+  bool bad = false;
+
+  for (int r0 = it.y + wave * SPW; r0 < r_end; r0 += nw * SPW) {
+    const int r = r0 + sl;
+    const bool valid = sl < SPW && jl < JL && r < r_end;
+    C ul = make_double2(0.0, 0.0);
+    C s = make_double2(0.0, 0.0);
+    int xl = 0;
+    C u0c[JN], u1c[JN];
+    int row0[JN], off1[JN];
+    if (valid) {
+      xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl, f.wrap[D - 1],
+                      a.K[D - 1]);
+      ul = u[a.u_off[D - 1] + (int64_t)r * JL + jl];
+      if (D == 1) {
+        s = cmulf(ul, box[xl]);
+      } else {
+        const int x0b = a.base[r] - f.T0[0];
+        const C* __restrict__ u0 = u + a.u_off[0] + (int64_t)r * J0;
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+          if (JE > 0 || j < J0) {
+            u0c[j] = u0[j];
+            row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+          }
+        C s0 = make_double2(0.0, 0.0);
+        if (D == 2) {
+#pragma unroll
+          for (int j0 = 0; j0 < JN; ++j0)
+            if (JE > 0 || j0 < J0) s0 = cfma(u0c[j0], box[row0[j0] + xl], s0);
+        } else {
+          const int x1b = a.base[a.M + r] - f.T0[1];
+          const C* __restrict__ u1 = u + a.u_off[1] + (int64_t)r * J1;
+#pragma unroll
+          for (int j = 0; j < JN; ++j)
+            if (JE > 0 || j < J1) {
+              u1c[j] = u1[j];
+              off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+            }
+#pragma unroll
+          for (int j0 = 0; j0 < JN; ++j0) {
+            if (JE > 0 || j0 < J0) {
+              C s1 = make_double2(0.0, 0.0);
+#pragma unroll
+              for (int j1 = 0; j1 < JN; ++j1)
+                if (JE > 0 || j1 < J1) s1 = cfma(u1c[j1], box[row0[j0] + off1[j1]], s1);
+              s0 = cfma(u0c[j0], s1, s0);
+            }
+          }
+        }
+        s = cmulf(ul, s0);
+      }
+    }
+    // sum over the row's lanes (all lanes take part), then every lane of
+    // the row takes the total from the row's first lane
+    for (int off = 1; off < LPS; off <<= 1) {
+      const C o = shfl_down_c(s, off);
+      if (jl + off < LPS) s = cadd(s, o);
+    }
+    C y;
+    y.x = __shfl(s.x, lane - jl, 64);
+    y.y = __shfl(s.y, lane - jl, 64);
+    if (!valid) continue;
+    if (!(fabs(y.x) + fabs(y.y) <= 1.0e300)) {
+      bad = true;
+      continue;
+    }
+    const double2 ys = make_double2(y.x * scale, y.y * scale);
+    const double2 wl = dmul(cconj(ul), ys);
+    if (D == 1) {
+      lds_add_fixed(acc + 2 * xl, wl);
+      continue;
+    }
+    if (D == 2) {
+#pragma unroll
+      for (int j0 = 0; j0 < JN; ++j0)
+        if (JE > 0 || j0 < J0)
+          lds_add_prod(reinterpret_cast<long long*>(accb + (row0[j0] + xl) * CB),
+                       cconj(u0c[j0]), wl);
+      continue;
+    }
+#pragma unroll
+    for (int j0 = 0; j0 < JN; ++j0) {
+      if (JE > 0 || j0 < J0) {
+        const double2 a0 = dmul(cconj(u0c[j0]), wl);
+#pragma unroll
+        for (int j1 = 0; j1 < JN; ++j1)
+          if (JE > 0 || j1 < J1)
+            lds_add_prod(reinterpret_cast<long long*>(accb + (row0[j0] + off1[j1]) * CB),
+                         cconj(u1c[j1]), a0);
+      }
+    }
+  }
+  if (bad) *poisoned = 1;
+  __syncthreads();
+  const bool poison = *poisoned != 0;
+  C* __restrict__ part = (C*)part_ + (int64_t)it.w * pc;
+  const double nan = __builtin_nan("");
+  for (int i = tid; i < pc; i += nthr)
+    part[i] = poison ? make_double2(nan, nan)
+                     : make_double2((double)acc[2 * i] * unscale, (double)acc[2 * i + 1] * unscale);
+}
+
+template <int D, int JN, int JE>
+__global__ void __launch_bounds__(1024)
+k2y2k_1p_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ k_,
+                     void* __restrict__ part_, const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  extern __shared__ long long smem_kk1[];
+  const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
+  k2y2k_item_1p<D, JN, JE>(a, b, it, smem_kk1, k_, part_);
+}
+
 template <typename T, int D, int JN, int JE, bool PK>
 __global__ void __launch_bounds__(1024)
 k2y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ k_, void* __restrict__ y_,
@@ -1206,7 +1421,21 @@ int launch_k2y2k(const InterpArgs& a, const BinArgs& b, const void* k, void* y, 
                  void* part, const int* skip, hipStream_t s) {
   void* slots = (char*)part + pnh_item_exponent_bytes(b.n_items);
   const size_t lds = y2k_lds_bytes(b.pad_cells, 1, packed_items<T>(b));
-  if (b.n_items > 0) {
+  const size_t lds1 = k2y2k_1p_lds_bytes(b.pad_cells);
+  if (sizeof(T) == 8 && b.row_bound > 0.0 && lds1 <= pnh_lds_limit()) {
+    // FP64 with a row bound: one pass over each item's rows
+    if (b.n_items > 0) {
+      int rc = dispatch_j<D, false>(a, [&](auto jn, auto je) {
+        auto kern = k2y2k_1p_tile_kernel<D, decltype(jn)::value, decltype(je)::value>;
+        allow_lds(kern, lds1);
+        hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds1, s, a, b, k,
+                           slots, skip);
+        PNH_LAUNCH_CHECK();
+        return 0;
+      });
+      if (rc) return rc;
+    }
+  } else if (b.n_items > 0) {
     int rc = dispatch_j<D, false>(a, [&](auto jn, auto je) {
       return dispatch_pk<T>(packed_items<T>(b), [&](auto pk) {
         auto kern = k2y2k_tile_kernel<T, D, decltype(jn)::value, decltype(je)::value,
@@ -1488,13 +1717,15 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
                                    (C*)part_ + probs[p].part_off, 0, nullptr);
 }
 
-// P^H P of every problem's work items (k2y then y2k per item, as k2y2k_item).
+// P^H P of every problem's work items (k2y then y2k per item, as k2y2k_item;
+// FP64 problems with a row bound in one pass, as k2y2k_item_1p, when
+// one_pass: the launch's LDS holds that pass's box and accumulator).
 template <typename T, int D, int JN, int JE>
 __global__ void __launch_bounds__(1024)
 k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
                         const void* __restrict__ k_, void* __restrict__ y_,
                         void* __restrict__ part_, const int* __restrict__ done,
-                        const int* __restrict__ all_done) {
+                        const int* __restrict__ all_done, int one_pass) {
   typedef typename cplx<T>::t C;
   if (all_done && *all_done) return;
   const int2 pi = items[blockIdx.x];
@@ -1504,6 +1735,13 @@ k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __re
   const InterpArgs a = sizeof(T) == 8 ? probs[p].ia64 : probs[p].ia;
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
+  if constexpr (sizeof(T) == 8) {
+    if (one_pass && b.row_bound > 0.0) {  // uniform over the workgroup
+      k2y2k_item_1p<D, JN, JE>(a, b, it, smem_kkb, (const C*)k_ + probs[p].k_off,
+                               (C*)part_ + probs[p].part_off);
+      return;
+    }
+  }
   k2y2k_item<T, D, JN, JE, false>(a, b, it, pi.y, smem_kkb, (const C*)k_ + probs[p].k_off,
                                   (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
 }
@@ -1597,12 +1835,12 @@ namespace {
 template <typename T, int D>
 int batch_k2y2k_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
                   size_t lds, int threads, const void* k, void* y, void* part, const int* done,
-                  const int* all_done, hipStream_t s) {
+                  const int* all_done, int one_pass, hipStream_t s) {
   return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
     auto kern = k2y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
     allow_lds(kern, lds);
     hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, k, y,
-                       part, done, all_done);
+                       part, done, all_done, one_pass);
     PNH_LAUNCH_CHECK();
     return 0;
   });
@@ -1613,14 +1851,16 @@ int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t
                            const BatchTable& gather, int D, const int* Jd, int fp64,
                            int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
                            void* part, const int* done, const int* all_done, hipStream_t s) {
-  const size_t lds = y2k_lds_bytes(pad_cells, 1, false);
+  size_t lds = y2k_lds_bytes(pad_cells, 1, false);
   if (lds > pnh_lds_limit()) {
     pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
                   pnh_lds_limit());
     return -2;
   }
+  const int one_pass = fp64 && k2y2k_1p_lds_bytes(pad_cells) <= pnh_lds_limit();
+  if (one_pass) lds = k2y2k_1p_lds_bytes(pad_cells);
   int rc = 0;
-#define KK(T, DD) rc = batch_k2y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, part, done, all_done, s)
+#define KK(T, DD) rc = batch_k2y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, part, done, all_done, one_pass, s)
   if (n_items > 0) {
     if (fp64) { if (D == 1) KK(double, 1); else if (D == 2) KK(double, 2); else KK(double, 3); }
     else { if (D == 1) KK(float, 1); else if (D == 2) KK(float, 2); else KK(float, 3); }

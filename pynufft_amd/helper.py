@@ -339,6 +339,17 @@ def coefficient_bound(ud):
     return float(prod.max()) * (1.0 + 2.0 ** -20)
 
 
+def row_sum_bound(ud):
+    """Upper bound of sum_j |coefficient(row, j)| = prod_d sum_j |u_d[row, j]|
+    over all rows, with the same margin as coefficient_bound."""
+    if not len(ud) or ud[0].shape[0] == 0:
+        return 1.0
+    prod = np.ones(ud[0].shape[0])
+    for u in ud:
+        prod = prod * np.abs(u).sum(axis=1)
+    return float(prod.max()) * (1.0 + 2.0 ** -20)
+
+
 _CACHE_VERSION = 1
 
 
@@ -464,6 +475,7 @@ def plan(om, Nd, Kd, Jd, ft_axes=None, tile=None, chunk_rows=None, cache_dir=Non
     perm, items, tile_slots = sort_and_bin(base, Kd, Jd, tile, chunk_rows)
     st.update(ud=ud, base=base, perm=perm, tile=tile, items=items,
               tile_slots=tile_slots, u_bound=coefficient_bound(ud),
+              row_bound=row_sum_bound(ud),
               fp64_ops=needs_fp64_operators(st['sn']))
     return st
 

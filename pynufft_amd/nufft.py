@@ -288,6 +288,7 @@ class NUFFT:
         d.sn_f32 = t['sn_f32'].data_ptr()
         d.sn_f64 = t['sn_f64'].data_ptr()
         d.u_bound = float(st['u_bound'])
+        d.row_bound = float(st['row_bound'])
         d.n_items = int(st['items'].shape[0])
         d.items = t['items'].data_ptr()
         d.n_tiles = int(st['tile_slots'].shape[0])

@@ -395,6 +395,19 @@ def test_autotune_candidates_are_valid():
     assert helper.coefficient_bound([np.array([[3 + 4j, 1]]), np.array([[2, -1]])]) >= 10
 
 
+def test_row_sum_bound_brute_force():
+    """row_sum_bound bounds sum_j |coefficient(row, j)| over every row: the
+    full Kronecker coefficients of random 3-D factors, summed directly."""
+    rng = np.random.default_rng(5)
+    ud = [rng.standard_normal((7, J)) + 1j * rng.standard_normal((7, J)) for J in (3, 4, 2)]
+    full = np.einsum('ma,mb,mc->mabc', *ud).reshape(7, -1)
+    exact = np.abs(full).sum(axis=1).max()
+    b = helper.row_sum_bound(ud)
+    assert exact <= b <= exact * (1 + 1e-5)
+    # one row, closed form: (|3+4i| + 1) * (2 + 1)
+    assert abs(helper.row_sum_bound([np.array([[3 + 4j, 1]]), np.array([[2, -1]])]) - 18) < 1e-4
+
+
 def test_plan_errors_follow_reference():
     om = np.zeros((4, 2))
     with pytest.raises(TypeError):
