@@ -910,10 +910,11 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
     } else {
       // The first slot of every source is loaded before any sum (one
       // round trip when the sources have one slot each, the common case);
-      // a heavy source's further slots follow in NQ running sums (slots
-      // k = 1 + j mod NQ, in row order each), combined pairwise in a fixed
-      // tree, so the result stays reproducible.  Eight for one coil, four
-      // for several (register budget).
+      // a heavy source's further slots follow in rounds of NQ loads into NQ
+      // running sums (slots 1 + k with k = j mod NQ, in row order each),
+      // combined pairwise in a fixed tree, so the result stays
+      // reproducible.  Eight for one coil, four for several (register
+      // budget).
       constexpr int NQ = NBC == 1 ? 8 : 4;
       const int64_t stride = b.pad_cells * NBC;
       C v0[1 << D][NBC];
@@ -950,17 +951,19 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
 #pragma unroll
             for (int cc = 0; cc < NBC; ++cc) q[j][cc] = mk<T>((T)0, (T)0);
           const int nr = ns - 1;
-          int k = 0;
-          for (; k + NQ <= nr; k += NQ) {
+          for (int k = 0; k < nr; k += NQ) {
+            // NQ loads in flight, the last round's beyond nr skipped
+            C v[NQ][NBC];
 #pragma unroll
             for (int j = 0; j < NQ; ++j)
 #pragma unroll
               for (int cc = 0; cc < NBC; ++cc)
-                q[j][cc] = cadd(q[j][cc], p[(k + j) * stride + cc]);
-          }
-          for (; k < nr; ++k)
+                v[j][cc] = k + j < nr ? p[(int64_t)(k + j) * stride + cc] : mk<T>((T)0, (T)0);
 #pragma unroll
-            for (int cc = 0; cc < NBC; ++cc) q[0][cc] = cadd(q[0][cc], p[k * stride + cc]);
+            for (int j = 0; j < NQ; ++j)
+#pragma unroll
+              for (int cc = 0; cc < NBC; ++cc) q[j][cc] = cadd(q[j][cc], v[j][cc]);
+          }
 #pragma unroll
           for (int w = NQ / 2; w >= 1; w /= 2)
 #pragma unroll
