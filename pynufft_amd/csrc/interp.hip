@@ -869,14 +869,40 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
   const int P1 = D > 1 ? b.pad[1] : 1, P2 = D > 2 ? b.pad[2] : 1;
   const int n = E[0] * E1 * E2;
   const int end = first + count < n ? first + count : n;
+  // per source: the padded-box cell offset of the tile's cells in the
+  // source's slot (uniform); a cell's slot index is its own padded-box index
+  // plus this
+  int offc[1 << D];
+#pragma unroll
+  for (int mask = 0; mask < (1 << D); ++mask) {
+    int o = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) o = o * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + off[mask][d];
+    offc[mask] = o;
+  }
+  int64_t g0 = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) g0 += (int64_t)T0[d] * a.Kstride[d];
+  // the cell's coordinates in the tile, advanced by blockDim.x cells per
+  // step (no division per cell)
+  int l[PNH_MAXD];
+  {
+    int rest = first + (int)threadIdx.x;
+    if (D > 2) { l[2] = rest % E2; rest /= E2; }
+    if (D > 1) { l[1] = rest % E1; rest /= E1; }
+    l[0] = rest;
+  }
+  int sl[PNH_MAXD];
+  {
+    int rest = (int)blockDim.x;
+    if (D > 2) { sl[2] = rest % E2; rest /= E2; }
+    if (D > 1) { sl[1] = rest % E1; rest /= E1; }
+    sl[0] = rest;
+  }
   for (int i = first + threadIdx.x; i < end; i += blockDim.x) {
-    int l[PNH_MAXD];
-    {
-      int rest = i;
-      if (D > 2) { l[2] = rest % E2; rest /= E2; }
-      if (D > 1) { l[1] = rest % E1; rest /= E1; }
-      l[0] = rest;
-    }
+    int pc = l[0];
+    if (D > 1) pc = pc * P1 + l[1];
+    if (D > 2) pc = pc * P2 + l[2];
     C s[NBC];
 #pragma unroll
     for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
@@ -889,15 +915,11 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
 #pragma unroll
       for (int mask = 0; mask < (1 << D); ++mask) {
         bool ok = src[mask].y > 0;
-        int pidx = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const bool bit = (mask >> d) & 1;
-          ok &= !bit || l[d] < below[d];
-          pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
-        }
+        for (int d = 0; d < D; ++d)
+          if ((mask >> d) & 1) ok &= l[d] < below[d];
         use[mask] = ok;
-        const C* p = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC : 0);
+        const C* p = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pc + offc[mask]) * NBC : 0);
 #pragma unroll
         for (int cc = 0; cc < NBC; ++cc) v[mask][cc] = p[cc];
       }
@@ -923,15 +945,11 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
 #pragma unroll
       for (int mask = 0; mask < (1 << D); ++mask) {
         bool ok = src[mask].y > 0;
-        int pidx = 0;
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const bool bit = (mask >> d) & 1;
-          ok &= !bit || l[d] < below[d];
-          pidx = pidx * (d == 0 ? 1 : (d == 1 ? P1 : P2)) + l[d] + off[mask][d];
-        }
+        for (int d = 0; d < D; ++d)
+          if ((mask >> d) & 1) ok &= l[d] < below[d];
         nsl[mask] = ok ? src[mask].y : 0;
-        ps[mask] = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pidx) * NBC : 0);
+        ps[mask] = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pc + offc[mask]) * NBC : 0);
 #pragma unroll
         for (int cc = 0; cc < NBC; ++cc)
           v0[mask][cc] = ok ? ps[mask][cc] : mk<T>((T)0, (T)0);
@@ -977,11 +995,28 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
         for (int cc = 0; cc < NBC; ++cc) s[cc] = cadd(s[cc], t[cc]);
       }
     }
-    int64_t g = 0;
-#pragma unroll
-    for (int d = 0; d < D; ++d) g += (int64_t)(T0[d] + l[d]) * a.Kstride[d];
+    int64_t g = g0 + l[0] * a.Kstride[0];
+    if (D > 1) g += l[1] * a.Kstride[1];
+    if (D > 2) g += l[2] * a.Kstride[2];
 #pragma unroll
     for (int cc = 0; cc < NBC; ++cc) grid[g * a.nb + c0 + cc] = s[cc];
+    // next cell: add the step with carries (each coordinate stays below
+    // twice its extent, so one subtraction suffices)
+    if (D > 2) {
+      l[2] += sl[2];
+      const int c2 = l[2] >= E2;
+      l[2] -= c2 ? E2 : 0;
+      l[1] += sl[1] + c2;
+    } else if (D > 1) {
+      l[1] += sl[1];
+    }
+    if (D > 1) {
+      const int c1 = l[1] >= E1;
+      l[1] -= c1 ? E1 : 0;
+      l[0] += sl[0] + c1;
+    } else {
+      l[0] += sl[0];
+    }
   }
 }
 
