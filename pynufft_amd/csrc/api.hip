@@ -1468,7 +1468,7 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
         nbk[i] = (lines + f.lines_per_block - 1) / f.lines_per_block;
         lds = std::max(lds, pnh_fft_lds_bytes(f, kind == BFFT_INV64));
         for (int q = 0; q < f.npass; ++q)
-          pow2 &= (f.radix[q] == 2 || f.radix[q] == 4 || f.radix[q] == 8);
+          pow2 &= (f.radix[q] == 2 || f.radix[q] == 4 || f.radix[q] == 8 || f.radix[q] == 16);
       }
       add_table(blob, nbk, &of_tab[kind][a], &of_first[kind][a], &b->fft_t[kind][a].n_blocks);
       b->fft_lds[kind][a] = lds;

@@ -453,7 +453,10 @@ fft_batch_kernel(const BatchProblem* __restrict__ probs, BatchTable t, int axis,
   const int64_t lb = (int64_t)blockIdx.x - t.first[p];
   extern __shared__ unsigned char smem_raw[];
   const BatchProblem& pr = probs[p];
-  const FftDesc d = kind == BFFT_FWD ? pr.fwd[axis] : (kind == BFFT_INV ? pr.inv[axis] : pr.inv64[axis]);
+  // a reference into the problem table (uniform loads); a by-value copy
+  // selected at run time went to per-thread scratch memory
+  const FftDesc& d = kind == BFFT_FWD ? pr.fwd[axis]
+                                      : (kind == BFFT_INV ? pr.inv[axis] : pr.inv64[axis]);
   const void* tw = kind == BFFT_INV64 ? pr.tw64[axis] : pr.tw32[axis];
   const C* in = (const C*)in_ + (in_image ? pr.x_off : pr.k_off);
   C* out = (C*)out_ + (out_image ? pr.x_off : pr.k_off);
