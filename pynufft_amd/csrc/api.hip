@@ -1331,6 +1331,8 @@ int pnh_solve_cg_noalloc(const pnh_plan* p, const void* y, void* x, int maxiter,
 struct pnh_batch {
   int n, D, J_last, J_max, grid_threads;
   int packed;                          // every problem has per-item bounds
+  int fixed_pad;                       // every problem's boxes have extent PNH_FIXED_PAD
+                                       // and every problem a row bound
   int J[PNH_MAXD];
   int64_t N_total, M_total, K_total, part_elems, pad_cells_max;
   int fft_threads;
@@ -1413,10 +1415,12 @@ int build_batch(const pnh_plan_desc* descs, int n, void* table, hipStream_t stre
   b->pad_cells_max = 1;
   b->grid_threads = 64;
   b->packed = 1;
+  b->fixed_pad = 1;
   for (int i = 0; i < n; ++i) {
     b->pad_cells_max = std::max(b->pad_cells_max, plans[i].bins.pad_cells);
     b->grid_threads = std::max(b->grid_threads, plans[i].bins.threads);
     if (!plans[i].bins.item_bound && plans[i].bins.n_items > 0) b->packed = 0;
+    if (!pnh_fixed_pad(plans[i].bins, D) || !(plans[i].bins.row_bound > 0.0)) b->fixed_pad = 0;
   }
   b->fft_threads = 256;
   b->vec_chunk = kVecChunk;
@@ -1613,8 +1617,8 @@ int batch_solve(const pnh_batch* b, const void* y, void* x, int maxiter, double 
     if ((rc = pnh_batch_vec_launch(BVEC_P_UPDATE, b->probs, gt, ch, w.p, w.r, nullptr,
                                    nullptr, cg, it, s))) return rc;
     if ((rc = pnh_batch_k2y2k_launch(b->probs, b->items, b->n_items, b->gather_t, b->D, b->J,
-                                     1, b->pad_cells_max, b->grid_threads, w.p, w.yt, w.q,
-                                     w.part, done, nullptr, s))) return rc;
+                                     1, b->pad_cells_max, b->grid_threads, b->fixed_pad, w.p,
+                                     w.yt, w.q, w.part, done, nullptr, s))) return rc;
     if ((rc = pnh_batch_vec_launch(BVEC_DOT_PQ, b->probs, gt, ch, w.p, w.q, nullptr,
                                    nullptr, cg, it, s))) return rc;
     if ((rc = pnh_batch_vec_launch(BVEC_XR_UPDATE, b->probs, gt, ch, w.x, w.r, w.p,

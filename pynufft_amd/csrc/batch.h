@@ -58,10 +58,13 @@ int pnh_batch_k2y_launch(const BatchProblem* probs, const int2* items, int64_t n
 // (one block per tile, by table `gather`).
 // k_out = P^H P k for every problem (fused k2y + y2k per work item, then the
 // gather); y receives P k (in row order within each problem).
+// fixed_pad: every problem's work list passes pnh_fixed_pad (boxes of the
+// compile-time extent PNH_FIXED_PAD) and has a row bound.
 int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
                            const BatchTable& gather, int D, const int* Jd, int fp64,
-                           int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
-                           void* part, const int* done, const int* all_done, hipStream_t s);
+                           int64_t pad_cells, int threads, int fixed_pad, const void* k, void* y,
+                           void* k_out, void* part, const int* done, const int* all_done,
+                           hipStream_t s);
 // packed: every problem has per-item bounds (complex64 packed accumulators).
 int pnh_batch_y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
                          const BatchTable& gather, int D, const int* Jd, int fp64, int packed,

@@ -56,6 +56,20 @@ __host__ __device__ inline int pnh_gather_parts(const BinArgs& b) {
   return (cells + 255) / 256;
 }
 
+// Padded-box extent every split axis has with 16-cell tiles and a support
+// of 6.  The gridding kernel and the one-pass FP64 P^H P have variants with
+// it as a compile-time constant (box and accumulator offsets become
+// instruction immediates, fewer registers).
+constexpr int PNH_FIXED_PAD = 21;
+// Whether the work list's every axis is split with padded extent
+// PNH_FIXED_PAD (the box never wraps).
+__host__ __device__ inline bool pnh_fixed_pad(const BinArgs& b, int D) {
+  if (D < 2) return false;
+  for (int d = 0; d < D; ++d)
+    if (!(b.tgrid[d] > 1 && b.pad[d] == PNH_FIXED_PAD)) return false;
+  return true;
+}
+
 // Lanes per sample used by k2y (and by the CSR table layout): one lane per
 // support offset on the last axis, rounded to a divisor-friendly width.
 __host__ __device__ inline int pnh_lanes_per_sample(int J_last) {

@@ -328,13 +328,10 @@ __device__ __forceinline__ int box_local(int x, int wrap, int K) {
   return FP ? x : wrap_local(x, wrap, K);
 }
 
-// Padded-box extent every split axis has with 16-cell tiles and a support
-// of 6: the gridding kernel has a variant with it as a compile-time constant
-// (accumulator offsets become instruction immediates; 3-D 128^3 tile kernel
-// 240 -> 211 us).  The interpolation kernel keeps run-time extents: its
-// variant needed 66 registers, above the 64 that let two 1 024-thread
-// workgroups share a CU, and ran 43 % slower.
-constexpr int kFixedPad = 21;
+// The interpolation kernel keeps run-time box extents: its variant with
+// PNH_FIXED_PAD needed 66 registers, above the 64 that let two 1 024-thread
+// workgroups share a CU, and ran 43 % slower (the gridding kernel's variant:
+// 3-D 128^3 tile kernel 240 -> 215 us).
 
 // q = i / n and r = i % n for 0 <= i < 2^22 and 1 <= n, with rcp = 1.0f / n
 // (the float estimate is off by at most one; one correction step fixes it)
@@ -1075,15 +1072,13 @@ void allow_lds(K kernel, size_t bytes) {
                               (int)bytes);
 }
 
-// Calls f(std::integral_constant<int, PADC>): kFixedPad when every axis is
-// split with that padded extent (a support of exactly 6, factor format, 2-D
-// or 3-D), else 0 (extents read at run time).
+// Calls f(std::integral_constant<int, PADC>): PNH_FIXED_PAD when every axis
+// is split with that padded extent (a support of exactly 6, factor format,
+// 2-D or 3-D), else 0 (extents read at run time).
 template <int D, bool CSRV, int JE, typename F>
 int dispatch_pad(const BinArgs& b, F&& f) {
   if constexpr (!CSRV && JE == 6 && D >= 2) {
-    bool fixed = true;
-    for (int d = 0; d < D; ++d) fixed = fixed && b.tgrid[d] > 1 && b.pad[d] == kFixedPad;
-    if (fixed) return f(std::integral_constant<int, kFixedPad>());
+    if (pnh_fixed_pad(b, D)) return f(std::integral_constant<int, PNH_FIXED_PAD>());
   }
   return f(std::integral_constant<int, 0>());
 }
@@ -1296,7 +1291,7 @@ __device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b
 // |Re k| + |Im k| of the box.  A non-finite sample poisons the item's slot,
 // as in the two-phase pass.  LDS: box [pad_cells] complex128, accumulator
 // [pad_cells][2] int64, then 16 doubles of reduction scratch and a flag.
-template <int D, int JN, int JE>
+template <int D, int JN, int JE, int PADC = 0>
 __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs& b,
                                               const int4 it, long long* smem,
                                               const void* __restrict__ k_,
@@ -1309,8 +1304,9 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
   double* red = reinterpret_cast<double*>(acc + 2 * (int64_t)pc);
   int* poisoned = reinterpret_cast<int*>(red + 16);
   const TileFrame f = tile_frame<D>(a, b, it.x);
-  const int P1 = D > 1 ? b.pad[1] : 1;
-  const int P2 = D > 2 ? b.pad[2] : 1;
+  constexpr bool FP = PADC > 0;          // every axis split, extent PADC
+  const int P1 = D > 1 ? (FP ? PADC : b.pad[1]) : 1;
+  const int P2 = D > 2 ? (FP ? PADC : b.pad[2]) : 1;
   const C* __restrict__ grid = (const C*)k_;
   // padded box -> LDS (last axis fastest), accumulator cleared, box maximum
   double kmax = 0.0;
@@ -1373,7 +1369,7 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
     C u0c[JN], u1c[JN];
     int row0[JN], off1[JN];
     if (valid) {
-      xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl, f.wrap[D - 1],
+      xl = box_local<FP>(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl, f.wrap[D - 1],
                       a.K[D - 1]);
       ul = u[a.u_off[D - 1] + (int64_t)r * JL + jl];
       if (D == 1) {
@@ -1385,7 +1381,7 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
         for (int j = 0; j < JN; ++j)
           if (JE > 0 || j < J0) {
             u0c[j] = u0[j];
-            row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+            row0[j] = box_local<FP>(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
           }
         C s0 = make_double2(0.0, 0.0);
         if (D == 2) {
@@ -1399,7 +1395,7 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
           for (int j = 0; j < JN; ++j)
             if (JE > 0 || j < J1) {
               u1c[j] = u1[j];
-              off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+              off1[j] = box_local<FP>(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
             }
 #pragma unroll
           for (int j0 = 0; j0 < JN; ++j0) {
@@ -1465,14 +1461,14 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
                      : make_double2((double)acc[2 * i] * unscale, (double)acc[2 * i + 1] * unscale);
 }
 
-template <int D, int JN, int JE>
+template <int D, int JN, int JE, int PADC>
 __global__ void __launch_bounds__(1024)
 k2y2k_1p_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ k_,
                      void* __restrict__ part_, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ long long smem_kk1[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  k2y2k_item_1p<D, JN, JE>(a, b, it, smem_kk1, k_, part_);
+  k2y2k_item_1p<D, JN, JE, PADC>(a, b, it, smem_kk1, k_, part_);
 }
 
 template <typename T, int D, int JN, int JE, bool PK>
@@ -1495,12 +1491,15 @@ int launch_k2y2k(const InterpArgs& a, const BinArgs& b, const void* k, void* y, 
     // FP64 with a row bound: one pass over each item's rows
     if (b.n_items > 0) {
       int rc = dispatch_j<D, false>(a, [&](auto jn, auto je) {
-        auto kern = k2y2k_1p_tile_kernel<D, decltype(jn)::value, decltype(je)::value>;
-        allow_lds(kern, lds1);
-        hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds1, s, a, b, k,
-                           slots, skip);
-        PNH_LAUNCH_CHECK();
-        return 0;
+        return dispatch_pad<D, false, decltype(je)::value>(b, [&](auto pad) {
+          auto kern = k2y2k_1p_tile_kernel<D, decltype(jn)::value, decltype(je)::value,
+                                           decltype(pad)::value>;
+          allow_lds(kern, lds1);
+          hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.threads), lds1, s, a, b,
+                             k, slots, skip);
+          PNH_LAUNCH_CHECK();
+          return 0;
+        });
       });
       if (rc) return rc;
     }
@@ -1788,8 +1787,9 @@ y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __rest
 
 // P^H P of every problem's work items (k2y then y2k per item, as k2y2k_item;
 // FP64 problems with a row bound in one pass, as k2y2k_item_1p, when
-// one_pass: the launch's LDS holds that pass's box and accumulator).
-template <typename T, int D, int JN, int JE>
+// one_pass: the launch's LDS holds that pass's box and accumulator).  PADC:
+// every problem's boxes have the extent PNH_FIXED_PAD on every axis.
+template <typename T, int D, int JN, int JE, int PADC>
 __global__ void __launch_bounds__(1024)
 k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __restrict__ items,
                         const void* __restrict__ k_, void* __restrict__ y_,
@@ -1805,14 +1805,18 @@ k2y2k_tile_batch_kernel(const BatchProblem* __restrict__ probs, const int2* __re
   const BinArgs b = probs[p].bins;
   const int4 it = reinterpret_cast<const int4*>(b.items)[pi.y];
   if constexpr (sizeof(T) == 8) {
-    if (one_pass && b.row_bound > 0.0) {  // uniform over the workgroup
-      k2y2k_item_1p<D, JN, JE>(a, b, it, smem_kkb, (const C*)k_ + probs[p].k_off,
-                               (C*)part_ + probs[p].part_off);
+    // PADC > 0: every problem has a row bound (the launch checked), so the
+    // kernel holds the one-pass variant only (fewer registers)
+    if (PADC > 0 || (one_pass && b.row_bound > 0.0)) {  // uniform over the workgroup
+      k2y2k_item_1p<D, JN, JE, PADC>(a, b, it, smem_kkb, (const C*)k_ + probs[p].k_off,
+                                     (C*)part_ + probs[p].part_off);
       return;
     }
   }
-  k2y2k_item<T, D, JN, JE, false>(a, b, it, pi.y, smem_kkb, (const C*)k_ + probs[p].k_off,
-                                  (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
+  if constexpr (PADC == 0) {
+    k2y2k_item<T, D, JN, JE, false>(a, b, it, pi.y, smem_kkb, (const C*)k_ + probs[p].k_off,
+                                    (C*)y_ + probs[p].y_off, (C*)part_ + probs[p].part_off);
+  }
 }
 
 // Gather of every problem's grid: pnh_gather_parts blocks per tile, blocks by table.
@@ -1904,22 +1908,30 @@ namespace {
 template <typename T, int D>
 int batch_k2y2k_t(const BatchProblem* probs, const int2* items, int64_t n_items, const int* Jd,
                   size_t lds, int threads, const void* k, void* y, void* part, const int* done,
-                  const int* all_done, int one_pass, hipStream_t s) {
+                  const int* all_done, int one_pass, int fixed_pad, hipStream_t s) {
   return dispatch_j<D, false>(j_only(Jd, D), [&](auto jn, auto je) {
-    auto kern = k2y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, k, y,
-                       part, done, all_done, one_pass);
-    PNH_LAUNCH_CHECK();
-    return 0;
+    auto go = [&](auto pad) {
+      auto kern = k2y2k_tile_batch_kernel<T, D, decltype(jn)::value, decltype(je)::value,
+                                          decltype(pad)::value>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((unsigned)n_items), dim3(threads), lds, s, probs, items, k,
+                         y, part, done, all_done, one_pass);
+      PNH_LAUNCH_CHECK();
+      return 0;
+    };
+    if constexpr (sizeof(T) == 8 && decltype(je)::value == 6 && D >= 2) {
+      if (fixed_pad && one_pass) return go(std::integral_constant<int, PNH_FIXED_PAD>());
+    }
+    return go(std::integral_constant<int, 0>());
   });
 }
 }  // namespace
 
 int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t n_items,
                            const BatchTable& gather, int D, const int* Jd, int fp64,
-                           int64_t pad_cells, int threads, const void* k, void* y, void* k_out,
-                           void* part, const int* done, const int* all_done, hipStream_t s) {
+                           int64_t pad_cells, int threads, int fixed_pad, const void* k, void* y,
+                           void* k_out, void* part, const int* done, const int* all_done,
+                           hipStream_t s) {
   size_t lds = y2k_lds_bytes(pad_cells, 1, false);
   if (lds > pnh_lds_limit()) {
     pnh_set_error("gridding tile needs %zu bytes of LDS (device limit %zu)", lds,
@@ -1929,7 +1941,7 @@ int pnh_batch_k2y2k_launch(const BatchProblem* probs, const int2* items, int64_t
   const int one_pass = fp64 && k2y2k_1p_lds_bytes(pad_cells) <= pnh_lds_limit();
   if (one_pass) lds = k2y2k_1p_lds_bytes(pad_cells);
   int rc = 0;
-#define KK(T, DD) rc = batch_k2y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, part, done, all_done, one_pass, s)
+#define KK(T, DD) rc = batch_k2y2k_t<T, DD>(probs, items, n_items, Jd, lds, threads, k, y, part, done, all_done, one_pass, fixed_pad, s)
   if (n_items > 0) {
     if (fp64) { if (D == 1) KK(double, 1); else if (D == 2) KK(double, 2); else KK(double, 3); }
     else { if (D == 1) KK(float, 1); else if (D == 2) KK(float, 2); else KK(float, 3); }
