@@ -333,14 +333,36 @@ __device__ __forceinline__ int box_local(int x, int wrap, int K) {
 // workgroups share a CU, and ran 43 % slower (the gridding kernel's variant:
 // 3-D 128^3 tile kernel 240 -> 215 us).
 
-// q = i / n and r = i % n for 0 <= i < 2^22 and 1 <= n, with rcp = 1.0f / n
-// (the float estimate is off by at most one; one correction step fixes it)
-__device__ __forceinline__ int fdivmod(int i, int n, float rcp, int& r) {
-  int q = (int)((float)i * rcp);
-  r = i - q * n;
-  if (r < 0) { --q; r += n; }
-  else if (r >= n) { ++q; r -= n; }
-  return q;
+// Walks the cells i = first, first + step, ... < n of a padded box of
+// extents (P0,) P1, P2 in C order, calling f(i, x0, x1, x2): the coordinates
+// advance by the step's own decomposition with carries (each stays below
+// twice its extent, so one subtraction suffices), with no division per cell.
+template <int D, typename F>
+__device__ __forceinline__ void for_box_cells(int n, int P1, int P2, int first, int step, F&& f) {
+  int x2 = 0, x1 = 0, x0 = first;
+  if (D > 2) { x2 = x0 % P2; x0 /= P2; }
+  if (D > 1) { x1 = x0 % P1; x0 /= P1; }
+  int s2 = 0, s1 = 0, s0 = step;
+  if (D > 2) { s2 = s0 % P2; s0 /= P2; }
+  if (D > 1) { s1 = s0 % P1; s0 /= P1; }
+  for (int i = first; i < n; i += step) {
+    f(i, x0, x1, x2);
+    if (D > 2) {
+      x2 += s2;
+      const int c = x2 >= P2;
+      x2 -= c ? P2 : 0;
+      x1 += s1 + c;
+    } else if (D > 1) {
+      x1 += s1;
+    }
+    if (D > 1) {
+      const int c = x1 >= P1;
+      x1 -= c ? P1 : 0;
+      x0 += s0 + c;
+    } else {
+      x0 += s0;
+    }
+  }
 }
 
 // Max over the workgroup; `red` holds one slot per wave.
@@ -666,27 +688,20 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
   const int P2 = D > 2 ? b.pad[2] : 1;
   const C* __restrict__ grid = (const C*)grid_;
   // padded box -> LDS (last axis fastest for coalescing)
-  {
-    const float r1 = 1.0f / (float)P1, r2 = 1.0f / (float)P2;
-    const int pc = (int)b.pad_cells;
-    for (int i = tid; i < pc; i += nthr) {
-      int x2 = 0, x1 = 0, x0 = i;
-      if (D > 2) x0 = fdivmod(x0, P2, r2, x2);
-      if (D > 1) x0 = fdivmod(x0, P1, r1, x1);
-      int c = f.T0[0] + x0;
-      int g = (c >= a.K[0] ? c - a.K[0] : c) * (int)a.Kstride[0];
-      if (D > 1) {
-        c = f.T0[1] + x1;
-        g += (c >= a.K[1] ? c - a.K[1] : c) * (int)a.Kstride[1];
-      }
-      if (D > 2) {
-        c = f.T0[2] + x2;
-        g += c >= a.K[2] ? c - a.K[2] : c;
-      }
-#pragma unroll
-      for (int cc = 0; cc < NBC; ++cc) box[i * NBC + cc] = grid[(int64_t)g * nb + c0 + cc];
+  for_box_cells<D>((int)b.pad_cells, P1, P2, tid, nthr, [&](int i, int x0, int x1, int x2) {
+    int c = f.T0[0] + x0;
+    int g = (c >= a.K[0] ? c - a.K[0] : c) * (int)a.Kstride[0];
+    if (D > 1) {
+      c = f.T0[1] + x1;
+      g += (c >= a.K[1] ? c - a.K[1] : c) * (int)a.Kstride[1];
     }
-  }
+    if (D > 2) {
+      c = f.T0[2] + x2;
+      g += c >= a.K[2] ? c - a.K[2] : c;
+    }
+#pragma unroll
+    for (int cc = 0; cc < NBC; ++cc) box[i * NBC + cc] = grid[(int64_t)g * nb + c0 + cc];
+  });
   __syncthreads();
 
   const int JL = a.J[D - 1];
@@ -1311,12 +1326,8 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
   // padded box -> LDS (last axis fastest), accumulator cleared, box maximum
   double kmax = 0.0;
   {
-    const float r1 = 1.0f / (float)P1, r2 = 1.0f / (float)P2;
     longlong2* acc2 = reinterpret_cast<longlong2*>(acc);
-    for (int i = tid; i < pc; i += nthr) {
-      int x2 = 0, x1 = 0, x0 = i;
-      if (D > 2) x0 = fdivmod(x0, P2, r2, x2);
-      if (D > 1) x0 = fdivmod(x0, P1, r1, x1);
+    for_box_cells<D>(pc, P1, P2, tid, nthr, [&](int i, int x0, int x1, int x2) {
       int c = f.T0[0] + x0;
       int g = (c >= a.K[0] ? c - a.K[0] : c) * (int)a.Kstride[0];
       if (D > 1) {
@@ -1333,7 +1344,7 @@ __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs
       // non-finite values are caught in the samples they reach
       const double m = fabs(v.x) + fabs(v.y);
       if (m <= 1.0e300) kmax = fmax(kmax, m);
-    }
+    });
   }
   if (tid == 0) *poisoned = 0;
   kmax = block_max(kmax, red);            // also the barrier after the box writes
