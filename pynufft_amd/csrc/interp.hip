@@ -674,7 +674,9 @@ y2k_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ y_,
 // the factors in registers; a segmented shuffle sums the lanes of a sample.
 // Nothing is accumulated across rows, so the result does not depend on
 // scheduling.  LDS: [pad_cells][NBC] of the operator's complex type.
-template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
+// NW: every axis of the work list is split, so the padded box never wraps
+// (no wrap tests on the box coordinates).
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool NW = false>
 __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, const int4 it,
                                          typename cplx<T>::t* __restrict__ box,
                                          const void* __restrict__ grid_,
@@ -723,7 +725,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
 #pragma unroll
     for (int c = 0; c < NBC; ++c) acc[c] = mk<T>((T)0, (T)0);
     if (valid) {
-      const int xl = wrap_local(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
+      const int xl = box_local<NW>(a.base[(int64_t)(D - 1) * a.M + r] - f.T0[D - 1] + jl,
                                 f.wrap[D - 1], a.K[D - 1]);
       if (CSRV) {
         const int G = D == 1 ? 1 : (D == 2 ? J0 : J0 * J1);
@@ -735,10 +737,10 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
         int j0 = 0, j1 = 0;
         for (int g = 0; g < G; ++g) {
           int idx = xl;
-          if (D == 2) idx += wrap_local(x0b + g, f.wrap[0], a.K[0]) * P1;
+          if (D == 2) idx += box_local<NW>(x0b + g, f.wrap[0], a.K[0]) * P1;
           if (D == 3)
-            idx += (wrap_local(x0b + j0, f.wrap[0], a.K[0]) * P1 +
-                    wrap_local(x1b + j1, f.wrap[1], a.K[1])) * P2;
+            idx += (box_local<NW>(x0b + j0, f.wrap[0], a.K[0]) * P1 +
+                    box_local<NW>(x1b + j1, f.wrap[1], a.K[1])) * P2;
           const C w = val[i0 + g * gs];
 #pragma unroll
           for (int c = 0; c < NBC; ++c) acc[c] = cfma(w, box[idx * NBC + c], acc[c]);
@@ -758,7 +760,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
           for (int j = 0; j < JN; ++j)
             if (JE > 0 || j < J0) {
               u0c[j] = u0[j];
-              row0[j] = wrap_local(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
+              row0[j] = box_local<NW>(x0b + j, f.wrap[0], a.K[0]) * P1 * P2;
             }
           if (D == 2) {
             C s0[NBC];
@@ -783,7 +785,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
             for (int j = 0; j < JN; ++j)
               if (JE > 0 || j < J1) {
                 u1c[j] = u1[j];
-                off1[j] = wrap_local(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
+                off1[j] = box_local<NW>(x1b + j, f.wrap[1], a.K[1]) * P2 + xl;
               }
             // acc = ul * sum_j0 u0[j0] * (sum_j1 u1[j1] * box[j0, j1]): one
             // complex multiply-add per support cell
@@ -835,7 +837,7 @@ __device__ __forceinline__ void k2y_item(const InterpArgs& a, const BinArgs& b, 
   }
 }
 
-template <typename T, int D, bool CSRV, int NBC, int JN, int JE>
+template <typename T, int D, bool CSRV, int NBC, int JN, int JE, bool NW>
 __global__ void __launch_bounds__(1024)
 k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
                 void* __restrict__ y_, int c0, const int* __restrict__ skip) {
@@ -843,7 +845,8 @@ k2y_tile_kernel(InterpArgs a, BinArgs b, const void* __restrict__ grid_,
   if (skip && *skip) return;
   extern __shared__ long long smem_box[];
   const int4 it = reinterpret_cast<const int4*>(b.items)[blockIdx.x];
-  k2y_item<T, D, CSRV, NBC, JN, JE>(a, b, it, reinterpret_cast<C*>(smem_box), grid_, y_, c0);
+  k2y_item<T, D, CSRV, NBC, JN, JE, NW>(a, b, it, reinterpret_cast<C*>(smem_box), grid_, y_,
+                                        c0);
 }
 
 // Grid cells from the partial slots that cover them, one workgroup per
@@ -1121,13 +1124,20 @@ int dispatch_j(const InterpArgs& a, F&& f) {
 template <typename T, int D, bool CSRV, int NBC>
 int launch_k2y_tile_j(const InterpArgs& a, const BinArgs& b, const void* grid, void* y, int c0,
                       size_t lds, const int* skip, hipStream_t s) {
+  bool split = true;                     // every axis split: the box never wraps
+  for (int d = 0; d < D; ++d) split = split && b.tgrid[d] > 1;
   return dispatch_j<D, CSRV>(a, [&](auto jn, auto je) {
-    auto kern = k2y_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.k2y_threads), lds, s, a, b, grid,
-                       y, c0, skip);
-    PNH_LAUNCH_CHECK();
-    return 0;
+    auto go = [&](auto nw) {
+      auto kern = k2y_tile_kernel<T, D, CSRV, NBC, decltype(jn)::value, decltype(je)::value,
+                                  decltype(nw)::value>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((unsigned)b.n_items), dim3(b.k2y_threads), lds, s, a, b,
+                         grid, y, c0, skip);
+      PNH_LAUNCH_CHECK();
+      return 0;
+    };
+    if (split) return go(std::integral_constant<bool, true>());
+    return go(std::integral_constant<bool, false>());
   });
 }
 
