@@ -938,9 +938,12 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
 #pragma unroll
     for (int k = 0; k < NBC; ++k) s[k] = mk<T>((T)0, (T)0);
     if (b.max_tile_slots == 1) {
-      // one slot per source at most: the loads of the sources that apply
-      // (on average 2.3 of 8 at 3-D 16^3) issued before any sum, so they
-      // overlap; the sum selects them in mask order
+      // one slot per source at most: the loads issued before any sum, so
+      // they overlap; the sum selects the sources that apply, in mask order.
+      // 3-D loads only those (on average 2.3 of 8 at 16^3); 1-D and 2-D
+      // load every source (the ones that do not apply read the slot array's
+      // first cell), which measured faster there
+      constexpr bool CL = D == 3;
       C v[1 << D][NBC];
       bool use[1 << D];
 #pragma unroll
@@ -952,7 +955,7 @@ __device__ __forceinline__ void gather_tile(const InterpArgs& a, const BinArgs& 
         use[mask] = ok;
         const C* p = part + (ok ? ((int64_t)src[mask].x * b.pad_cells + pc + offc[mask]) * NBC : 0);
 #pragma unroll
-        for (int cc = 0; cc < NBC; ++cc) v[mask][cc] = ok ? p[cc] : mk<T>((T)0, (T)0);
+        for (int cc = 0; cc < NBC; ++cc) v[mask][cc] = (!CL || ok) ? p[cc] : mk<T>((T)0, (T)0);
       }
 #pragma unroll
       for (int mask = 0; mask < (1 << D); ++mask)
