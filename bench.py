@@ -77,14 +77,42 @@ def lds_atomic_floor_ms(M, Jd, nb=1, per_coef=1):
     return 1e3 * nbytes / (LDS_BYTES_PER_CLK * N_CU * CLOCK_HZ)
 
 
+def _strip_comments(text):
+    """C/C++ source without its comments and with runs of whitespace
+    collapsed (string literals kept), so that comment edits leave the hash of
+    the code unchanged."""
+    out, i, n = [], 0, len(text)
+    while i < n:
+        c = text[i]
+        if c == '"' or c == "'":
+            j = i + 1
+            while j < n and text[j] != c:
+                j += 2 if text[j] == '\\' else 1
+            out.append(text[i:j + 1])
+            i = j + 1
+        elif text.startswith('//', i):
+            j = text.find('\n', i)
+            i = n if j < 0 else j
+        elif text.startswith('/*', i):
+            j = text.find('*/', i + 2)
+            i = n if j < 0 else j + 2
+            out.append(' ')
+        else:
+            out.append(c)
+            i += 1
+    return ' '.join(''.join(out).split())
+
+
 def source_hash():
-    """SHA-1 (16 hex digits) of the kernel sources; profiles record the hash
-    of the build they were measured on."""
+    """SHA-1 (16 hex digits) of the kernel sources' code (comments and
+    whitespace ignored); profiles record the hash of the build they were
+    measured on."""
     h = hashlib.sha1()
     csrc = os.path.join(HERE, 'pynufft_amd', 'csrc')
     for name in sorted(os.listdir(csrc)):
-        with open(os.path.join(csrc, name), 'rb') as f:
-            h.update(name.encode() + b'\0' + f.read())
+        with open(os.path.join(csrc, name), encoding='utf-8') as f:
+            code = _strip_comments(f.read())
+        h.update(name.encode() + b'\0' + code.encode())
     return h.hexdigest()[:16]
 
 

@@ -1316,9 +1316,12 @@ __device__ __forceinline__ void k2y2k_item(const InterpArgs& a, const BinArgs& b
 // so the samples never leave registers.  The fixed-point exponent is fixed
 // before the first add from a bound of every sample of the item:
 // |Re y_r| + |Im y_r| <= sqrt(2) |y_r| <= sqrt(2) row_bound m, m the largest
-// |Re k| + |Im k| of the box.  A non-finite sample poisons the item's slot,
-// as in the two-phase pass.  LDS: box [pad_cells] complex128, accumulator
+// |Re k| + |Im k| of the box; with 2^e > 1.5 row_bound u_bound m every
+// contribution's parts stay below 2^e, as the two-phase pass guarantees
+// from the exact samples.  A non-finite sample poisons the item's slot, as
+// in the two-phase pass.  LDS: box [pad_cells] complex128, accumulator
 // [pad_cells][2] int64, then 16 doubles of reduction scratch and a flag.
+// PADC > 0: every axis split with that padded extent (compile-time offsets).
 template <int D, int JN, int JE, int PADC = 0>
 __device__ __forceinline__ void k2y2k_item_1p(const InterpArgs& a, const BinArgs& b,
                                               const int4 it, long long* smem,
