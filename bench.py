@@ -177,8 +177,9 @@ def cpu_baseline(w, m_sample, seed=7):
 
 def bench_batched(torch, W, maxiter=20, reps=3):
     """Batched uneven 2-D CG (64 problems, gridding knobs autotuned on the
-    batched CG) vs looping the plain operator, each problem planned with the
-    defaults as a user of the reference would (one NUFFT per problem)."""
+    batched CG, the solve in BatchNUFFT's default stream groups) vs looping
+    the plain operator, each problem planned with the defaults as a user of
+    the reference would (one NUFFT per problem)."""
     from pynufft_amd.batch import BatchNUFFT
     from pynufft_amd.nufft import NUFFT
     ps = W.batched2d()
@@ -228,6 +229,7 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         'final_resid_median': float(np.median(res_batch)), 'plan_s': t_plan,
         'tuned': {'tile': B.tuned['tile'], 'chunk': B.tuned['chunk'],
                   'threads': B.tuned['threads']},
+        'stream_groups': max(1, len(B._groups)),
     }
 
 
