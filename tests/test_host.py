@@ -528,3 +528,56 @@ def test_plan_cache_round_trip(tmp_path, monkeypatch):
     files[0].write_bytes(b'not a plan')
     c = helper.plan(g['om'], Nd, Kd, Jd, cache_dir=str(tmp_path))
     assert np.array_equal(a['ud'][0], c['ud'][0])
+
+
+def test_source_hash_ignores_comments():
+    """The profiles' source hash covers the kernel sources' code: comments and
+    whitespace runs are dropped, string literals (which may hold comment
+    markers) are kept."""
+    import bench
+    code = 'int a = 1;  // note\n/* block\n comment */ int b;\nconst char* s = "// x /* y */";'
+    assert bench._strip_comments(code) == 'int a = 1; int b; const char* s = "// x /* y */";'
+    assert bench._strip_comments("char q = '\\'';  int c;") == "char q = '\\''; int c;"
+    assert len(bench.source_hash()) == 16
+
+
+def test_one_pass_exponent_bound():
+    """The one-pass FP64 P^H P fixes each work item's fixed-point exponent
+    before forming any sample: 2^e > 1.5 row_bound u_bound m, m the largest
+    |Re k| + |Im k| over the item's padded input box.  Every sample the item
+    forms then has |Re y| + |Im y| <= sqrt(2) row_bound m, and every
+    contribution conj(coef) y stays inside the 2^50 fixed-point range, on
+    random (white-noise) grids, the case that loosens the bound most."""
+    rng = np.random.default_rng(11)
+    for Nd, Kd, Jd, M in [((12, 10), (24, 20), (6, 6), 300), ((8, 6, 6), (16, 12, 12), (4, 5, 6), 200)]:
+        D = len(Nd)
+        om = rng.uniform(-np.pi, np.pi, (M, D))
+        st = helper.plan(om, Nd, Kd, Jd, chunk_rows=17)
+        k = rng.standard_normal(Kd) + 1j * rng.standard_normal(Kd)
+        tile = st['tile']
+        tgrid = [(Kd[a] + tile[a] - 1) // tile[a] for a in range(D)]
+        pad = helper.tile_pad(tile, Kd, Jd)
+        perm = st['perm']
+        base = st['base'][:, perm]
+        u = [st['ud'][a][perm] for a in range(D)]
+        worst = 0.0
+        for tid, first, cnt, slot in st['items']:
+            T0, wrap = tile_frame(int(tid), tile, tgrid, Kd)
+            box = np.array([k[tuple((T0[d] + x[d]) % Kd[d] for d in range(D))]
+                            for x in np.ndindex(*pad)])
+            m = np.max(np.abs(box.real) + np.abs(box.imag))
+            e = np.frexp(1.5 * st['row_bound'] * st['u_bound'] * m)[1]
+            for r in range(first, first + cnt):
+                coefs, y = [], 0.0
+                for j in np.ndindex(*Jd):
+                    c = [(base[d][r] + j[d]) % Kd[d] for d in range(D)]
+                    w = 1.0
+                    for d in range(D):
+                        w = w * u[d][r][j[d]]
+                    coefs.append(w)
+                    y = y + w * k[tuple(c)]
+                assert abs(y.real) + abs(y.imag) <= np.sqrt(2) * st['row_bound'] * m
+                for w in coefs:
+                    v = np.conj(w) * y
+                    worst = max(worst, abs(v.real) * 2.0 ** -e, abs(v.imag) * 2.0 ** -e)
+        assert worst < 1.0
