@@ -230,6 +230,7 @@ def bench_batched(torch, W, maxiter=20, reps=3):
         'tuned': {'tile': B.tuned['tile'], 'chunk': B.tuned['chunk'],
                   'threads': B.tuned['threads']},
         'stream_groups': max(1, len(B._groups)),
+        'stream_sets_ms': [round(v, 3) for v in B.tuned.get('stream_sets_ms', [])],
     }
 
 

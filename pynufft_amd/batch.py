@@ -49,7 +49,9 @@ class BatchNUFFT:
         splits the CG solves into that many problem groups of about equal
         work (at most one per problem), run concurrently on their own
         streams; on the 64-problem 2-D workload 1, 4, 8 and 16 groups gave
-        about 3 180, 3 420, 3 420-3 600 and 3 210 solves/s."""
+        about 3 180, 3 420, 3 420-3 600 and 3 210 solves/s.  With ``tune``
+        the groups' streams are also picked by timing a few stream sets
+        (``pynufft_amd.autotune.tune_streams``)."""
         self.release()
         probs = [(p['om'], tuple(p['Nd']), tuple(p['Kd']), tuple(p['Jd']))
                  if isinstance(p, dict) else
@@ -69,6 +71,9 @@ class BatchNUFFT:
             self.tuned = cfg
         self._build(probs, sts, threads)
         self._build_groups(int(streams))
+        if tune and self._groups:
+            from . import autotune
+            self.tuned['stream_sets_ms'] = autotune.tune_streams(self)
         return 0
 
     def _build(self, probs, sts, threads):
