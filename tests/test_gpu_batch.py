@@ -98,6 +98,9 @@ def test_autotuned_batch_matches_plain(hip, tmp_path, monkeypatch):
     probs = [(p['om'], p['Nd'], p['Kd'], p['Jd']) for p in ps]
     assert (B.tuned['tile'], B.tuned['chunk'], B.tuned['threads']) in \
         autotune.batch_candidates(probs)
+    # four stream sets timed for the problem groups (equal-work groups of
+    # the 5 problems, at most one per problem), the fastest kept
+    assert len(B.tuned['stream_sets_ms']) == 4 and 2 <= len(B._groups) <= 5
     xs = [torch.from_numpy(p['x']).to(B.device) for p in ps]
     y = B.forward(xs)
     xb = B.unpack_images(B.solve(y, maxiter=8, rtol=0.0))
