@@ -316,11 +316,15 @@ int plan_init(const pnh_plan_desc& d, pnh_plan* p) {
   p->bins.row_bound = d.row_bound;
   if (d.threads == 0) {
     // gridding: by the complex64 accumulator, one 64-bit word per cell when
-    // packed (two 512-thread workgroups share a CU at 16^3), else two;
-    // interpolation: one workgroup per CU on large boxes (3-D 16^3 k2y
+    // packed, else two.  Two packed boxes of 40-80 KiB share a CU's LDS
+    // with 768 threads each: the 3-D 16^3 packed kernel's 80 registers
+    // allow six waves per SIMD, which 2 x 768 threads fill (0.332 ms per
+    // y2k against 0.345 at 512 and 0.382 at 1024, same bits).
+    // Interpolation: one workgroup per CU on large boxes (3-D 16^3 k2y
     // measured 0.20 ms at 1024 threads, 0.26 ms at 512)
     const int64_t bytes = pad_cells * (d.item_bound ? 8 : 16);
-    p->bins.threads = bytes > 81920 ? 1024 : (bytes > 40960 ? 512 : 256);
+    const int mid = d.item_bound ? 768 : 512;
+    p->bins.threads = bytes > 81920 ? 1024 : (bytes > 40960 ? mid : 256);
     const int64_t kbytes = pad_cells * 16;
     p->bins.k2y_threads = kbytes > 81920 ? 1024 : (kbytes > 40960 ? 512 : 256);
   } else {
